@@ -1,0 +1,253 @@
+/*
+ * psphray_amd.h -- C ABI of the MI355X (gfx950) photon-packet tracer.
+ *
+ * Drop-in boundary for the per-tick hot path of P-SPHRAY: the octree build
+ * (rt_switch_epoch), ray traversal + per-ray distance sort (rt_trace + trace),
+ * photon deposition (deposit), active-set merge (merge_pars) and the
+ * per-particle chemistry update (update_pars / step_evolve).  The reference
+ * has no FFI layer; its internal C linkage is listed per entry point.
+ *
+ * Conventions
+ *   - every entry point returns PSR_OK (0) or a PSR_ERR_* code and never
+ *     signals (the reference raises SIGTRAP via ERROR, support/messages.h:17);
+ *     psr_last_error() gives the message;
+ *   - all pointers are host pointers borrowed for the duration of the call,
+ *     except where a name ends in _dev;
+ *   - a context owns its device memory and one HIP stream; one host thread
+ *     per context; several contexts may share a device.
+ *   - particle arrays are in the caller's order (index ipar, as PSYS(field,
+ *     ipar), src/psystem.h:30); the device keeps them in tree order and maps
+ *     back on download.
+ */
+#ifndef PSPHRAY_AMD_H
+#define PSPHRAY_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSR_OK 0
+#define PSR_ERR_ARG 1        /* bad argument / call order */
+#define PSR_ERR_HIP 2        /* HIP runtime error */
+#define PSR_ERR_CAPACITY 3   /* caller buffer too small; sizes returned */
+#define PSR_ERR_TREE 4       /* octree deeper than the key resolution */
+#define PSR_ERR_SOLVER 5     /* chemistry did not reach t = 1 */
+#define PSR_ERR_STATE 6      /* missing tables / tree / particles */
+#define PSR_ERR_NOTSUP 7     /* option outside this implementation */
+
+typedef struct psr_ctx psr_ctx;
+
+/* internal units and constants (src/units.c:66-104; src/config.c:43-51) */
+typedef struct psr_units {
+    double C_H, C_HMF, C_HEMF, C_1_CMH, C_1_GRAMH, C_1_SECH;
+    double U_CM, U_GRAM, U_SEC, U_M, U_KM, U_KPC, U_YR, U_MYR, U_KG, U_MSUN,
+           U_MPROTON, U_J, U_EV, U_ERG, U_RY_ENG;
+    double C_BOLTZMANN, C_PLANCK, C_SPEED_LIGHT, C_H_PER_MASS, C_HE_PER_MASS,
+           C_HI_ENERGY, C_HEI_ENERGY, C_HEII_ENERGY,
+           U_MPROTON_OVER_C_BOLTZMANN, C_BOLTZMANN_OVER_U_MPROTON;
+} psr_units;
+
+/* run flags: the CFG_* switches of src/config.c:22-42 that the hot path
+ * reads.  EOS (psphray.eos) is not supported: PSR_ERR_NOTSUP. */
+typedef struct psr_flags {
+    int32_t isothermal, adiabatic, on_the_spot, h_only, secondary_ionization,
+            trace_only, no_photon, comoving;
+    double redshift;
+    double fake_temperature;
+} psr_flags;
+
+/* host-built lookup tables, all in internal units */
+typedef struct psr_tables {
+    /* atomic rates (src/atomic-rate.c:253-278): row-major [16][ar_nrows]:
+     * logT, then HIci HIIrcA HIIrccA HIIrcB HIIrccB HeIci HeIIci HeIIrcA
+     * HeIIrcB HeIIIrcA HeIIIrcB HIcic HIcec Brem Compton */
+    int32_t ar_nrows;
+    double ar_min, ar_step_inv;
+    const double *ar;
+    /* cross sections (src/atomic-rate.c:280-318): [4][xs_nrows]: energy,
+     * sigma_HI, sigma_HeI, sigma_HeII; xs_cut = 13.6, 24.5, 54.4 eV */
+    int32_t xs_nrows;
+    double xs_min, xs_step_inv;
+    double xs_cut[3];
+    const double *xs;
+    /* SPH column kernel (src/sph-depth.c:3-33) */
+    int32_t kernel_n;
+    const double *kernel;
+    /* secondary ionization TabFun2 (src/secion.c:75-104): x[si_n1], y[si_n2],
+     * then PhiHI, PhiHeI, Eh each [si_n1][si_n2]; si_n1 == 0 disables */
+    int32_t si_n1, si_n2;
+    double si_min1, si_min2, si_step1_inv, si_step2_inv;
+    const double *si_x, *si_y, *si_phi_hi, *si_phi_hei, *si_eh;
+} psr_tables;
+
+/* particle system view (PSystem, src/psystem.h:32-78).  For uploads every
+ * pointer must be valid; for downloads a NULL pointer skips that field. */
+typedef struct psr_state {
+    int64_t npar;
+    float *pos; /* [npar][3] */
+    float *sml, *mass, *rho, *ie, *yeMET;
+    int8_t *flag;
+    double *lambdaH, *lambdaHeI, *lambdaHeII;
+    float *yGdepHI, *yGdepHeI, *yGdepHeII;
+    float *yGrecHII, *yGrecHeII, *yGrecHeIII;
+    float *heat;
+    int64_t *lasthit;
+    int32_t *hits;
+} psr_state;
+
+/* one ray packet (struct r_t, src/loop.c:21-35), 64 bytes.  field selects
+ * the particle field of a batched context (0 for a single field). */
+typedef struct psr_ray {
+    float s[3];
+    float dir[3];
+    double Nph;
+    double freq;
+    double length;
+    int32_t type;
+    int32_t field;
+    int64_t src_or_par;
+} psr_ray;
+
+/* chemistry step (Step, src/psystem.h:84-105) */
+typedef struct psr_step {
+    double lambdaH, lambdaHeI, lambdaHeII, ie, yeMET, yGdepHI, yGdepHeI,
+           yGdepHeII, heat, time, nH, T, yGrecHII, yGrecHeII, yGrecHeIII,
+           step_remain;
+    int32_t refined;
+    int32_t nsteps;
+    int64_t ipar;
+} psr_step;
+
+/* per-tick counters (stat_t subset, src/stat.h:1-56) */
+typedef struct psr_stats {
+    int64_t ticks;
+    int64_t total_deposit_count;      /* sum of full hit-list lengths */
+    int64_t saturated_deposit_count;
+    int64_t active_count;             /* particles updated */
+    int64_t solver_steps;
+    int64_t solver_failures;
+    int64_t surrogate_count;          /* updates taken from the surrogate */
+    int64_t surrogate_fallbacks;      /* surrogate rejected by the residual check */
+    double lost_photon_count_sum;
+    double first_ionization_HI, first_ionization_HeI, first_ionization_HeII;
+    double secondary_ionization_HI, secondary_ionization_HeI;
+} psr_stats;
+
+/* tick phases (src/loop.c:131-148) */
+#define PSR_PHASE_TRACE 1
+#define PSR_PHASE_DEPOSIT 2
+#define PSR_PHASE_UPDATE 4
+#define PSR_PHASE_ALL 7
+/* chemistry modes */
+#define PSR_CHEM_EXACT 0
+#define PSR_CHEM_SURROGATE 1
+
+/* ---- context ---- */
+int psr_create(psr_ctx **out, int device);
+void psr_destroy(psr_ctx *ctx);
+const char *psr_last_error(const psr_ctx *ctx);
+int psr_sync(psr_ctx *ctx);
+
+/* ---- host helpers (restating the reference's table set-up) ---- */
+/* units_init, src/units.c:66-104 */
+int psr_units_init(psr_units *u, double h, double hmf, double hemf, double cmh,
+                   double gramh, double sech);
+/* builds all tables: ar_init (src/atomic-rate.c:253-274) from an
+ * atomic_rates_*.txt file, xs_init (:280-311), the SPH column kernel
+ * (src/sph-depth.c) by quadrature, secion_init (src/secion.c:75-99) when
+ * with_secion.  The returned struct owns its arrays: psr_tables_free. */
+int psr_tables_build(psr_tables *t, const psr_units *u, const char *rates_path,
+                     int with_secion, char *err, size_t errlen);
+void psr_tables_free(psr_tables *t);
+/* host table lookups (for tests): ar_get, xs_get, sph_depth */
+double psr_tab_ar_get(const psr_tables *t, int col, double logT);
+double psr_tab_xs_get(const psr_tables *t, int id, double eng);
+double psr_tab_sph_depth(const psr_tables *t, double r_h);
+
+/* ---- physics set-up ---- */
+int psr_set_physics(psr_ctx *ctx, const psr_units *u, const psr_flags *f,
+                    const psr_tables *t);
+
+/* ---- epoch: tree + particles ----
+ * psr_tree_build replaces rt_switch_epoch (src/raytrace.c:41-151): it
+ * rebuilds, on the device, the reference octree (split at the 9th particle,
+ * float midpoints, leaf AABB = union of pos +- sml, internal AABB = union of
+ * the 8 children) for nfield independent fields laid end to end
+ * (field_offsets[nfield+1], boxsizes[nfield]).  It must precede
+ * psr_state_upload, whose arrays it reorders. */
+int psr_tree_build(psr_ctx *ctx, const float *pos, const float *sml,
+                   const int8_t *flag, int64_t npar, int32_t nfield,
+                   const int64_t *field_offsets, const double *boxsizes);
+/* tree summary: node count and depth; optional export of the node boxes in
+ * the device's level order (first_child -1 for leaves) */
+int psr_tree_info(psr_ctx *ctx, int64_t *nnodes, int32_t *depth,
+                  int64_t *skipped);
+int psr_tree_export(psr_ctx *ctx, int64_t cap, float *bot, float *top,
+                    int32_t *first_child, int32_t *npar, int32_t *level);
+int psr_state_upload(psr_ctx *ctx, const psr_state *s);
+int psr_state_download(psr_ctx *ctx, psr_state *s);
+
+/* ---- rays ---- */
+/* stores nray rays in the context's device ray pool (replaces the caller's
+ * r[] array, src/loop.c:69) */
+int psr_rays_upload(psr_ctx *ctx, const psr_ray *rays, int64_t nray);
+
+/* parity mode: rt_trace + qsort for each ray of the pool range
+ * [ray_first, ray_first + nray) (src/raytrace.c:153-210,
+ * src/loop.c:375-386).  Lists are sorted ascending by (d, ipar).  offsets
+ * (nray+1) is always filled; when offsets[nray] > cap the call returns
+ * PSR_ERR_CAPACITY without writing the lists. */
+int psr_trace_lists(psr_ctx *ctx, int64_t ray_first, int64_t nray,
+                    int64_t *offsets, int64_t cap, int64_t *ipar, double *d,
+                    double *b);
+
+/* one tick over the pool range: trace, deposit, merge, update
+ * (src/loop.c:131-148) restricted to the phases in the mask.  tick is the
+ * new psys.tick; tick_time the tick length (internal units). */
+int psr_tick(psr_ctx *ctx, int64_t ray_first, int64_t nray, int64_t tick,
+             double tick_time, int32_t phases, int32_t chem_mode);
+/* per-ray outputs of the last deposit: real ray length (r[i].length =
+ * x[j-1].d, src/loop.c:594) and photons left (TM) */
+int psr_ray_results(psr_ctx *ctx, int64_t nray, double *length, double *tm);
+int psr_get_stats(psr_ctx *ctx, psr_stats *st);
+int psr_reset_stats(psr_ctx *ctx);
+/* phase timers: HIP events on the context stream around trace, sort,
+ * deposit and update of every psr_tick while enabled; ms[5] = accumulated
+ * trace, sort, deposit, update and whole-tick milliseconds */
+int psr_set_timing(psr_ctx *ctx, int enable);
+int psr_get_timers(psr_ctx *ctx, double *ms);
+
+/* step_evolve on the device for n independent steps (src/solve.c:317-323),
+ * in place; rtol <= 0 selects the reference tolerances (src/solve.c:151) */
+int psr_solve_steps(psr_ctx *ctx, psr_step *steps, int64_t n, double rtol);
+
+/* ---- learned chemistry (MLP surrogate of step_evolve) ---- */
+#define PSR_MLP_IN 16
+#define PSR_MLP_HID 64
+#define PSR_MLP_OUT 4
+/* feature vector of a step (host-side definition shared with the device) */
+int psr_mlp_features(const psr_step *s, const psr_units *u, float *x /*16*/);
+/* train on n (features, targets) pairs on the device with the project's own
+ * BF16-MFMA forward/backward kernels; returns the final loss */
+int psr_mlp_train(psr_ctx *ctx, const float *x, const float *y, int64_t n,
+                  int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                  float *loss_out);
+int psr_mlp_forward(psr_ctx *ctx, const float *x, int64_t n, float *y);
+/* parameter access: W1[16][64] b1[64] W2[64][64] b2 W3[64][64] b3
+ * W4[64][4] b4, fp32 master copy; sizes in floats */
+int64_t psr_mlp_param_count(void);
+int psr_mlp_get_params(psr_ctx *ctx, float *p);
+int psr_mlp_set_params(psr_ctx *ctx, const float *p);
+/* gradient of the mean-squared loss at the current parameters on one batch
+ * (for the gradient check against the FP64 oracle) */
+int psr_mlp_grad(psr_ctx *ctx, const float *x, const float *y, int64_t n,
+                 float *grad, float *loss);
+/* residual threshold for the fallback to the exact solver (default 1e-4) */
+int psr_mlp_set_residual(psr_ctx *ctx, double tol);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

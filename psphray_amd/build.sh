@@ -1,0 +1,27 @@
+#!/bin/bash
+# Builds psphray_amd/libpsphray_amd.so for gfx950 (in-tree) and the CPU
+# oracle (oracle/liboracle.so).  Usage: psphray_amd/build.sh [-j N]
+set -euo pipefail
+HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
+ROOT="$(dirname "$HERE")"
+OUT="$HERE/_build"
+mkdir -p "$OUT"
+HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
+# -ffp-contract=off: the traversal reproduces separately rounded float
+#   products of the reference geometry; -packed-fp32-ops off: see DESIGN.md
+HIPFLAGS=(--offload-arch=gfx950 -O3 -std=c++17 -fPIC -ffp-contract=off -munsafe-fp-atomics
+          -Xclang -target-feature -Xclang -packed-fp32-ops
+          -I"$ROOT/include" -I"$HERE/csrc" -Wno-unused-result)
+pids=()
+for src in "$HERE"/csrc/*.hip; do
+  obj="$OUT/$(basename "${src%.hip}").o"
+  if [[ ! -f "$obj" || "$src" -nt "$obj" || "$HERE/csrc/psr_internal.h" -nt "$obj" || "$ROOT/include/psphray_amd.h" -nt "$obj" ]]; then
+    "$HIPCC" "${HIPFLAGS[@]}" -c "$src" -o "$obj" &
+    pids+=($!)
+  fi
+done
+gcc -std=gnu99 -O2 -fPIC -ffp-contract=off -I"$ROOT/include" -c "$HERE/host/psr_host.c" -o "$OUT/psr_host.o" &
+pids+=($!)
+for p in "${pids[@]}"; do wait "$p"; done
+"$HIPCC" --offload-arch=gfx950 -shared -fPIC -o "$HERE/libpsphray_amd.so" "$OUT"/*.o -lm
+make -s -C "$ROOT/oracle" all

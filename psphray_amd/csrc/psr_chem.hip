@@ -1,0 +1,472 @@
+// psr_chem.hip -- per-particle ionization / temperature update on gfx950
+// (update_pars, src/loop.c:647-776; step_evolve, src/solve.c:76-224).
+//
+// One thread per particle in FP64.  The right-hand side is the reference's
+// 7-variable system (src/solve.c:42-141); variables the run flags freeze
+// (zero derivative) are left out of the implicit solve, so the linear algebra
+// is NV x NV with NV = 1 for an isothermal hydrogen-only on-the-spot run and
+// NV = 4 for H+He with temperature.  The integrator is a two-stage L-stable
+// Rosenbrock method (Verwer et al. 1999) with an embedded first-order error
+// estimate and the reference tolerances (rtol 1e-4, atol 1e-7 / 1 for ie);
+// the Jacobian is a forward finite difference, as LSODA's.  The x_HI
+// variable and x_HII = 1 - x_HI are both kept in FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "psr_internal.h"
+
+namespace {
+
+enum {
+    AR_HI_CI = 0, AR_HII_RC_A, AR_HII_RCC_A, AR_HII_RC_B, AR_HII_RCC_B, AR_HEI_CI, AR_HEII_CI,
+    AR_HEII_RC_A, AR_HEII_RC_B, AR_HEIII_RC_A, AR_HEIII_RC_B, AR_HI_CIC, AR_HI_CEC, AR_E_BREMC,
+    AR_E_COMPC
+};
+
+struct Rates {
+    double gamma_HI, alpha_HII_A, alpha_HII_B, alpha_HII_AB, gamma_HeI, gamma_HeII, alpha_HeII_A,
+        alpha_HeII_B, alpha_HeII_AB, alpha_HeIII_A, alpha_HeIII_B, alpha_HeIII_AB, eta_HII, psi_HI,
+        zeta_HI, beta, chi;
+};
+
+// ar_get for every column at one logT (src/tabfun.c:38-58), times nH as
+// FETCH_VARS does (src/solve.c:56-72)
+template <bool OTS>
+__device__ __forceinline__ void rates_at(const DevTables &T, double logT, double nH, Rates &R) {
+    const int n = T.ar_nrows;
+    const long long idx = (long long)((logT - T.ar_min) * T.ar_step_inv);
+    double lw = 0, rw = 0;
+    int i0 = 0;
+    int mode;  // 0 clamp low, 1 interior, 2 clamp high
+    if (idx < 0) {
+        mode = 0;
+    } else if (idx >= n - 1) {
+        mode = 2;
+        i0 = n - 1;
+    } else {
+        mode = 1;
+        i0 = (int)idx;
+        lw = T.ar[i0 + 1] - logT;
+        rw = logT - T.ar[i0];
+    }
+    auto col = [&](int k) -> double {
+        const double *c = T.ar + (size_t)(k + 1) * n;
+        if (mode != 1) return c[i0];
+        return (lw * c[i0] + rw * c[i0 + 1]) * T.ar_step_inv;
+    };
+    R.gamma_HI = col(AR_HI_CI) * nH;
+    R.alpha_HII_A = col(AR_HII_RC_A) * nH;
+    R.alpha_HII_B = col(AR_HII_RC_B) * nH;
+    R.alpha_HII_AB = fdim(R.alpha_HII_A, R.alpha_HII_B);
+    R.gamma_HeI = col(AR_HEI_CI) * nH;
+    R.gamma_HeII = col(AR_HEII_CI) * nH;
+    R.alpha_HeII_A = col(AR_HEII_RC_A) * nH;
+    R.alpha_HeII_B = col(AR_HEII_RC_B) * nH;
+    R.alpha_HeII_AB = fdim(R.alpha_HeII_A, R.alpha_HeII_B);
+    R.alpha_HeIII_A = col(AR_HEIII_RC_A) * nH;
+    R.alpha_HeIII_B = col(AR_HEIII_RC_B) * nH;
+    R.alpha_HeIII_AB = fdim(R.alpha_HeIII_A, R.alpha_HeIII_B);
+    R.eta_HII = (OTS ? col(AR_HII_RCC_B) : col(AR_HII_RCC_A)) * nH;
+    R.psi_HI = col(AR_HI_CEC) * nH;
+    R.zeta_HI = col(AR_HI_CIC) * nH;
+    R.beta = col(AR_E_BREMC) * nH;
+    R.chi = col(AR_E_COMPC);
+}
+
+template <bool ISO, bool ADI, bool OTS, bool HONLY>
+struct Sys {
+    static constexpr int NV = 1 + (HONLY ? 0 : 2) + (OTS ? 0 : (HONLY ? 1 : 3)) + ((ADI || ISO) ? 0 : 1);
+    __device__ static constexpr int act(int k) {
+        // evolving variables in increasing index order
+        int idx[7] = {0, 0, 0, 0, 0, 0, 0};
+        int n = 0;
+        idx[n++] = 0;
+        if (!HONLY) { idx[n++] = 1; idx[n++] = 2; }
+        if (!OTS) {
+            idx[n++] = 3;
+            if (!HONLY) { idx[n++] = 4; idx[n++] = 5; }
+        }
+        if (!(ADI || ISO)) idx[n++] = 6;
+        return idx[k];
+    }
+};
+
+struct StepIn {
+    double yGdepHI, yGdepHeI, yGdepHeII, heat, time, nH, T, yeMET;
+};
+
+// RHS of src/solve.c:76-141, scaled by the step time
+template <bool ISO, bool ADI, bool OTS, bool HONLY>
+__device__ void rhs(const double x[7], double f[7], const StepIn &s, const DevTables &T,
+                    const DevPhys &P, const Rates &Riso) {
+    const double xHI = x[0];
+    const double xHII = 1.0 - x[0];
+    const double xHeI = x[1];
+    const double xHeII = x[2];
+    const double xHeIII = 1.0 - x[2] - x[1];
+    const double ye = (1.0 - x[0]) + P.C_HEMF / P.C_HMF * 0.25 * (x[2] + 2 * (1.0 - x[2] - x[1])) + s.yeMET;
+    Rates R;
+    if (ISO) {
+        R = Riso;
+    } else {
+        const double Tk = P.U_MPROTON_OVER_C_BOLTZMANN * x[6] / (P.C_HEMF * 0.25 + P.C_HMF + P.C_HMF * ye) * 2.0 / 3.0;
+        rates_at<OTS>(T, log10(Tk), s.nH, R);
+    }
+    if (!OTS) {
+        f[0] = -s.yGdepHI / s.time + (-R.gamma_HI * ye * xHI + R.alpha_HII_A * ye * xHII);
+        f[3] = R.alpha_HII_AB * ye * xHII;
+        if (HONLY) {
+            f[1] = f[2] = f[4] = f[5] = 0.0;
+        } else {
+            f[1] = -s.yGdepHeI / s.time + (-R.gamma_HeI * ye * xHeI + R.alpha_HeII_A * ye * xHeII);
+            f[2] = -s.yGdepHeII / s.time +
+                   (-R.gamma_HeII * ye * xHeII - R.alpha_HeII_A * ye * xHeII + R.alpha_HeIII_A * ye * xHeIII);
+            f[4] = R.alpha_HeII_AB * ye * xHeII;
+            f[5] = R.alpha_HeIII_AB * ye * xHeIII;
+        }
+    } else {
+        f[0] = -s.yGdepHI / s.time + (-R.gamma_HI * ye * xHI + R.alpha_HII_B * ye * xHII);
+        if (HONLY) {
+            f[1] = f[2] = 0.0;
+        } else {
+            // x_HI in the He lines, as src/solve.c:103-104
+            f[1] = -s.yGdepHeI / s.time + (-R.gamma_HeI * ye * xHI + R.alpha_HeII_B * ye * xHeII);
+            f[2] = -s.yGdepHeII / s.time +
+                   (-R.gamma_HeII * ye * xHI - R.alpha_HeII_B * ye * xHeII + R.alpha_HeIII_B * ye * xHeIII);
+        }
+        f[3] = f[4] = f[5] = 0;
+    }
+    if (ADI || ISO) {
+        f[6] = 0.0;
+    } else {
+        const double L = P.C_H_PER_MASS * ((R.zeta_HI + R.psi_HI) * ye * xHI + (R.eta_HII * ye * xHII) +
+                                           R.beta * ye * xHII + R.chi * ye);
+        f[6] = s.heat / s.time - L;
+    }
+#pragma unroll
+    for (int i = 0; i < 7; i++) f[i] *= s.time;
+}
+
+#define ROS2_GAMMA 1.7071067811865475
+#define ROS2_MAXSTEPS 100000
+
+// LU with partial pivoting (first strictly larger pivot, as the oracle);
+// every index is static after unrolling so the matrix stays in registers
+template <int N>
+__device__ __forceinline__ bool lu_factor(double a[N * N], int piv[N]) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        int p = k;
+        double m = fabs(a[k * N + k]);
+#pragma unroll
+        for (int i = k + 1; i < N; i++)
+            if (fabs(a[i * N + k]) > m) {
+                m = fabs(a[i * N + k]);
+                p = i;
+            }
+        piv[k] = p;
+        if (m == 0.0) return false;
+#pragma unroll
+        for (int q = k + 1; q < N; q++)
+            if (q == p) {
+#pragma unroll
+                for (int j = 0; j < N; j++) {
+                    const double t = a[k * N + j];
+                    a[k * N + j] = a[q * N + j];
+                    a[q * N + j] = t;
+                }
+            }
+        const double inv = 1.0 / a[k * N + k];
+#pragma unroll
+        for (int i = k + 1; i < N; i++) {
+            const double l = a[i * N + k] * inv;
+            a[i * N + k] = l;
+#pragma unroll
+            for (int j = k + 1; j < N; j++) a[i * N + j] -= l * a[k * N + j];
+        }
+    }
+    return true;
+}
+
+template <int N>
+__device__ __forceinline__ void lu_solve(const double a[N * N], const int piv[N], double b[N]) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+#pragma unroll
+        for (int q = k + 1; q < N; q++)
+            if (q == piv[k]) {
+                const double t = b[k];
+                b[k] = b[q];
+                b[q] = t;
+            }
+#pragma unroll
+        for (int i = k + 1; i < N; i++) b[i] -= a[i * N + k] * b[k];
+    }
+#pragma unroll
+    for (int k = N - 1; k >= 0; k--) {
+        double s = b[k];
+#pragma unroll
+        for (int j = k + 1; j < N; j++) s -= a[k * N + j] * b[j];
+        b[k] = s / a[k * N + k];
+    }
+}
+
+// integrate normalized time 0..1 (src/solve.c:170-224); returns steps taken
+// (negative when t = 1 was not reached)
+template <bool ISO, bool ADI, bool OTS, bool HONLY>
+__device__ int evolve(double x[7], const StepIn &s, const DevTables &T, const DevPhys &P, double rtol) {
+    using S = Sys<ISO, ADI, OTS, HONLY>;
+    constexpr int N = S::NV;
+    const double atol[7] = {1e-7, 1e-7, 1e-7, 1e-7, 1e-7, 1e-7, 1.};
+    Rates Riso{};
+    if (ISO) rates_at<OTS>(T, log10(s.T), s.nH, Riso);
+    double t = 0.0, h = 1.0;
+    int nsteps = 0;
+    bool rejected = false;
+    double f0[7], f1[7], fp[7], xt[7], J[N * N], M[N * N], k1[N], k2[N];
+    int piv[N];
+    while (t < 1.0 && nsteps < ROS2_MAXSTEPS) {
+        if (h > 1.0 - t) h = 1.0 - t;
+        rhs<ISO, ADI, OTS, HONLY>(x, f0, s, T, P, Riso);
+#pragma unroll
+        for (int jj = 0; jj < N; jj++) {
+            const int j = S::act(jj);
+#pragma unroll
+            for (int q = 0; q < 7; q++) xt[q] = x[q];
+            const double dl = 1.4901161193847656e-08 * fmax(fabs(x[j]), 1e-8);
+            xt[j] = x[j] + dl;
+            const double dlr = xt[j] - x[j];
+            rhs<ISO, ADI, OTS, HONLY>(xt, fp, s, T, P, Riso);
+#pragma unroll
+            for (int ii = 0; ii < N; ii++) J[ii * N + jj] = (fp[S::act(ii)] - f0[S::act(ii)]) / dlr;
+        }
+        for (;;) {
+            nsteps++;
+            const double gh = ROS2_GAMMA * h;
+#pragma unroll
+            for (int ii = 0; ii < N; ii++)
+#pragma unroll
+                for (int jj = 0; jj < N; jj++) M[ii * N + jj] = (ii == jj ? 1.0 : 0.0) - gh * J[ii * N + jj];
+            const bool ok = lu_factor<N>(M, piv);
+            double E = INFINITY;
+            double xn[7];
+            if (ok) {
+#pragma unroll
+                for (int ii = 0; ii < N; ii++) k1[ii] = f0[S::act(ii)];
+                lu_solve<N>(M, piv, k1);
+#pragma unroll
+                for (int q = 0; q < 7; q++) xt[q] = x[q];
+#pragma unroll
+                for (int ii = 0; ii < N; ii++) xt[S::act(ii)] = x[S::act(ii)] + h * k1[ii];
+                rhs<ISO, ADI, OTS, HONLY>(xt, f1, s, T, P, Riso);
+#pragma unroll
+                for (int ii = 0; ii < N; ii++) k2[ii] = f1[S::act(ii)] - 2.0 * k1[ii];
+                lu_solve<N>(M, piv, k2);
+#pragma unroll
+                for (int q = 0; q < 7; q++) xn[q] = x[q];
+                double acc = 0.0;
+#pragma unroll
+                for (int ii = 0; ii < N; ii++) {
+                    const int i = S::act(ii);
+                    xn[i] = x[i] + 1.5 * h * k1[ii] + 0.5 * h * k2[ii];
+                    const double e = 0.5 * h * (k1[ii] + k2[ii]);
+                    const double sc = atol[i] + rtol * fmax(fabs(x[i]), fabs(xn[i]));
+                    acc += (e / sc) * (e / sc);
+                }
+                E = sqrt(acc / N);
+                if (!(E == E)) E = INFINITY;
+            }
+            if (E <= 1.0 || h < 1e-14) {
+                t += h;
+#pragma unroll
+                for (int q = 0; q < 7; q++) x[q] = xn[q];
+                double fac = 0.9 / sqrt(fmax(E, 1e-10));
+                fac = fmin(5.0, fmax(0.2, fac));
+                if (rejected) fac = fmin(fac, 1.0);
+                rejected = false;
+                h *= fac;
+                if (1.0 - t < 1e-13) t = 1.0;
+                break;
+            }
+            rejected = true;
+            const double fac = (E == INFINITY) ? 0.2 : fmax(0.2, 0.9 / sqrt(E));
+            h *= fac;
+        }
+    }
+    // clamps of src/solve.c:208-213
+    x[0] = fmin(1., fmax(0., x[0]));
+    x[1] = fmin(1., fmax(0., x[1]));
+    x[2] = fmin(1. - x[1], fmax(0., x[2]));
+    x[3] = fmax(0., x[3]);
+    x[4] = fmax(0., x[4]);
+    x[5] = fmax(0., x[5]);
+    return t >= 1.0 ? nsteps : -nsteps;
+}
+
+__device__ __forceinline__ double ieye2T(const DevPhys &P, double ie, double ye) {
+    return P.U_MPROTON_OVER_C_BOLTZMANN * ie / (P.C_HEMF * 0.25 + P.C_HMF + P.C_HMF * ye) * 2.0 / 3.0;
+}
+__device__ __forceinline__ double Tye2ie(const DevPhys &P, double T, double ye) {
+    return T * P.C_BOLTZMANN_OVER_U_MPROTON * (P.C_HEMF * 0.25 + P.C_HMF + P.C_HMF * ye) / 2.0 * 3.0;
+}
+__device__ __forceinline__ double lambda_ye(const DevPhys &P, double lH, double lHeI, double lHeII) {
+    return (1.0 - lH) + P.C_HEMF / P.C_HMF * 0.25 * (lHeII + 2 * (1.0 - lHeII - lHeI));
+}
+
+template <bool ISO, bool ADI, bool OTS, bool HONLY>
+__global__ __launch_bounds__(256) void k_update(DevState S, int64_t npar, int64_t tick, double tick_time,
+                                                DevTables T, DevPhys P, DevStats *__restrict__ st) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    bool act = false;
+    int steps = 0, fail = 0;
+    if (i < npar && S.active[i]) {
+        act = true;
+        // Step set-up, src/loop.c:662-711
+        const double lH = S.lH[i], lHeI = S.lHeI[i], lHeII = S.lHeII[i];
+        const double rho = S.rho[i] * P.scaling_fac3_inv;
+        StepIn s;
+        s.time = (tick - S.lasthit[i]) * tick_time;
+        const double ie0 = S.ie[i];
+        s.T = P.fake_temperature > 0 ? P.fake_temperature : ieye2T(P, ie0, lambda_ye(P, lH, lHeI, lHeII));
+        s.yGdepHI = S.yGdepHI[i];
+        s.yGdepHeI = S.yGdepHeI[i];
+        s.yGdepHeII = S.yGdepHeII[i];
+        s.heat = S.heat[i];
+        s.yeMET = S.yeMET[i];
+        s.nH = P.C_H_PER_MASS * rho;
+        double x[7] = {lH, lHeI, lHeII, (double)S.yGrecHII[i], (double)S.yGrecHeII[i],
+                       (double)S.yGrecHeIII[i], ie0};
+        const int ns = evolve<ISO, ADI, OTS, HONLY>(x, s, T, P, 1e-4);
+        steps = ns < 0 ? -ns : ns;
+        fail = ns < 0;
+        // write-back, src/loop.c:745-766 (step_remain = 0 on this path)
+        S.yGdepHI[i] = 0.0f;
+        S.yGdepHeI[i] = 0.0f;
+        S.yGdepHeII[i] = 0.0f;
+        S.yGrecHII[i] = (float)x[3];
+        S.yGrecHeII[i] = (float)x[4];
+        S.yGrecHeIII[i] = (float)x[5];
+        const double nlH = fmax(0.0, fmin(1.0, x[0]));
+        const double nlHeI = fmax(0.0, fmin(1.0, x[1]));
+        const double nlHeII = fmax(0.0, fmin(1.0 - nlHeI, x[2]));
+        S.lH[i] = nlH;
+        S.lHeI[i] = nlHeI;
+        S.lHeII[i] = nlHeII;
+        float ie_new = ADI ? S.ie[i] : (float)x[6];
+        S.lasthit[i] = tick;
+        if (ISO) ie_new = (float)Tye2ie(P, s.T, lambda_ye(P, nlH, nlHeI, nlHeII));
+        S.ie[i] = ie_new;
+        S.heat[i] = 0.0f;
+        S.active[i] = 0;
+    }
+    // counters: one atomic per wavefront
+    const unsigned long long am = __ballot(act);
+    int sv = steps, fv = fail;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        sv += __shfl_xor(sv, o, 64);
+        fv += __shfl_xor(fv, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && am) {
+        atomicAdd(&st->active_count, (unsigned long long)__popcll(am));
+        atomicAdd(&st->solver_steps, (unsigned long long)sv);
+        if (fv) atomicAdd(&st->solver_failures, (unsigned long long)fv);
+    }
+}
+
+template <bool ISO, bool ADI, bool OTS, bool HONLY>
+__global__ __launch_bounds__(256) void k_solve(psr_step *__restrict__ steps, int64_t n, DevTables T, DevPhys P,
+                                               double rtol) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    psr_step st = steps[i];
+    StepIn s;
+    s.yGdepHI = st.yGdepHI;
+    s.yGdepHeI = st.yGdepHeI;
+    s.yGdepHeII = st.yGdepHeII;
+    s.heat = st.heat;
+    s.time = st.time;
+    s.nH = st.nH;
+    s.T = st.T;
+    s.yeMET = st.yeMET;
+    double x[7] = {st.lambdaH, st.lambdaHeI, st.lambdaHeII, st.yGrecHII, st.yGrecHeII, st.yGrecHeIII, st.ie};
+    const int ns = evolve<ISO, ADI, OTS, HONLY>(x, s, T, P, rtol);
+    st.lambdaH = x[0];
+    st.lambdaHeI = x[1];
+    st.lambdaHeII = x[2];
+    st.yGrecHII = x[3];
+    st.yGrecHeII = x[4];
+    st.yGrecHeIII = x[5];
+    st.ie = x[6];
+    st.step_remain = 0.0;
+    st.nsteps = ns;
+    steps[i] = st;
+}
+
+}  // namespace
+
+#define CHEM_DISPATCH(KERNEL, ...)                                                                       \
+    do {                                                                                                 \
+        const int key_ = (c->F.isothermal ? 8 : 0) | (c->F.adiabatic ? 4 : 0) |                          \
+                         (c->F.on_the_spot ? 2 : 0) | (c->F.h_only ? 1 : 0);                             \
+        switch (key_) {                                                                                  \
+        case 0: KERNEL<false, false, false, false><<<grid, 256, 0, c->stream>>>(__VA_ARGS__); break;     \
+        case 1: KERNEL<false, false, false, true><<<grid, 256, 0, c->stream>>>(__VA_ARGS__); break;      \
+        case 2: KERNEL<false, false, true, false><<<grid, 256, 0, c->stream>>>(__VA_ARGS__); break;      \
+        case 3: KERNEL<false, false, true, true><<<grid, 256, 0, c->stream>>>(__VA_ARGS__); break;       \
+        case 4: KERNEL<false, true, false, false><<<grid, 256, 0, c->stream>>>(__VA_ARGS__); break;      \
+        case 5: KERNEL<false, true, false, true><<<grid, 256, 0, c->stream>>>(__VA_ARGS__); break;       \
+        case 6: KERNEL<false, true, true, false><<<grid, 256, 0, c->stream>>>(__VA_ARGS__); break;       \
+        case 7: KERNEL<false, true, true, true><<<grid, 256, 0, c->stream>>>(__VA_ARGS__); break;        \
+        case 8: case 12: KERNEL<true, false, false, false><<<grid, 256, 0, c->stream>>>(__VA_ARGS__); break; \
+        case 9: case 13: KERNEL<true, false, false, true><<<grid, 256, 0, c->stream>>>(__VA_ARGS__); break;  \
+        case 10: case 14: KERNEL<true, false, true, false><<<grid, 256, 0, c->stream>>>(__VA_ARGS__); break; \
+        default: KERNEL<true, false, true, true><<<grid, 256, 0, c->stream>>>(__VA_ARGS__); break;           \
+        }                                                                                                \
+    } while (0)
+
+int psr_launch_update(psr_ctx *c, int64_t tick, double tick_time, int32_t) {
+    if (c->npar == 0) return PSR_OK;
+    const DevPhys P = psr_dev_phys(c);
+    const DevState S = c->state();
+    const unsigned grid = (unsigned)((c->npar + 255) / 256);
+    PSR_EV(c, 3, 0);
+    CHEM_DISPATCH(k_update, S, c->npar, tick, tick_time, c->T, P, c->d_stats);
+    PSR_HIP(c, hipGetLastError());
+    PSR_EV(c, 3, 1);
+    return PSR_OK;
+}
+
+int psr_solve_steps_dev(psr_ctx *c, psr_step *d_steps, int64_t n, double rtol) {
+    if (n == 0) return PSR_OK;
+    const DevPhys P = psr_dev_phys(c);
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    CHEM_DISPATCH(k_solve, d_steps, n, c->T, P, rtol);
+    PSR_HIP(c, hipGetLastError());
+    return PSR_OK;
+}
+
+extern "C" int psr_solve_steps(psr_ctx *c, psr_step *steps, int64_t n, double rtol) {
+    if (!c || (n > 0 && !steps) || n < 0) return PSR_ERR_ARG;
+    if (!c->have_physics) PSR_FAIL(c, PSR_ERR_STATE, "no physics: call psr_set_physics");
+    if (rtol <= 0) rtol = 1e-4;
+    PSR_HIP(c, hipSetDevice(c->device));
+    psr_step *d = nullptr;
+    PSR_HIP(c, hipMalloc(&d, sizeof(psr_step) * std::max<int64_t>(n, 1)));
+    hipError_t e = hipMemcpyAsync(d, steps, sizeof(psr_step) * n, hipMemcpyHostToDevice, c->stream);
+    int rc = PSR_OK;
+    if (e == hipSuccess) rc = psr_solve_steps_dev(c, d, n, rtol);
+    if (e == hipSuccess && rc == PSR_OK)
+        e = hipMemcpyAsync(steps, d, sizeof(psr_step) * n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) PSR_FAIL(c, PSR_ERR_HIP, std::string("psr_solve_steps: ") + hipGetErrorString(e));
+    if (rc) return rc;
+    for (int64_t i = 0; i < n; i++)
+        if (steps[i].nsteps < 0) {
+            c->err = "psr_solve_steps: a step did not reach t = 1";
+            return PSR_ERR_SOLVER;
+        }
+    return PSR_OK;
+}

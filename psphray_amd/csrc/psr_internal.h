@@ -1,0 +1,193 @@
+// psr_internal.h -- device-side data layout shared by the gfx950 kernels.
+//
+// HBM layout (all per context, particles in octree-key order):
+//   ps      float4[npar]   x, y, z, sml            (traversal: 16 B/particle)
+//   orig    uint32[npar]   caller index | PSR_INVALID_BIT (PF_INVALID)
+//   mass, rho, ie, yeMET          float[npar]
+//   lambdaH, lambdaHeI, lambdaHeII double[npar]  (x_HI, x_HeI, x_HeII)
+//   yGdepHI/HeI/HeII, yGrec*, heat float[npar]   (f32 atomics)
+//   lasthit int64[npar], hits int32[npar], active uint8[npar]
+//   nodes   Node[nnode]    32 B: float bot[3], top[3]; int a, b
+//           internal: a = first child (8 contiguous), b = -1
+//           leaf:     a = first particle, b = particle count (0..8)
+//   hit slots (per ray, capacity hit_cap): d f64, b f64, slot u32
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "psphray_amd.h"
+
+#define PSR_INVALID_BIT 0x80000000u
+#define PSR_MAXDEPTH 21  // octree levels held by a 63-bit key
+
+struct Node {
+    float bot[3];
+    float top[3];
+    int32_t a;
+    int32_t b;
+};
+static_assert(sizeof(Node) == 32, "Node must be 32 bytes");
+static_assert(sizeof(psr_ray) == 64, "psr_ray must be 64 bytes");
+
+// tables in device memory (layouts as psr_tables)
+struct DevTables {
+    const double *ar;
+    int32_t ar_nrows;
+    double ar_min, ar_step_inv;
+    const double *xs;
+    int32_t xs_nrows;
+    double xs_min, xs_step_inv;
+    double xs_cut[3];
+    const double *kernel;
+    int32_t kernel_n;
+    int32_t si_n1, si_n2;
+    double si_min1, si_min2, si_step1_inv, si_step2_inv;
+    const double *si_x, *si_y, *si_phi_hi, *si_phi_hei, *si_eh;
+};
+
+// constants the kernels need, passed by value
+struct DevPhys {
+    double C_H_PER_MASS, C_HE_PER_MASS, C_HI_ENERGY, C_HEI_ENERGY, C_HEII_ENERGY, C_HMF,
+        C_HEMF, U_MPROTON_OVER_C_BOLTZMANN, C_BOLTZMANN_OVER_U_MPROTON;
+    int32_t isothermal, adiabatic, on_the_spot, h_only, secondary_ionization, no_photon;
+    double scaling_fac2_inv, scaling_fac3_inv;
+    double fake_temperature;
+};
+
+// particle state pointers (tree order)
+struct DevState {
+    const float4 *ps;
+    const uint32_t *orig;
+    float *mass, *rho, *ie, *yeMET;
+    double *lH, *lHeI, *lHeII;
+    float *yGdepHI, *yGdepHeI, *yGdepHeII;
+    float *yGrecHII, *yGrecHeII, *yGrecHeIII;
+    float *heat;
+    int64_t *lasthit;
+    int32_t *hits;
+    uint8_t *active;
+};
+
+// hit slots
+struct DevHits {
+    double *d;
+    double *b;
+    uint32_t *slot;
+    int32_t *count;
+    int32_t cap;
+};
+
+// device counters (one struct, atomics)
+struct DevStats {
+    unsigned long long total_deposit_count;
+    unsigned long long saturated_deposit_count;
+    unsigned long long active_count;
+    unsigned long long solver_steps;
+    unsigned long long solver_failures;
+    unsigned long long surrogate_count;
+    unsigned long long surrogate_fallbacks;
+    double lost_photon_count_sum;
+    double first_ionization_HI, first_ionization_HeI, first_ionization_HeII;
+    double secondary_ionization_HI, secondary_ionization_HeI;
+    int32_t flags;  // PSR_DF_* below
+};
+#define PSR_DF_HITCAP 1
+#define PSR_DF_STACK 2
+#define PSR_DF_SORTCAP 4
+
+struct MlpState;
+
+struct psr_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int cu_count = 256;
+    // physics
+    bool have_physics = false;
+    psr_units U{};
+    psr_flags F{};
+    DevTables T{};
+    double *d_tables = nullptr;  // one allocation holding every table
+    // tree
+    bool have_tree = false;
+    int64_t npar = 0;
+    int32_t nfield = 0;
+    int64_t nnode = 0;
+    int32_t depth = 0;
+    int64_t skipped = 0;
+    Node *d_nodes = nullptr;
+    int32_t *d_node_level = nullptr;
+    int32_t *d_field_root = nullptr;
+    uint32_t *d_perm = nullptr;  // tree slot -> caller index
+    // particles
+    bool have_state = false;
+    float4 *d_ps = nullptr;
+    uint32_t *d_orig = nullptr;
+    float *d_f32 = nullptr;   // mass rho ie yeMET yGdep*3 yGrec*3 heat : 11 arrays
+    double *d_f64 = nullptr;  // lambdaH lambdaHeI lambdaHeII
+    int64_t *d_lasthit = nullptr;
+    int32_t *d_hits = nullptr;
+    uint8_t *d_active = nullptr;
+    // rays
+    psr_ray *d_rays = nullptr;
+    int64_t nray_pool = 0;
+    int64_t ray_cap = 0;
+    // hit slots
+    DevHits H{};
+    int64_t hit_slots = 0;  // number of rays the slot arrays hold
+    double *d_ray_len = nullptr, *d_ray_tm = nullptr;
+    int64_t last_nray = 0;
+    // stats and phase timers
+    DevStats *d_stats = nullptr;
+    bool timing = false;
+    // event pairs around the trace, sort, deposit and update kernels and the
+    // whole tick; ev_mask marks the pairs recorded in the current tick
+    hipEvent_t ev[10] = {};
+    int ev_mask = 0;
+    double t_ms[5] = {0, 0, 0, 0, 0};
+    int64_t ticks = 0;
+    // scratch
+    void *d_scratch = nullptr;
+    size_t scratch_bytes = 0;
+    // surrogate
+    MlpState *mlp = nullptr;
+    double mlp_residual = 1e-4;
+
+    DevState state() const;
+};
+
+// ---- launchers implemented in the .hip files ----
+int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray);
+int psr_launch_sort(psr_ctx *c, int64_t nray);
+int psr_launch_deposit(psr_ctx *c, int64_t ray_first, int64_t nray);
+int psr_launch_update(psr_ctx *c, int64_t tick, double tick_time, int32_t chem_mode);
+int psr_solve_steps_dev(psr_ctx *c, psr_step *d_steps, int64_t n, double rtol);
+DevPhys psr_dev_phys(const psr_ctx *c);
+int psr_ensure_scratch(psr_ctx *c, size_t bytes);
+int psr_mlp_update_dev(psr_ctx *c, int64_t tick, double tick_time);
+void psr_mlp_free(psr_ctx *c);
+
+#define PSR_HIP(c, call)                                                           \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            (c)->err = std::string(#call) + ": " + hipGetErrorString(e_);          \
+            return PSR_ERR_HIP;                                                    \
+        }                                                                          \
+    } while (0)
+
+// record event pair member (phase k, 0 = start / 1 = end) when timing
+#define PSR_EV(c, k, e)                                                          \
+    do {                                                                         \
+        if ((c)->timing) {                                                       \
+            PSR_HIP(c, hipEventRecord((c)->ev[2 * (k) + (e)], (c)->stream));     \
+            (c)->ev_mask |= 1 << (k);                                            \
+        }                                                                        \
+    } while (0)
+
+#define PSR_FAIL(c, code, msg)   \
+    do {                         \
+        (c)->err = (msg);        \
+        return (code);           \
+    } while (0)

@@ -1,0 +1,838 @@
+// psr_trace.hip -- per-tick hot path on gfx950: traversal, per-ray sort,
+// photon deposition (src/raytrace.c:153-230, src/loop.c:375-608).
+//
+// k_trace   one 256-thread workgroup per ray.  A LIFO stack in LDS holds
+//           the first-child index of hit internal nodes; each step expands
+//           up to 32 of them (32 x 8 children = 256 box tests), pushing hit
+//           internal children and queueing hit leaves; 32 queued leaves
+//           (x 8 particle slots) are tested per leaf step.  Every visited box
+//           gets the reference's float Pluecker test and every particle the
+//           reference's float/double impact test, so the hit set is the
+//           reference's exactly.  Hits go to the ray's slot in HBM.
+// k_sort    one workgroup per ray: bitonic sort of (d, ipar) in LDS, writes
+//           the sorted order as a permutation of the slot.
+// k_deposit one wavefront per ray: lanes gather and precompute 64 hits at a
+//           time (tau, expm1); the transmission recurrence TM -= absorb with
+//           the saturation clamp and the 1e-10 cut-off runs lane-serially
+//           over readlane'd values; lanes then scatter with f32 atomics.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "psr_internal.h"
+
+#define TR_THREADS 256
+#define TR_SCAP 4096
+#define TR_LCAP 320
+
+// ------------------------------------------------------------ geometry
+// segment/AABB test of src/pluecker-sse.c:28-198 (see the oracle for the
+// per-pair rule that covers its eight direction classes); float products,
+// compiled without contraction
+__device__ __forceinline__ bool box_hit(const float s[3], const float dir[3], float len,
+                                        const Node &nd) {
+    float s2b[3], s2t[3], e2b[3], e2t[3];
+    bool sg[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        s2b[i] = nd.bot[i] - s[i];
+        s2t[i] = nd.top[i] - s[i];
+        const float m = len * dir[i];
+        e2t[i] = s2t[i] - m;
+        e2b[i] = s2b[i] - m;
+        sg[i] = dir[i] > 0;
+    }
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        ok = ok && !(sg[i] ? (s2t[i] < 0) : (s2b[i] > 0));
+        ok = ok && !(sg[i] ? (e2b[i] > 0) : (e2t[i] < 0));
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int i = k == 2 ? 1 : 0, j = k == 0 ? 1 : 2;
+        const float A = sg[i] ? s2t[j] : s2b[j];
+        const float B = sg[j] ? s2b[i] : s2t[i];
+        const float C = sg[i] ? s2b[j] : s2t[j];
+        const float D = sg[j] ? s2t[i] : s2b[i];
+        const float r0 = dir[i] * A, r1 = dir[j] * B, r2 = dir[i] * C, r3 = dir[j] * D;
+        ok = ok && !(r0 < r1 || r2 > r3);
+    }
+    return ok;
+}
+
+__global__ __launch_bounds__(TR_THREADS) void k_trace(
+    const psr_ray *__restrict__ rays, int64_t nray, const Node *__restrict__ nodes,
+    const int32_t *__restrict__ froot, const float4 *__restrict__ ps,
+    const uint32_t *__restrict__ orig, DevHits H, uint8_t *__restrict__ active,
+    DevStats *__restrict__ st) {
+    __shared__ int32_t stack[TR_SCAP];
+    __shared__ int32_t leafq[TR_LCAP];
+    __shared__ int32_t s_sp, s_lq, s_nh, s_cs, s_cl, s_ovf;
+    const int tid = threadIdx.x;
+    for (int64_t r = blockIdx.x; r < nray; r += gridDim.x) {
+        const psr_ray R = rays[r];
+        const float s[3] = {R.s[0], R.s[1], R.s[2]};
+        const float dir[3] = {R.dir[0], R.dir[1], R.dir[2]};
+        const float len = (float)R.length;
+        const int64_t base = r * (int64_t)H.cap;
+        if (tid == 0) {
+            s_sp = 0;
+            s_lq = 0;
+            s_nh = 0;
+            s_cs = 0;
+            s_cl = 0;
+            s_ovf = 0;
+            const int root = froot[R.field];
+            const Node nd = nodes[root];
+            if (nd.b != 0 && box_hit(s, dir, len, nd)) {
+                if (nd.b < 0) stack[s_sp++] = nd.a;
+                else leafq[s_lq++] = root;
+            }
+        }
+        for (;;) {
+            __syncthreads();
+            const int sp = s_sp, lq = s_lq;
+            if (sp == 0 && lq == 0) break;
+            if (lq >= 32 || sp == 0) {
+                const int nl = min(lq, 32);
+                const int t = tid >> 3, slot = tid & 7;
+                const int leaf = t < nl ? leafq[lq - nl + t] : -1;
+                if (leaf >= 0) {
+                    const Node L = nodes[leaf];
+                    if (slot < L.b) {
+                        const int k = L.a + slot;
+                        const uint32_t o = orig[k];
+                        if (!(o & PSR_INVALID_BIT)) {
+                            // impact test of src/raytrace.c:171-186
+                            const float4 p = ps[k];
+                            const float dx = p.x - s[0], dy = p.y - s[1], dz = p.z - s[2];
+                            const float px = dx * dir[0], py = dy * dir[1], pz = dz * dir[2];
+                            const float qx = dx * dx, qy = dy * dy, qz = dz * dz;
+                            double proj = (double)px;
+                            proj += (double)py;
+                            proj += (double)pz;
+                            double d2 = (double)qx;
+                            d2 += (double)qy;
+                            d2 += (double)qz;
+                            const double dist = sqrt(d2);
+                            const double r2 = (dist - proj) * (dist + proj);
+                            const float hh = p.w * p.w;
+                            if (!((double)hh < r2)) {
+                                const int h = atomicAdd(&s_nh, 1);
+                                if (h < H.cap) {
+                                    H.d[base + h] = dist;
+                                    H.b[base + h] = sqrt(fabs(r2));
+                                    H.slot[base + h] = (uint32_t)k;
+                                }
+                                active[k] = 1;
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+                if (tid == 0) s_lq = lq - nl;
+            } else {
+                const int nn = min(sp, 32);
+                const int t = tid >> 3, c = tid & 7;
+                const int fc = t < nn ? stack[sp - nn + t] : -1;
+                __syncthreads();
+                if (fc >= 0) {
+                    const int ch = fc + c;
+                    const Node C = nodes[ch];
+                    if (C.b != 0 && box_hit(s, dir, len, C)) {
+                        if (C.b < 0) {
+                            const int p = sp - nn + atomicAdd(&s_cs, 1);
+                            if (p < TR_SCAP) stack[p] = C.a;
+                            else s_ovf = 1;
+                        } else {
+                            const int p = lq + atomicAdd(&s_cl, 1);
+                            leafq[p] = ch;
+                        }
+                    }
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    s_sp = min(sp - nn + s_cs, TR_SCAP);
+                    s_lq = lq + s_cl;
+                    s_cs = 0;
+                    s_cl = 0;
+                }
+            }
+        }
+        if (tid == 0) {
+            H.count[r] = s_nh;
+            if (s_nh > H.cap) atomicOr(&st->flags, PSR_DF_HITCAP);
+            if (s_ovf) atomicOr(&st->flags, PSR_DF_STACK);
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ sort
+template <int MAXN>
+__global__ __launch_bounds__(256) void k_sort(int64_t nray, DevHits H, const uint32_t *__restrict__ orig,
+                                              uint32_t *__restrict__ perm, int32_t lo, int32_t hi) {
+    __shared__ unsigned long long key[MAXN];
+    __shared__ uint32_t sec[MAXN];
+    __shared__ uint16_t pos[MAXN];
+    const int tid = threadIdx.x;
+    for (int64_t r = blockIdx.x; r < nray; r += gridDim.x) {
+        const int n = H.count[r];
+        if (n <= lo || n > hi) continue;
+        const int64_t base = r * (int64_t)H.cap;
+        int P = 1;
+        while (P < n) P <<= 1;
+        for (int i = tid; i < P; i += 256) {
+            if (i < n) {
+                key[i] = (unsigned long long)__double_as_longlong(H.d[base + i]);
+                sec[i] = orig[H.slot[base + i]] & ~PSR_INVALID_BIT;
+            } else {
+                key[i] = ~0ull;
+                sec[i] = ~0u;
+            }
+            pos[i] = (uint16_t)i;
+        }
+        __syncthreads();
+        for (int k = 2; k <= P; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = tid; i < P; i += 256) {
+                    const int ixj = i ^ j;
+                    if (ixj > i) {
+                        const unsigned long long ka = key[i], kb = key[ixj];
+                        const uint32_t sa = sec[i], sb = sec[ixj];
+                        const bool gt = ka > kb || (ka == kb && sa > sb);
+                        const bool up = (i & k) == 0;
+                        if (gt == up) {
+                            key[i] = kb;
+                            key[ixj] = ka;
+                            sec[i] = sb;
+                            sec[ixj] = sa;
+                            const uint16_t t = pos[i];
+                            pos[i] = pos[ixj];
+                            pos[ixj] = t;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int i = tid; i < n; i += 256) perm[base + i] = pos[i];
+        __syncthreads();
+    }
+}
+
+// global-memory bitonic sort for rays beyond the LDS bins (rare): one
+// workgroup per listed ray, scratch of P entries per ray
+__global__ __launch_bounds__(256) void k_sort_global(const int32_t *__restrict__ rays_big, int32_t nbig,
+                                                     DevHits H, const uint32_t *__restrict__ orig,
+                                                     uint32_t *__restrict__ perm,
+                                                     unsigned long long *__restrict__ gkey,
+                                                     uint32_t *__restrict__ gsec, int32_t pmax) {
+    const int tid = threadIdx.x;
+    for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
+        const int64_t r = rays_big[b];
+        const int n = H.count[r];
+        const int64_t base = r * (int64_t)H.cap;
+        unsigned long long *key = gkey + (int64_t)b * pmax;
+        uint32_t *sec = gsec + (int64_t)b * pmax;
+        uint32_t *pos = perm + base;  // sorted in place over a padded temp
+        int P = 1;
+        while (P < n) P <<= 1;
+        // pos needs P entries; the slot holds cap >= n; pad lives in gsec's upper half
+        uint32_t *ppos = gsec + (int64_t)nbig * pmax + (int64_t)b * pmax;
+        for (int i = tid; i < P; i += 256) {
+            if (i < n) {
+                key[i] = (unsigned long long)__double_as_longlong(H.d[base + i]);
+                sec[i] = orig[H.slot[base + i]] & ~PSR_INVALID_BIT;
+            } else {
+                key[i] = ~0ull;
+                sec[i] = ~0u;
+            }
+            ppos[i] = (uint32_t)i;
+        }
+        __syncthreads();
+        for (int k = 2; k <= P; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = tid; i < P; i += 256) {
+                    const int ixj = i ^ j;
+                    if (ixj > i) {
+                        const unsigned long long ka = key[i], kb = key[ixj];
+                        const uint32_t sa = sec[i], sb = sec[ixj];
+                        const bool gt = ka > kb || (ka == kb && sa > sb);
+                        const bool up = (i & k) == 0;
+                        if (gt == up) {
+                            key[i] = kb;
+                            key[ixj] = ka;
+                            sec[i] = sb;
+                            sec[ixj] = sa;
+                            const uint32_t t = ppos[i];
+                            ppos[i] = ppos[ixj];
+                            ppos[ixj] = t;
+                        }
+                    }
+                }
+                __threadfence_block();
+                __syncthreads();
+            }
+        }
+        for (int i = tid; i < n; i += 256) pos[i] = ppos[i];
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ tables
+// clamped linear interpolation of src/tabfun.c:38-58
+__device__ __forceinline__ double tab_lin(const double *xcol, const double *ycol, int nrows,
+                                          double min, double step_inv, double v) {
+    const long long idx = (long long)((v - min) * step_inv);
+    if (idx < 0) return ycol[0];
+    if (idx >= nrows - 1) return ycol[nrows - 1];
+    const double left = xcol[idx], right = xcol[idx + 1];
+    const double lw = (right - v), rw = (v - left);
+    return (lw * ycol[idx] + rw * ycol[idx + 1]) * step_inv;
+}
+
+__device__ __forceinline__ double xs_get(const DevTables &T, int id, double eng) {
+    if (eng < T.xs_cut[id]) return 0.0;
+    return tab_lin(T.xs, T.xs + (size_t)(id + 1) * T.xs_nrows, T.xs_nrows, T.xs_min, T.xs_step_inv, eng);
+}
+
+// src/sph-depth.c:224-231
+__device__ __forceinline__ double sph_depth(const DevTables &T, double r_h) {
+    const int n = (int)(T.kernel_n * r_h);
+    return (n < T.kernel_n && n >= 0) ? T.kernel[n] : 0.0;
+}
+
+// bilinear TabFun2 lookup of src/tabfun.c:173-223
+__device__ double tab2_get(const DevTables &T, const double *d, double x, double y) {
+    const int n1 = T.si_n1, n2 = T.si_n2;
+    const int xind = (int)((x - T.si_min1) * T.si_step1_inv);
+    const int yind = (int)((y - T.si_min2) * T.si_step2_inv);
+    int linoffset = -1;
+    if (xind < 0) linoffset = 0;
+    if (xind >= n1 - 1) linoffset = (n1 - 1) * n2;
+    if (linoffset != -1) {
+        if (yind < 0) return d[linoffset];
+        if (yind >= n2 - 1) return d[linoffset + n2 - 1];
+        const double left = T.si_y[yind], right = T.si_y[yind + 1];
+        return ((right - y) * d[linoffset + yind] + (y - left) * d[linoffset + yind + 1]) * T.si_step2_inv;
+    }
+    int lin = -1;
+    if (yind < 0) lin = 0;
+    if (yind >= n2 - 1) lin = n2 - 1;
+    if (lin != -1) {
+        const double left = T.si_x[xind], right = T.si_x[xind + 1];
+        return ((right - x) * d[(size_t)xind * n2 + lin] + (x - left) * d[(size_t)(xind + 1) * n2 + lin]) *
+               T.si_step1_inv;
+    }
+    const double x1 = T.si_x[xind], x2 = T.si_x[xind + 1];
+    const double y1 = T.si_y[yind], y2 = T.si_y[yind + 1];
+    const double fac = T.si_step1_inv * T.si_step2_inv;
+    const double f11 = d[(size_t)xind * n2 + yind];
+    const double f12 = d[(size_t)xind * n2 + yind + 1];
+    const double f21 = d[(size_t)(xind + 1) * n2 + yind];
+    const double f22 = d[(size_t)(xind + 1) * n2 + yind + 1];
+    return fac * (f11 * (x2 - x) * (y2 - y) + f21 * (x - x1) * (y2 - y) + f12 * (x2 - x) * (y - y1) +
+                  f22 * (x - x1) * (y - y1));
+}
+
+// ------------------------------------------------------------ deposit
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <bool HE>
+__global__ __launch_bounds__(256) void k_deposit(const psr_ray *__restrict__ rays, int64_t nray,
+                                                 DevHits H, const uint32_t *__restrict__ perm,
+                                                 DevState S, DevTables T, DevPhys P,
+                                                 DevStats *__restrict__ st, double *__restrict__ ray_len,
+                                                 double *__restrict__ ray_tm) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave0 = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t nwaves = gridDim.x * (int64_t)(blockDim.x >> 6);
+    for (int64_t r = wave0; r < nray; r += nwaves) {
+        const psr_ray R = rays[r];
+        const int n = H.count[r];
+        if (lane == 0) {
+            ray_len[r] = R.length;
+            ray_tm[r] = R.Nph;
+        }
+        if (n == 0 || R.Nph == 0) continue;
+        const int64_t base = r * (int64_t)H.cap;
+        double TM = R.Nph;
+        if (P.no_photon && R.type < 0) TM = 0.0;
+        const double Nph = R.Nph;
+        const double freq = R.freq;
+        const double sigmaHI = xs_get(T, 0, freq);
+        const double sigmaHeI = HE ? xs_get(T, 1, freq) : 0.0;
+        const double sigmaHeII = HE ? xs_get(T, 2, freq) : 0.0;
+        const double dEHI = fdim(freq, P.C_HI_ENERGY);
+        const double dEHeI = fdim(freq, P.C_HEI_ENERGY);
+        const double dEHeII = fdim(freq, P.C_HEII_ENERGY);
+        const double hf_HI = P.C_H_PER_MASS * dEHI;
+        const double hf_HeI = P.C_HE_PER_MASS * dEHeI;
+        const double hf_HeII = P.C_HE_PER_MASS * dEHeII;
+        const bool secion_terms = P.secondary_ionization && (dEHI > 0 || dEHeI > 0) && T.si_n1 > 0;
+        int last = n - 1;
+        bool done = false;
+        int nsat = 0;
+        double fiHI = 0, fiHeI = 0, fiHeII = 0, siHI = 0, siHeI = 0;
+        for (int c0 = 0; c0 < n && !done; c0 += 64) {
+            const int i = c0 + lane;
+            const bool valid = i < n;
+            uint32_t slot = 0;
+            double b = 0, sml = 1, mass = 0, xHI = 0, xHeI = 0, xHeII = 0;
+            if (valid) {
+                const uint32_t q = perm[base + i];
+                slot = H.slot[base + q];
+                b = H.b[base + q];
+                sml = S.ps[slot].w;
+                mass = S.mass[slot];
+                xHI = S.lH[slot];
+                xHeI = S.lHeI[slot];
+                xHeII = S.lHeII[slot];
+            }
+            // src/loop.c:427-477
+            const double sml_inv = 1.0 / sml;
+            const double NH = mass * P.C_H_PER_MASS;
+            const double NHe = mass * P.C_HE_PER_MASS;
+            const double NH_inv = 1.0 / NH;
+            const double NHe_inv = 1.0 / NHe;
+            const double kernel = sph_depth(T, b * sml_inv) * (sml_inv * sml_inv) * P.scaling_fac2_inv;
+            const double NHI = xHI * NH;
+            const double tauHI = sigmaHI * (kernel * NHI);
+            double tausum = tauHI;
+            double tauHeI = 0, tauHeII = 0;
+            const bool he = HE && NHe != 0.0;
+            if (he) {
+                tauHeI = sigmaHeI * (kernel * (xHeI * NHe));
+                tausum += tauHeI;
+                tauHeII = sigmaHeII * (kernel * (xHeII * NHe));
+                tausum += tauHeII;
+            }
+            const bool skip = !valid || tausum == 0.0;
+            const double em = skip ? 0.0 : expm1(-tausum);
+            const double fHI = skip ? 0.0 : tauHI / tausum;
+            const double fHeI = (skip || !he) ? 0.0 : tauHeI / tausum;
+            const double fHeII = (skip || !he) ? 0.0 : tauHeII / tausum;
+            const double NHeI = xHeI * NHe, NHeII = xHeII * NHe;
+            const unsigned long long skipmask = __ballot(skip);
+            const unsigned long long hemask = __ballot(he);
+            const int m = min(64, n - c0);
+            double my_aHI = 0, my_aHeI = 0, my_aHeII = 0;
+            bool mine = false;
+            // transmission recurrence, src/loop.c:479-504, 580-589
+            for (int e = 0; e < m; e++) {
+                if ((skipmask >> e) & 1ull) continue;
+                const double absorb_est = -TM * readlane_f64(em, e);
+                double aHI = readlane_f64(fHI, e) * absorb_est;
+                const double nhi = readlane_f64(NHI, e);
+                if (aHI > nhi) {
+                    aHI = nhi;
+                    nsat++;
+                }
+                double absorb = aHI;
+                double aHeI = 0, aHeII = 0;
+                if (HE && ((hemask >> e) & 1ull)) {
+                    aHeI = readlane_f64(fHeI, e) * absorb_est;
+                    const double n1 = readlane_f64(NHeI, e);
+                    if (aHeI > n1) {
+                        aHeI = n1;
+                        nsat++;
+                    }
+                    aHeII = readlane_f64(fHeII, e) * absorb_est;
+                    const double n2 = readlane_f64(NHeII, e);
+                    if (aHeII > n2) {
+                        aHeII = n2;
+                        nsat++;
+                    }
+                    absorb += (aHeI + aHeII);
+                }
+                if (lane == e) {
+                    my_aHI = aHI;
+                    my_aHeI = aHeI;
+                    my_aHeII = aHeII;
+                    mine = true;
+                }
+                TM -= absorb;
+                if (TM < 0.0 || TM / Nph < 1e-10) {
+                    last = c0 + e;
+                    done = true;
+                    break;
+                }
+            }
+            if (mine) {
+                // src/loop.c:486-573
+                const double deltaHI = my_aHI * NH_inv;
+                const double deltaHeI = he ? my_aHeI * NHe_inv : 0.0;
+                const double deltaHeII = he ? my_aHeII * NHe_inv : 0.0;
+                const double ye = (1.0 - xHI) + P.C_HEMF / P.C_HMF * 0.25 * (xHeII + 2 * (1.0 - xHeII - xHeI));
+                double x = ye * NH / (NH + NHe);
+                if (x > 1.0) x = 1.0;
+                if (P.secondary_ionization && x > 0.0) {
+                    atomicAdd(&S.yGdepHI[slot], (float)deltaHI);
+                    fiHI += deltaHI * NH;
+                    if (secion_terms) {
+                        const double log10x = log10(x);
+                        const double sec =
+                            deltaHI * dEHI / P.C_HI_ENERGY * tab2_get(T, T.si_phi_hi, dEHI, log10x) +
+                            deltaHeI * dEHeI / P.C_HI_ENERGY * tab2_get(T, T.si_phi_hi, dEHeI, log10x);
+                        atomicAdd(&S.yGdepHI[slot], (float)sec);
+                        siHI += sec * NH;
+                        atomicAdd(&S.heat[slot], (float)(hf_HI * deltaHI * tab2_get(T, T.si_eh, dEHI, log10x)));
+                        if (he) {
+                            const double sec2 =
+                                deltaHI * dEHI / P.C_HEI_ENERGY * tab2_get(T, T.si_phi_hei, dEHI, log10x) +
+                                deltaHeI * dEHeI / P.C_HEI_ENERGY * tab2_get(T, T.si_phi_hei, dEHeI, log10x);
+                            atomicAdd(&S.yGdepHeI[slot], (float)(deltaHeI + sec2));
+                            siHeI += sec2 * NHe;
+                            fiHeI += deltaHeI * NHe;
+                            atomicAdd(&S.heat[slot],
+                                      (float)(hf_HeI * deltaHeI * tab2_get(T, T.si_eh, dEHeI, log10x)));
+                            atomicAdd(&S.yGdepHeII[slot], (float)deltaHeII);
+                            fiHeII += deltaHeII * NHe;
+                        }
+                    } else if (he) {
+                        atomicAdd(&S.yGdepHeI[slot], (float)deltaHeI);
+                        fiHeI += deltaHeI * NHe;
+                        atomicAdd(&S.yGdepHeII[slot], (float)deltaHeII);
+                        fiHeII += deltaHeII * NHe;
+                    }
+                } else {
+                    atomicAdd(&S.yGdepHI[slot], (float)deltaHI);
+                    if (hf_HI != 0.0) atomicAdd(&S.heat[slot], (float)(hf_HI * deltaHI));
+                    if (he) {
+                        atomicAdd(&S.yGdepHeI[slot], (float)deltaHeI);
+                        atomicAdd(&S.yGdepHeII[slot], (float)deltaHeII);
+                        if (hf_HeI != 0.0) atomicAdd(&S.heat[slot], (float)(deltaHeI * hf_HeI));
+                        if (hf_HeII != 0.0) atomicAdd(&S.heat[slot], (float)(deltaHeII * hf_HeII));
+                    }
+                }
+                atomicAdd(&S.hits[slot], 1);
+            }
+        }
+        // ray results and counters
+        const double dl = H.d[base + perm[base + last]];
+        fiHI = wave_sum(fiHI);
+        siHI = wave_sum(siHI);
+        if (HE) {
+            fiHeI = wave_sum(fiHeI);
+            fiHeII = wave_sum(fiHeII);
+            siHeI = wave_sum(siHeI);
+        }
+        if (lane == 0) {
+            ray_len[r] = dl;
+            ray_tm[r] = TM;
+            atomicAdd(&st->total_deposit_count, (unsigned long long)n);
+            if (nsat) atomicAdd(&st->saturated_deposit_count, (unsigned long long)nsat);
+            atomicAdd(&st->lost_photon_count_sum, TM);
+            atomicAdd(&st->first_ionization_HI, fiHI);
+            atomicAdd(&st->secondary_ionization_HI, siHI);
+            if (HE) {
+                atomicAdd(&st->first_ionization_HeI, fiHeI);
+                atomicAdd(&st->first_ionization_HeII, fiHeII);
+                atomicAdd(&st->secondary_ionization_HeI, siHeI);
+            }
+        }
+    }
+}
+
+// total_deposit_count when deposit is skipped (traceOnly)
+__global__ void k_count_hits(const int32_t *__restrict__ count, int64_t nray, DevStats *st) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int v = i < nray ? count[i] : 0;
+    v = wave_sum_i(v);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(&st->total_deposit_count, (unsigned long long)v);
+}
+
+// parity export: CSR lists with caller indices
+__global__ void k_export_lists(int64_t nray, DevHits H, const uint32_t *__restrict__ perm,
+                               const uint32_t *__restrict__ orig, const int64_t *__restrict__ off,
+                               int64_t *__restrict__ oi, double *__restrict__ od, double *__restrict__ ob) {
+    const int64_t r = blockIdx.x;
+    if (r >= nray) return;
+    const int n = H.count[r];
+    const int64_t base = r * (int64_t)H.cap;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint32_t q = perm[base + i];
+        oi[off[r] + i] = (int64_t)(orig[H.slot[base + q]] & ~PSR_INVALID_BIT);
+        od[off[r] + i] = H.d[base + q];
+        ob[off[r] + i] = H.b[base + q];
+    }
+}
+
+// ------------------------------------------------------------ launchers
+static int ensure_hits(psr_ctx *c, int64_t nray, int32_t cap) {
+    if (nray <= c->hit_slots && cap <= c->H.cap && c->H.d) return PSR_OK;
+    const int64_t ns = std::max(nray, c->hit_slots);
+    const int32_t cp = std::max(cap, c->H.cap);
+    if (c->H.d) (void)hipFree(c->H.d);
+    if (c->H.b) (void)hipFree(c->H.b);
+    if (c->H.slot) (void)hipFree(c->H.slot);
+    if (c->H.count) (void)hipFree(c->H.count);
+    if (c->d_ray_len) (void)hipFree(c->d_ray_len);
+    if (c->d_ray_tm) (void)hipFree(c->d_ray_tm);
+    c->H = DevHits{};
+    c->d_ray_len = c->d_ray_tm = nullptr;
+    c->hit_slots = 0;
+    const size_t tot = (size_t)ns * cp;
+    PSR_HIP(c, hipMalloc(&c->H.d, sizeof(double) * tot));
+    PSR_HIP(c, hipMalloc(&c->H.b, sizeof(double) * tot));
+    // slot array followed by the sort permutation (same size)
+    PSR_HIP(c, hipMalloc(&c->H.slot, sizeof(uint32_t) * tot * 2));
+    PSR_HIP(c, hipMalloc(&c->H.count, sizeof(int32_t) * ns));
+    PSR_HIP(c, hipMalloc(&c->d_ray_len, sizeof(double) * ns));
+    PSR_HIP(c, hipMalloc(&c->d_ray_tm, sizeof(double) * ns));
+    c->H.cap = cp;
+    c->hit_slots = ns;
+    return PSR_OK;
+}
+
+static inline uint32_t *perm_of(psr_ctx *c) { return c->H.slot + (size_t)c->hit_slots * c->H.cap; }
+
+static int read_flags(psr_ctx *c, int32_t *flags) {
+    PSR_HIP(c, hipMemcpyAsync(flags, &c->d_stats->flags, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    PSR_HIP(c, hipStreamSynchronize(c->stream));
+    return PSR_OK;
+}
+static int clear_flags(psr_ctx *c) {
+    PSR_HIP(c, hipMemsetAsync(&c->d_stats->flags, 0, sizeof(int32_t), c->stream));
+    return PSR_OK;
+}
+
+int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray) {
+    int32_t cap0 = c->H.cap > 0 ? c->H.cap : 2048;
+    int rc = ensure_hits(c, nray, cap0);
+    if (rc) return rc;
+    for (int attempt = 0; attempt < 8; attempt++) {
+        if ((rc = clear_flags(c))) return rc;
+        const unsigned grid = (unsigned)std::min<int64_t>(nray, 65535 * 4);
+        PSR_EV(c, 0, 0);
+        k_trace<<<grid, TR_THREADS, 0, c->stream>>>(c->d_rays + ray_first, nray, c->d_nodes,
+                                                    c->d_field_root, c->d_ps, c->d_orig, c->H,
+                                                    c->d_active, c->d_stats);
+        PSR_HIP(c, hipGetLastError());
+        PSR_EV(c, 0, 1);
+        int32_t fl = 0;
+        if ((rc = read_flags(c, &fl))) return rc;
+        if (fl & PSR_DF_STACK) PSR_FAIL(c, PSR_ERR_CAPACITY, "traversal stack overflow");
+        if (!(fl & PSR_DF_HITCAP)) {
+            c->last_nray = nray;
+            return PSR_OK;
+        }
+        // grow the per-ray hit capacity to the largest list and trace again
+        std::vector<int32_t> cnt(nray);
+        PSR_HIP(c, hipMemcpy(cnt.data(), c->H.count, sizeof(int32_t) * nray, hipMemcpyDeviceToHost));
+        const int32_t mx = *std::max_element(cnt.begin(), cnt.end());
+        int32_t ncap = c->H.cap;
+        while (ncap < mx) ncap *= 2;
+        c->H.cap = 0;  // force reallocation at the new size
+        const int64_t ns = c->hit_slots;
+        c->hit_slots = 0;
+        if ((rc = ensure_hits(c, std::max(ns, nray), ncap))) return rc;
+    }
+    PSR_FAIL(c, PSR_ERR_CAPACITY, "hit capacity did not converge");
+}
+
+int psr_launch_sort(psr_ctx *c, int64_t nray) {
+    uint32_t *perm = perm_of(c);
+    const unsigned grid = (unsigned)std::min<int64_t>(nray, 65535 * 4);
+    PSR_EV(c, 1, 0);
+    k_sort<2048><<<grid, 256, 0, c->stream>>>(nray, c->H, c->d_orig, perm, -1, 2048);
+    PSR_HIP(c, hipGetLastError());
+    if (c->H.cap > 2048) {
+        k_sort<8192><<<grid, 256, 0, c->stream>>>(nray, c->H, c->d_orig, perm, 2048, 8192);
+        PSR_HIP(c, hipGetLastError());
+    }
+    if (c->H.cap > 8192) {
+        std::vector<int32_t> cnt(nray), big;
+        PSR_HIP(c, hipMemcpyAsync(cnt.data(), c->H.count, sizeof(int32_t) * nray, hipMemcpyDeviceToHost,
+                                  c->stream));
+        PSR_HIP(c, hipStreamSynchronize(c->stream));
+        int32_t pmax = 1;
+        for (int64_t r = 0; r < nray; r++)
+            if (cnt[r] > 8192) {
+                big.push_back((int32_t)r);
+                while (pmax < cnt[r]) pmax <<= 1;
+            }
+        if (!big.empty()) {
+            const int32_t nb = (int32_t)big.size();
+            const size_t need = sizeof(int32_t) * nb + (size_t)nb * pmax * (8 + 4 + 4) + 256;
+            int rc = psr_ensure_scratch(c, need);
+            if (rc) return rc;
+            char *p = (char *)c->d_scratch;
+            int32_t *d_big = (int32_t *)p;
+            unsigned long long *gkey = (unsigned long long *)(p + ((sizeof(int32_t) * nb + 255) / 256) * 256);
+            uint32_t *gsec = (uint32_t *)(gkey + (size_t)nb * pmax);
+            PSR_HIP(c, hipMemcpyAsync(d_big, big.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, c->stream));
+            PSR_HIP(c, hipStreamSynchronize(c->stream));  // 'big' is released on return
+            k_sort_global<<<std::min(nb, 1024), 256, 0, c->stream>>>(d_big, nb, c->H, c->d_orig, perm, gkey,
+                                                                     gsec, pmax);
+            PSR_HIP(c, hipGetLastError());
+        }
+    }
+    PSR_EV(c, 1, 1);
+    return PSR_OK;
+}
+
+int psr_launch_deposit(psr_ctx *c, int64_t ray_first, int64_t nray) {
+    const DevPhys P = psr_dev_phys(c);
+    const DevState S = c->state();
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nray + 3) / 4, 65535 * 4));
+    const bool he = !c->F.h_only;
+    PSR_EV(c, 2, 0);
+    if (he)
+        k_deposit<true><<<grid, 256, 0, c->stream>>>(c->d_rays + ray_first, nray, c->H, perm_of(c), S, c->T,
+                                                     P, c->d_stats, c->d_ray_len, c->d_ray_tm);
+    else
+        k_deposit<false><<<grid, 256, 0, c->stream>>>(c->d_rays + ray_first, nray, c->H, perm_of(c), S, c->T,
+                                                      P, c->d_stats, c->d_ray_len, c->d_ray_tm);
+    PSR_HIP(c, hipGetLastError());
+    PSR_EV(c, 2, 1);
+    return PSR_OK;
+}
+
+static int check_ready(psr_ctx *c, int64_t ray_first, int64_t nray) {
+    if (!c->have_tree) PSR_FAIL(c, PSR_ERR_STATE, "no tree: call psr_tree_build");
+    if (!c->have_state) PSR_FAIL(c, PSR_ERR_STATE, "no particle state: call psr_state_upload");
+    if (ray_first < 0 || nray < 0 || ray_first + nray > c->nray_pool)
+        PSR_FAIL(c, PSR_ERR_ARG, "ray range outside the uploaded pool");
+    return PSR_OK;
+}
+
+extern "C" int psr_trace_lists(psr_ctx *c, int64_t ray_first, int64_t nray, int64_t *offsets, int64_t cap,
+                               int64_t *ipar, double *d, double *b) {
+    if (!c || !offsets) return PSR_ERR_ARG;
+    int rc = check_ready(c, ray_first, nray);
+    if (rc) return rc;
+    PSR_HIP(c, hipSetDevice(c->device));
+    offsets[0] = 0;
+    if (nray == 0) return PSR_OK;
+    if ((rc = psr_launch_trace(c, ray_first, nray))) return rc;
+    // parity mode marks particles active like the tick does; clear them
+    PSR_HIP(c, hipMemsetAsync(c->d_active, 0, c->npar > 0 ? c->npar : 1, c->stream));
+    if ((rc = psr_launch_sort(c, nray))) return rc;
+    std::vector<int32_t> cnt(nray);
+    PSR_HIP(c, hipMemcpyAsync(cnt.data(), c->H.count, sizeof(int32_t) * nray, hipMemcpyDeviceToHost, c->stream));
+    PSR_HIP(c, hipStreamSynchronize(c->stream));
+    for (int64_t r = 0; r < nray; r++) offsets[r + 1] = offsets[r] + cnt[r];
+    const int64_t total = offsets[nray];
+    if (total > cap || !ipar || !d || !b) {
+        c->err = "psr_trace_lists: output capacity smaller than the hit count";
+        return PSR_ERR_CAPACITY;
+    }
+    if (total == 0) return PSR_OK;
+    int64_t *d_off = nullptr, *d_i = nullptr;
+    double *d_d = nullptr, *d_b = nullptr;
+    auto fr = [&]() {
+        if (d_off) (void)hipFree(d_off);
+        if (d_i) (void)hipFree(d_i);
+        if (d_d) (void)hipFree(d_d);
+        if (d_b) (void)hipFree(d_b);
+    };
+    if (hipMalloc(&d_off, sizeof(int64_t) * (nray + 1)) != hipSuccess ||
+        hipMalloc(&d_i, sizeof(int64_t) * total) != hipSuccess ||
+        hipMalloc(&d_d, sizeof(double) * total) != hipSuccess ||
+        hipMalloc(&d_b, sizeof(double) * total) != hipSuccess) {
+        fr();
+        PSR_FAIL(c, PSR_ERR_HIP, "psr_trace_lists: out of device memory");
+    }
+    hipError_t e = hipMemcpyAsync(d_off, offsets, sizeof(int64_t) * (nray + 1), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        k_export_lists<<<(unsigned)nray, 256, 0, c->stream>>>(nray, c->H, perm_of(c), c->d_orig, d_off, d_i, d_d, d_b);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(ipar, d_i, sizeof(int64_t) * total, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d, d_d, sizeof(double) * total, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(b, d_b, sizeof(double) * total, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    fr();
+    if (e != hipSuccess) PSR_FAIL(c, PSR_ERR_HIP, std::string("psr_trace_lists: ") + hipGetErrorString(e));
+    return PSR_OK;
+}
+
+extern "C" int psr_tick(psr_ctx *c, int64_t ray_first, int64_t nray, int64_t tick, double tick_time,
+                        int32_t phases, int32_t chem_mode) {
+    if (!c) return PSR_ERR_ARG;
+    int rc = check_ready(c, ray_first, nray);
+    if (rc) return rc;
+    if (!c->have_physics) PSR_FAIL(c, PSR_ERR_STATE, "no physics: call psr_set_physics");
+    if (!(phases & PSR_PHASE_TRACE)) PSR_FAIL(c, PSR_ERR_ARG, "a tick starts with the trace phase");
+    if (chem_mode != PSR_CHEM_EXACT && chem_mode != PSR_CHEM_SURROGATE)
+        PSR_FAIL(c, PSR_ERR_ARG, "unknown chemistry mode");
+    PSR_HIP(c, hipSetDevice(c->device));
+    c->ev_mask = 0;
+    PSR_EV(c, 4, 0);
+    if (nray > 0) {
+        if ((rc = psr_launch_trace(c, ray_first, nray))) return rc;
+        if (phases & PSR_PHASE_DEPOSIT) {
+            if ((rc = psr_launch_sort(c, nray))) return rc;
+            if ((rc = psr_launch_deposit(c, ray_first, nray))) return rc;
+        } else {
+            k_count_hits<<<(unsigned)((nray + 255) / 256), 256, 0, c->stream>>>(c->H.count, nray, c->d_stats);
+            PSR_HIP(c, hipGetLastError());
+        }
+    }
+    if (phases & PSR_PHASE_UPDATE) {
+        if (chem_mode == PSR_CHEM_SURROGATE) rc = psr_mlp_update_dev(c, tick, tick_time);
+        else rc = psr_launch_update(c, tick, tick_time, chem_mode);
+        if (rc) return rc;
+    } else {
+        // merge without update: the active set is dropped (traceOnly)
+        PSR_HIP(c, hipMemsetAsync(c->d_active, 0, c->npar > 0 ? c->npar : 1, c->stream));
+    }
+    PSR_EV(c, 4, 1);
+    if (c->timing) {
+        PSR_HIP(c, hipEventSynchronize(c->ev[9]));
+        for (int k = 0; k < 5; k++) {
+            if (!(c->ev_mask & (1 << k))) continue;
+            float a = 0;
+            PSR_HIP(c, hipEventElapsedTime(&a, c->ev[2 * k], c->ev[2 * k + 1]));
+            c->t_ms[k] += a;
+        }
+    }
+    c->ticks++;
+    return PSR_OK;
+}
+
+extern "C" int psr_set_timing(psr_ctx *c, int enable) {
+    if (!c) return PSR_ERR_ARG;
+    PSR_HIP(c, hipSetDevice(c->device));
+    if (enable && !c->ev[0])
+        for (int k = 0; k < 10; k++) PSR_HIP(c, hipEventCreate(&c->ev[k]));
+    c->timing = enable != 0;
+    for (int k = 0; k < 5; k++) c->t_ms[k] = 0;
+    return PSR_OK;
+}
+
+extern "C" int psr_get_timers(psr_ctx *c, double *ms) {
+    if (!c || !ms) return PSR_ERR_ARG;
+    for (int k = 0; k < 5; k++) ms[k] = c->t_ms[k];
+    return PSR_OK;
+}
+
+extern "C" int psr_ray_results(psr_ctx *c, int64_t nray, double *length, double *tm) {
+    if (!c) return PSR_ERR_ARG;
+    if (nray > c->last_nray) PSR_FAIL(c, PSR_ERR_ARG, "more rays than the last tick traced");
+    PSR_HIP(c, hipStreamSynchronize(c->stream));
+    if (length) PSR_HIP(c, hipMemcpy(length, c->d_ray_len, sizeof(double) * nray, hipMemcpyDeviceToHost));
+    if (tm) PSR_HIP(c, hipMemcpy(tm, c->d_ray_tm, sizeof(double) * nray, hipMemcpyDeviceToHost));
+    return PSR_OK;
+}

@@ -1,0 +1,418 @@
+"""Pins the CPU oracle (oracle/psr_oracle.c) before it is trusted as the
+checker: golden vectors from the reference's own compiled sources
+(oracle/_ref -> tests/golden), published constants and fits, closed forms
+and brute force on tiny fields."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+
+@pytest.fixture(scope="module")
+def orc_h():
+    return po.Oracle(po.units(hmf=1.0), po.flags())
+
+
+@pytest.fixture(scope="module")
+def orc_he():
+    return po.Oracle(po.units(hmf=0.76), po.flags(isothermal=False, h_only=False))
+
+
+# ---------------------------------------------------------------- geometry
+def test_pluecker_matches_reference_kat(golden):
+    k = np.load(f"{golden}/pluecker_kat.npz")
+    L = po.lib()
+    n = len(k["hit"])
+    out = np.zeros(n, np.int32)
+    for i in range(n):
+        d = np.ascontiguousarray(k["dir"][i])
+        ds = np.array([k["dist"][i]], np.float32)
+        b = np.ascontiguousarray(k["s2b"][i])
+        t = np.ascontiguousarray(k["s2t"][i])
+        out[i] = L.or_pluecker(d.ctypes.data, ds.ctypes.data, b.ctypes.data, t.ctypes.data)
+    assert 0.1 < k["hit"].mean() < 0.9
+    assert np.array_equal(out, k["hit"])
+
+
+def exact_segment_box(d, dist, s2b, s2t):
+    """slab test in float64 for the segment 0 -> dist*d against [s2b, s2t]"""
+    t0, t1 = 0.0, float(dist)
+    for i in range(3):
+        if d[i] == 0:
+            if s2b[i] > 0 or s2t[i] < 0:
+                return 0
+            continue
+        a, b = s2b[i] / d[i], s2t[i] / d[i]
+        lo, hi = min(a, b), max(a, b)
+        t0, t1 = max(t0, lo), min(t1, hi)
+        if t0 > t1:
+            return 0
+    return 1
+
+
+def test_pluecker_agrees_with_exact_slab_test(golden):
+    k = np.load(f"{golden}/pluecker_kat.npz")
+    ex = np.array([exact_segment_box(k["dir"][i].astype(np.float64), k["dist"][i],
+                                     k["s2b"][i].astype(np.float64), k["s2t"][i].astype(np.float64))
+                   for i in range(len(k["hit"]))])
+    # the float test differs from exact geometry only inside rounding bands
+    # (touching faces); more than 99% agree and no disagreement is far from a face
+    agree = ex == k["hit"]
+    assert agree.mean() > 0.99
+
+
+def test_sph_depth_matches_reference(golden, orc_h):
+    k = np.load(f"{golden}/sph_depth_kat.npz")
+    v = np.array([orc_h.sph_depth(float(r)) for r in k["r"]])
+    assert np.array_equal(v, k["v"])
+
+
+def _column(b):
+    from scipy import integrate
+
+    def W(r):
+        n = 8 / np.pi
+        return n * (1 - 6 * r * r + 6 * r ** 3) if r <= 0.5 else (n * 2 * (1 - r) ** 3 if r <= 1 else 0)
+    if b >= 1:
+        return 0.0
+    zm = np.sqrt(1 - b * b)
+    pts = [np.sqrt(0.25 - b * b)] if b < 0.5 else None
+    v, _ = integrate.quad(lambda z: W(np.sqrt(b * b + z * z)), 0, zm, points=pts, epsabs=1e-14,
+                          epsrel=1e-13, limit=200)
+    return 2 * v
+
+
+def test_kernel_table_is_the_column_integral(golden):
+    # src/kernel.akline = column integral of the cubic spline at b = 4n/16383
+    tab = np.load(f"{golden}/kernel_akline.npy")
+    assert tab.shape == (4096,)
+    assert tab[0] == np.float32(6 / np.pi)
+    idx = np.arange(0, 4096, 37)
+    ex = np.array([_column(4 * n / 16383) for n in idx])
+    m = tab[idx] != 0
+    assert np.max(np.abs(ex[m] - tab[idx][m]) / tab[idx][m]) < 3e-7
+
+
+# ------------------------------------------------------------ tables/units
+def test_units_closed_form(orc_h):
+    u = orc_h.u
+    # 1 kpc = 3.08568025e21 cm; internal length is kpc/h with h = 0.72
+    assert u.U_KPC == pytest.approx(0.72 * 3.08568025e21 / 3.0835455558318480e21, rel=1e-14)
+    assert u.C_H_PER_MASS * u.U_MPROTON == pytest.approx(1.0, rel=1e-14)
+    for T in (1e2, 1e4, 3e5):
+        assert orc_h.ieye2T(orc_h.Tye2ie(T, 0.3), 0.3) == pytest.approx(T, rel=1e-14)
+
+
+def test_rate_table_columns_match_hui_gnedin_fits(orc_h):
+    u = orc_h.u
+    cm3s = u.U_CM ** 3 / u.U_SEC
+    for logT in (2.0, 3.0, 4.0, 4.5, 5.0, 6.0):
+        T = 10 ** logT
+        lam = 2 * 157807.0 / T
+        aA = 1.269e-13 * lam ** 1.503 / (1 + (lam / 0.522) ** 0.470) ** 1.923
+        aB = 2.753e-14 * lam ** 1.500 / (1 + (lam / 2.740) ** 0.407) ** 2.242
+        assert orc_h.ar_get(1, logT) / cm3s == pytest.approx(aA, rel=2e-4)   # HIIrcA
+        assert orc_h.ar_get(3, logT) / cm3s == pytest.approx(aB, rel=2e-4)   # HIIrcB
+    # table row logT = 4.000, column HIIrcB
+    assert orc_h.ar_get(3, 4.0) / cm3s == pytest.approx(2.59182e-13, rel=1e-12)
+
+
+def test_stromgren_radius_from_table(orc_h):
+    u = orc_h.u
+    aB = orc_h.ar_get(3, 4.0) / (u.U_CM ** 3 / u.U_SEC)
+    rs_cm = (3 * 5e48 / (4 * np.pi * aB * 1e-3 ** 2)) ** (1 / 3)
+    assert rs_cm / 3.08568025e21 == pytest.approx(5.392, abs=0.001)
+    assert 1 / (aB * 1e-3) / (3.15576926e13) == pytest.approx(122.3, abs=0.2)
+
+
+def test_cross_sections_published_fits(orc_h):
+    L = po.lib()
+    # Verner et al. 1996: sigma_HI(13.6 eV) = 6.3e-18 cm^2
+    assert L.or_verner_hi(1.0) == pytest.approx(6.30e-18, rel=0.01)
+    # Osterbrock HeII ~ E^-3 above threshold (54.4 eV = 4 Ry)
+    assert L.or_osterbrok_heii(8.0) / L.or_osterbrok_heii(4.0) == pytest.approx(1 / 8, rel=1e-5)
+    # the reference's HeI form uses +0.66 (src/verner.F90:46)
+    assert L.or_osterbrok_hei(24.587 / 13.6) == pytest.approx(7.2e-18 * 2.32, rel=1e-5)
+    # cut below 13.6 eV (src/atomic-rate.c:307)
+    u = orc_h.u
+    assert orc_h.xs_get(0, 13.59 * u.U_EV) == 0.0
+    assert orc_h.xs_get(0, 13.6 * u.U_EV) > 0.0
+
+
+# ------------------------------------------------------------- chemistry
+def test_lsoda_fixture_matches_published_robertson(golden):
+    d = np.load(f"{golden}/lsoda_robertson.npz")
+    rel = np.abs(d["ref"] - d["published"]) / np.maximum(np.abs(d["published"]), 1e-30)
+    assert rel.max() < 1e-2
+
+
+def test_chemistry_matches_reference_lsoda_hydrogen(golden, orc_h):
+    c = np.load(f"{golden}/chem_lsoda.npz")
+    assert (c["h_iso_state"] == 2).all()
+    o, bad = orc_h.step_evolve(c["h_iso_in"])
+    assert bad == 0
+    assert np.abs(o["lambdaH"] - c["h_iso_out"]["lambdaH"]).max() < 1e-3
+
+
+def test_chemistry_matches_reference_lsoda_helium_temperature(golden, orc_he):
+    c = np.load(f"{golden}/chem_lsoda.npz")
+    o, bad = orc_he.step_evolve(c["he_in"])
+    assert bad == 0
+    r = c["he_out"]
+    assert np.abs(o["lambdaH"] - r["lambdaH"]).max() < 1e-3
+    assert np.abs(o["lambdaHeI"] - r["lambdaHeI"]).max() < 1e-2
+    assert np.abs(o["lambdaHeII"] - r["lambdaHeII"]).max() < 1e-2
+    # ie carries atol = 1 (src/solve.c:152): compare in bulk
+    rel = np.abs(o["ie"] - r["ie"]) / np.abs(r["ie"])
+    assert np.median(rel) < 1e-3 and np.percentile(rel, 95) < 5e-2
+
+
+def test_chemistry_converges_to_tight_solution(golden, orc_h):
+    c = np.load(f"{golden}/chem_lsoda.npz")
+    a, _ = orc_h.step_evolve(c["h_iso_in"], rtol=1e-4)
+    b, _ = orc_h.step_evolve(c["h_iso_in"], rtol=1e-10)
+    assert np.abs(a["lambdaH"] - b["lambdaH"]).max() < 3e-4
+
+
+def _step(orc, **kw):
+    s = np.zeros(1, po.STEP_DTYPE)
+    s["lambdaHeI"] = 1.0
+    for k, v in kw.items():
+        s[k] = v
+    return s
+
+
+def test_chemistry_riccati_closed_form():
+    """isothermal H-only at 1e3 K (no collisional ionization in the table):
+    dx/dt = G - a x^2 with x = x_HII, so x(t) = s tanh(a s t + atanh(x0/s)),
+    s = sqrt(G/a).  The deposit enters as -yGdepHI/time (src/solve.c:98)."""
+    orc = po.Oracle(po.units(hmf=1.0), po.flags())
+    u = orc.u
+    logT = 3.0
+    assert orc.ar_get(0, logT) == 0.0
+    nH = 1e-3 / u.U_CM ** 3
+    a = orc.ar_get(3, logT) * nH                      # alpha_B * nH (ye = x_HII)
+    t_rec = 1 / a
+    for frac, x0, G_over_a in ((0.3, 0.1, 0.25), (2.0, 1e-3, 0.04), (0.05, 0.5, 0.81)):
+        time = frac * t_rec
+        G = G_over_a * a                             # per internal time
+        s = _step(orc, lambdaH=1 - x0, nH=nH, T=1e3, time=time, yGdepHI=G * time)
+        o, bad = orc.step_evolve(s, rtol=1e-9)
+        sq = np.sqrt(G / a)
+        x = sq * np.tanh(a * sq * time + np.arctanh(x0 / sq))
+        assert 1 - o["lambdaH"][0] == pytest.approx(x, abs=1e-6)
+
+
+def test_chemistry_equilibrium_case_a_formula():
+    """src/test-solve.c:41-47: with case A and no photons the neutral fraction
+    relaxes to P/q = alpha/(gamma + alpha)"""
+    orc = po.Oracle(po.units(hmf=1.0), po.flags(on_the_spot=False))
+    u = orc.u
+    logT = 4.8
+    nH = 1.0 / u.U_CM ** 3
+    RC = orc.ar_get(1, logT)
+    CI = orc.ar_get(0, logT)
+    P, Q, R = RC, -(CI + 2 * RC), CI + RC
+    q = -0.5 * (Q + np.copysign(np.sqrt(Q * Q - 4 * R * P), Q))
+    s = _step(orc, lambdaH=0.5, nH=nH, T=10 ** logT, time=1000 / ((CI + RC) * nH))
+    o, bad = orc.step_evolve(s, rtol=1e-9)
+    assert o["lambdaH"][0] == pytest.approx(P / q, rel=1e-5)
+
+
+# --------------------------------------------------------- tree and trace
+def tiny_field(n=6, seed=3, boxsize=1.0):
+    rng = np.random.default_rng(seed)
+    g = (np.indices((n, n, n)).reshape(3, -1).T + 0.5 + rng.uniform(-0.25, 0.25, (n ** 3, 3))) * boxsize / n
+    pos = g.astype(np.float32)
+    from psphray_amd.fields import smoothing_lengths
+    sml, _ = smoothing_lengths(pos.astype(np.float64))
+    return pos, sml
+
+
+def test_tree_invariants():
+    pos, sml = tiny_field(8)
+    t = po.Tree(pos, sml, 1.0)
+    e = t.export()
+    fc, npar = e["first_child"], e["npar"]
+    leaf = t.leaf_of()
+    assert (leaf >= 0).all()
+    # a node is internal iff its region holds more than 8 particles
+    cnt = np.zeros(len(fc), np.int64)
+    for i in range(len(pos)):
+        c = leaf[i]
+        while c != -1:
+            cnt[c] += 1
+            c = e["parent"][c]
+    assert ((fc != -1) == (cnt > 8)).all()
+    assert (npar[fc == -1] == cnt[fc == -1]).all()
+    # leaf boxes: union of pos -+ sml; empty leaves keep their octant
+    for c in np.nonzero((fc == -1) & (cnt > 0))[0]:
+        m = leaf == c
+        assert np.array_equal(e["bot"][c], (pos[m] - sml[m, None]).min(axis=0))
+        assert np.array_equal(e["top"][c], (pos[m] + sml[m, None]).max(axis=0))
+    for c in np.nonzero(fc != -1)[0]:
+        ch = np.arange(fc[c], fc[c] + 8)
+        assert np.array_equal(e["bot"][c], e["bot"][ch].min(axis=0))
+        assert np.array_equal(e["top"][c], e["top"][ch].max(axis=0))
+
+
+def impact_numpy(pos, sml, s, d):
+    """src/raytrace.c:172-186 in numpy: float products, double sums"""
+    dd = (pos - s[None, :]).astype(np.float32)
+    pr = (dd * d[None, :]).astype(np.float32)
+    sq = (dd * dd).astype(np.float32)
+    proj = pr[:, 0].astype(np.float64) + pr[:, 1] + pr[:, 2]
+    dist = np.sqrt(sq[:, 0].astype(np.float64) + sq[:, 1] + sq[:, 2])
+    r2 = (dist - proj) * (dist + proj)
+    hh = (sml * sml).astype(np.float32).astype(np.float64)
+    return ~(hh < r2), dist, np.sqrt(np.abs(r2))
+
+
+def test_trace_matches_brute_force_rule():
+    """hit set = particles whose leaf and every ancestor box pass the
+    Pluecker test and whose impact parameter is within sml"""
+    pos, sml = tiny_field(8, seed=5)
+    t = po.Tree(pos, sml, 1.0)
+    e = t.export()
+    leaf = t.leaf_of()
+    rng = np.random.default_rng(9)
+    nray = 40
+    rays = np.zeros(nray, po.RAY_DTYPE)
+    rays["s"] = rng.uniform(0, 1, (nray, 3)).astype(np.float32)
+    dv = rng.normal(size=(nray, 3))
+    rays["dir"] = (dv / np.linalg.norm(dv, axis=1, keepdims=True)).astype(np.float32)
+    rays["length"] = rng.choice([0.3, 2.0], nray)
+
+    class PS:
+        pass
+    ps = PS()
+    ps.npar = len(sml)
+    ps.pos = pos
+    ps.sml = sml
+    for f in ("flag",):
+        setattr(ps, f, np.zeros(len(sml), np.int8))
+    for f in ("mass rho ie yeMET yGdepHI yGdepHeI yGdepHeII yGrecHII yGrecHeII yGrecHeIII heat".split()):
+        setattr(ps, f, np.zeros(len(sml), np.float32))
+    for f in ("lambdaH", "lambdaHeI", "lambdaHeII"):
+        setattr(ps, f, np.zeros(len(sml)))
+    ps.lasthit = np.zeros(len(sml), np.int64)
+    ps.hits = np.zeros(len(sml), np.int32)
+    off, hits = po.trace_rays(t, ps, rays)
+    L = po.lib()
+
+    def boxhit(r, c):
+        d = np.ascontiguousarray(r["dir"], np.float32)
+        ds = np.array([np.float32(r["length"])], np.float32)
+        b = np.ascontiguousarray(e["bot"][c] - r["s"], np.float32)
+        tt = np.ascontiguousarray(e["top"][c] - r["s"], np.float32)
+        return L.or_pluecker(d.ctypes.data, ds.ctypes.data, b.ctypes.data, tt.ctypes.data)
+
+    total = 0
+    for i in range(nray):
+        r = rays[i]
+        ok, dist, b = impact_numpy(pos, sml, r["s"], r["dir"])
+        expect = []
+        for p in np.nonzero(ok)[0]:
+            c = leaf[p]
+            good = True
+            while c != -1:
+                if not boxhit(r, c):
+                    good = False
+                    break
+                c = e["parent"][c]
+            if good:
+                expect.append((dist[p], p, b[p]))
+        expect.sort()
+        got = hits[off[i]:off[i + 1]]
+        assert [x[1] for x in expect] == list(got["ipar"])
+        assert np.array_equal([x[0] for x in expect], got["d"])
+        assert np.array_equal([x[2] for x in expect], got["b"])
+        total += len(expect)
+    assert total > 100
+
+
+# ----------------------------------------------------------------- deposit
+def chain_system(n, orc, xHI=1.0, mass=1.0):
+    class PS:
+        pass
+    ps = PS()
+    ps.npar = n
+    ps.pos = np.zeros((n, 3), np.float32)
+    ps.sml = np.full(n, 1.0, np.float32)
+    ps.flag = np.zeros(n, np.int8)
+    for f in ("mass rho ie yeMET yGdepHI yGdepHeI yGdepHeII yGrecHII yGrecHeII yGrecHeIII heat".split()):
+        setattr(ps, f, np.zeros(n, np.float32))
+    ps.mass[:] = mass
+    ps.lambdaH = np.full(n, xHI)
+    ps.lambdaHeI = np.ones(n)
+    ps.lambdaHeII = np.zeros(n)
+    ps.lasthit = np.zeros(n, np.int64)
+    ps.hits = np.zeros(n, np.int32)
+    return ps
+
+
+def test_deposit_closed_forms(orc_h):
+    u = orc_h.u
+    n = 12
+    sigma = orc_h.xs_get(0, 13.6 * u.U_EV)
+    # choose masses so tau_j = 0.1 * (j + 1) at impact parameter b = 0.2 sml
+    k = orc_h.sph_depth(0.2)
+    taus = 0.1 * (np.arange(n) + 1)
+    mass = taus / (sigma * k * u.C_H_PER_MASS)
+    ps = chain_system(n, orc_h)
+    ps.mass[:] = mass
+    hits = np.zeros(n, po.HIT_DTYPE)
+    hits["ipar"] = np.arange(n)
+    hits["d"] = np.arange(n) + 1.0
+    hits["b"] = 0.2
+    off = np.array([0, n], np.int64)
+    rays = np.zeros(1, po.RAY_DTYPE)
+    Nph = 1e-3 * float(ps.mass.min()) * u.C_H_PER_MASS    # far from saturation
+    rays["Nph"] = Nph
+    rays["freq"] = 13.6 * u.U_EV
+    ln, tm, st = po.deposit(orc_h, ps, rays, off, hits)
+    tau = sigma * k * ps.mass.astype(np.float64) * u.C_H_PER_MASS
+    assert tm[0] == pytest.approx(Nph * np.exp(-tau.sum()), rel=1e-12)
+    absorbed = ps.yGdepHI.astype(np.float64) * ps.mass * u.C_H_PER_MASS
+    TM = Nph * np.exp(-np.concatenate([[0], np.cumsum(tau)[:-1]]))
+    assert absorbed == pytest.approx(TM * -np.expm1(-tau), rel=2e-7)
+    assert absorbed.sum() + tm[0] == pytest.approx(Nph, rel=1e-6)
+    assert (ps.hits == 1).all() and ln[0] == n
+
+
+def test_deposit_saturation_and_cutoff(orc_h):
+    u = orc_h.u
+    n = 6
+    ps = chain_system(n, orc_h)
+    ps.lambdaH[:] = 0.5
+    NH = float(ps.mass[0]) * u.C_H_PER_MASS
+    hits = np.zeros(n, po.HIT_DTYPE)
+    hits["ipar"] = np.arange(n)
+    hits["d"] = np.arange(n) + 1.0
+    hits["b"] = 0.0
+    off = np.array([0, n], np.int64)
+    rays = np.zeros(1, po.RAY_DTYPE)
+    rays["freq"] = 13.6 * u.U_EV
+    # photons >> atoms: every particle saturates at its neutral content
+    rays["Nph"] = 100 * NH
+    ln, tm, st = po.deposit(orc_h, ps, rays, off, hits)
+    assert st.saturated_deposit_count == n
+    assert ps.yGdepHI == pytest.approx(np.full(n, 0.5), rel=1e-7)
+    assert tm[0] == pytest.approx(100 * NH - n * 0.5 * NH, rel=1e-12)
+    # an opaque first particle stops the ray (TM/Nph < 1e-10, src/loop.c:585)
+    ps2 = chain_system(n, orc_h)
+    ps2.mass[:] = 1e30
+    rays["Nph"] = 1.0
+    ln, tm, st = po.deposit(orc_h, ps2, rays, off, hits)
+    assert ps2.hits.tolist() == [1, 0, 0, 0, 0, 0]
+    assert ln[0] == 1.0
+
+
+def test_merge_first_hit_order():
+    class PS:
+        npar = 10
+    off = np.array([0, 4, 7], np.int64)
+    hits = np.zeros(7, po.HIT_DTYPE)
+    hits["ipar"] = [5, 2, 9, 2, 2, 7, 5]
+    act = po.merge(PS(), off, hits)
+    assert act.tolist() == [5, 2, 9, 7]
