@@ -1,0 +1,132 @@
+"""CPU-side checks of the product library: it loads, exports every symbol
+the header declares, and its host-built tables (rates file parser, cross
+sections, SPH column kernel, secondary ionization) equal the oracle's."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import psphray_amd as psr
+from psphray_amd import _abi
+from oracle import pyoracle as po
+
+from conftest import RATES, ROOT
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "psphray_amd.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(psr_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_header_symbol():
+    lib = _abi.lib()
+    syms = header_symbols()
+    assert len(syms) >= 30
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing
+    # and the ctypes mirror declares every one of them
+    assert set(syms) == set(_abi.SIGNATURES)
+
+
+@pytest.fixture(scope="module")
+def phys():
+    return psr.Physics(RATES, psr.default_units(hmf=0.76),
+                       psr.make_flags(isothermal=False, h_only=False), with_secion=True)
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return po.Oracle(po.units(hmf=0.76), po.flags(isothermal=False, h_only=False))
+
+
+def test_units_equal_oracle(phys, orc):
+    for name, _ in _abi.Units._fields_:
+        assert getattr(phys.units, name) == getattr(orc.u, name), name
+
+
+def test_rate_table_equals_oracle(phys, orc):
+    ot, omin, ostep = orc.ar_table()
+    n = phys.tables.ar_nrows
+    dt = np.ctypeslib.as_array(phys.tables.ar, shape=(16 * n,)).reshape(16, n)
+    assert n == ot.shape[1] == 2251
+    assert np.array_equal(dt, ot)
+    assert phys.tables.ar_min == omin and phys.tables.ar_step_inv == ostep
+    rng = np.random.default_rng(1)
+    for logT in np.concatenate([rng.uniform(-0.5, 9.5, 200), [0.0, 4.0, 9.0, 8.999]]):
+        for col in range(15):
+            assert phys.ar_get(col, logT) == orc.ar_get(col, logT)
+
+
+def test_cross_sections_equal_oracle(phys, orc):
+    u = phys.units
+    for e in np.concatenate([np.linspace(13.0, 1400, 997), [13.6, 24.5, 24.6, 54.4, 54.5]]):
+        for i in range(3):
+            assert phys.xs_get(i, e * u.U_EV) == orc.xs_get(i, e * u.U_EV)
+
+
+def test_kernel_table_matches_reference_table():
+    """the product computes the column kernel by quadrature; it reproduces
+    src/kernel.akline bit for bit in 99% of the entries and to 2.1e-7
+    elsewhere, except entry 1431 where the reference table itself is
+    2.3e-6 off the closed form (the oracle test pins that closed form)"""
+    p = psr.Physics(RATES)
+    mine = p.kernel_table()
+    ref = po.reference_kernel_table()
+    m = ref != 0
+    rel = np.abs(mine[m] - ref[m]) / ref[m]
+    assert rel.max() < 2.5e-6
+    assert np.sort(rel)[-2] < 2.1e-7
+    assert np.mean(mine == ref) > 0.99
+    assert abs(mine[~m]).max() < 1e-12
+    for r in (0.0, 0.3, 0.66, 0.9999, 1.0, -0.1):
+        assert p.sph_depth(r) == (mine[int(4096 * r)] if 0 <= int(4096 * r) < 4096 else 0.0)
+
+
+def test_secondary_ionization_table_equals_oracle(phys, orc):
+    u = phys.units
+    t = phys.tables
+    assert t.si_n1 == 4096 and t.si_n2 == 1024
+    n12 = t.si_n1 * t.si_n2
+    phi = np.ctypeslib.as_array(t.si_phi_hi, shape=(n12,))
+    eh = np.ctypeslib.as_array(t.si_eh, shape=(n12,))
+    assert np.isfinite(phi).all() and np.isfinite(eh).all()
+    # spot values against the oracle's table lookups at grid points
+    xs = np.ctypeslib.as_array(t.si_x, shape=(t.si_n1,))
+    ys = np.ctypeslib.as_array(t.si_y, shape=(t.si_n2,))
+    for i, j in ((1, 1), (100, 500), (4000, 1000), (2048, 3)):
+        assert orc.secion_get(0, xs[i], ys[j]) == pytest.approx(phi[i * t.si_n2 + j], rel=1e-12)
+        assert orc.secion_get(2, xs[i], ys[j]) == pytest.approx(eh[i * t.si_n2 + j], rel=1e-12)
+
+
+def test_rates_parser_errors(tmp_path):
+    bad = tmp_path / "bad.txt"
+    bad.write_text("# nothing here\n")
+    with pytest.raises(psr.PsrError):
+        psr.Physics(str(bad))
+    with pytest.raises(psr.PsrError):
+        psr.Physics(str(tmp_path / "missing.txt"))
+
+
+def test_ray_list_is_deterministic():
+    from psphray_amd.fields import make_workload
+    a = make_workload(2, RATES, n=8, nray=512)
+    b = make_workload(2, RATES, n=8, nray=512)
+    for t in (1, 2, 7):
+        ra, rb = a.rays(t), b.rays(t)
+        assert ra.tobytes() == rb.tobytes()
+    r = a.rays(8)
+    assert np.allclose(np.linalg.norm(r["dir"].astype(np.float64), axis=1), 1, atol=1e-6)
+    # photons split equally among the rays of one source (src/loop.c:305-321)
+    for s in np.unique(r["src_or_par"]):
+        assert np.ptp(r["Nph"][r["src_or_par"] == s]) == 0
+    assert (np.diff(r["src_or_par"]) >= 0).all()
+
+
+def test_philox_known_answer():
+    # Philox4x32-10 known-answer vector (Random123 kat_vectors: zero counter
+    # and key)
+    from psphray_amd.rays import philox4x32
+    out = philox4x32([0], [0], [0], [0], 0, 0)
+    assert [int(x[0]) for x in out] == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
