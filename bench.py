@@ -155,6 +155,7 @@ def main():
         "trace_deposit_intersections_per_s": inter / (td_ms / 1e3) if td_ms > 0 else None,
         "kernel_ms_per_tick": {k: v / K for k, v in tm.items()},
         "active_per_tick": st["active_count"] / K,
+        "deposit_walked_per_tick": st["deposit_processed"] / K,
         "solver_steps_per_update": st["solver_steps"] / max(st["active_count"], 1),
         "roofline": {
             "kernel": "k_trace",

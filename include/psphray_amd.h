@@ -130,6 +130,7 @@ typedef struct psr_stats {
     int64_t solver_failures;
     int64_t surrogate_count;          /* updates taken from the surrogate */
     int64_t surrogate_fallbacks;      /* surrogate rejected by the residual check */
+    int64_t deposit_processed;        /* hits walked by deposit before the cut-off */
     double lost_photon_count_sum;
     double first_ionization_HI, first_ionization_HeI, first_ionization_HeII;
     double secondary_ionization_HI, secondary_ionization_HeI;

@@ -78,7 +78,7 @@ assert STEP_DTYPE.itemsize == 144
 class Stats(C.Structure):
     _fields_ = [(n, _I64) for n in
                 "ticks total_deposit_count saturated_deposit_count active_count solver_steps "
-                "solver_failures surrogate_count surrogate_fallbacks".split()] + \
+                "solver_failures surrogate_count surrogate_fallbacks deposit_processed".split()] + \
                [(n, _D) for n in
                 "lost_photon_count_sum first_ionization_HI first_ionization_HeI "
                 "first_ionization_HeII secondary_ionization_HI secondary_ionization_HeI".split()]

@@ -71,6 +71,19 @@ DevPhys psr_dev_phys(const psr_ctx *c) {
     return p;
 }
 
+int psr_sync_dev_ctx(psr_ctx *c) {
+    if (!c->ctx_dirty && c->d_ctx) return PSR_OK;
+    if (!c->d_ctx) PSR_HIP(c, hipMalloc(&c->d_ctx, sizeof(DevCtx)));
+    DevCtx h;
+    h.S = c->state();
+    h.T = c->T;
+    h.P = psr_dev_phys(c);
+    PSR_HIP(c, hipMemcpyAsync(c->d_ctx, &h, sizeof(DevCtx), hipMemcpyHostToDevice, c->stream));
+    PSR_HIP(c, hipStreamSynchronize(c->stream));
+    c->ctx_dirty = false;
+    return PSR_OK;
+}
+
 int psr_ensure_scratch(psr_ctx *c, size_t bytes) {
     if (bytes <= c->scratch_bytes) return PSR_OK;
     if (c->d_scratch) PSR_HIP(c, hipFree(c->d_scratch));
@@ -144,9 +157,16 @@ extern "C" void psr_destroy(psr_ctx *c) {
     dfree(c->H.b);
     dfree(c->H.slot);
     dfree(c->H.count);
+    dfree(c->H.perm);
+    dfree(c->H.sslot);
+    dfree(c->H.sb);
     dfree(c->d_ray_len);
     dfree(c->d_ray_tm);
     dfree(c->d_stats);
+    dfree(c->d_qlist);
+    dfree(c->d_q);
+    dfree(c->d_saved);
+    dfree(c->d_ctx);
     for (int k = 0; k < 10; k++)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
     if (c->d_scratch) (void)hipFree(c->d_scratch);
@@ -216,6 +236,7 @@ extern "C" int psr_set_physics(psr_ctx *c, const psr_units *u, const psr_flags *
     c->U = *u;
     c->F = *f;
     c->have_physics = true;
+    c->ctx_dirty = true;
     return PSR_OK;
 }
 
@@ -620,6 +641,7 @@ extern "C" int psr_tree_build(psr_ctx *c, const float *pos, const float *sml, co
     cleanup2();
 #undef TB_HIP
     c->have_tree = true;
+    c->ctx_dirty = true;
     return PSR_OK;
 }
 
@@ -725,6 +747,7 @@ extern "C" int psr_state_upload(psr_ctx *c, const psr_state *s) {
 #undef UP
     PSR_HIP(c, hipStreamSynchronize(c->stream));
     c->have_state = true;
+    c->ctx_dirty = true;
     return PSR_OK;
 }
 
@@ -793,6 +816,7 @@ extern "C" int psr_get_stats(psr_ctx *c, psr_stats *out) {
     out->solver_failures = (int64_t)h.solver_failures;
     out->surrogate_count = (int64_t)h.surrogate_count;
     out->surrogate_fallbacks = (int64_t)h.surrogate_fallbacks;
+    out->deposit_processed = (int64_t)h.deposit_processed;
     out->lost_photon_count_sum = h.lost_photon_count_sum;
     out->first_ionization_HI = h.first_ionization_HI;
     out->first_ionization_HeI = h.first_ionization_HeI;

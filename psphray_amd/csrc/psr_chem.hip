@@ -11,6 +11,7 @@
 // the Jacobian is a forward finite difference, as LSODA's.  The x_HI
 // variable and x_HII = 1 - x_HI are both kept in FP64.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 
@@ -212,97 +213,141 @@ __device__ __forceinline__ void lu_solve(const double a[N * N], const int piv[N]
     }
 }
 
-// integrate normalized time 0..1 (src/solve.c:170-224); returns steps taken
-// (negative when t = 1 was not reached)
+// Integrator state of one particle: normalized time 0..1 of
+// src/solve.c:170-224.  attempt() performs one Rosenbrock step attempt (the
+// right-hand side and Jacobian are re-evaluated after every accepted step),
+// so a caller can interleave particles lane by lane.
 template <bool ISO, bool ADI, bool OTS, bool HONLY>
-__device__ int evolve(double x[7], const StepIn &s, const DevTables &T, const DevPhys &P, double rtol) {
+struct Integ {
     using S = Sys<ISO, ADI, OTS, HONLY>;
-    constexpr int N = S::NV;
-    const double atol[7] = {1e-7, 1e-7, 1e-7, 1e-7, 1e-7, 1e-7, 1.};
-    Rates Riso{};
-    if (ISO) rates_at<OTS>(T, log10(s.T), s.nH, Riso);
-    double t = 0.0, h = 1.0;
-    int nsteps = 0;
-    bool rejected = false;
-    double f0[7], f1[7], fp[7], xt[7], J[N * N], M[N * N], k1[N], k2[N];
-    int piv[N];
-    while (t < 1.0 && nsteps < ROS2_MAXSTEPS) {
-        if (h > 1.0 - t) h = 1.0 - t;
-        rhs<ISO, ADI, OTS, HONLY>(x, f0, s, T, P, Riso);
+    static constexpr int N = S::NV;
+    double x[7];
+    StepIn s;
+    Rates Riso;
+    double t, h;
+    bool rejected, need_fj;
+    int nsteps;
+    double f0[7];
+    double J[N * N];
+
+    __device__ void init(const double x0[7], const StepIn &s0, const DevTables &T) {
 #pragma unroll
-        for (int jj = 0; jj < N; jj++) {
-            const int j = S::act(jj);
+        for (int q = 0; q < 7; q++) x[q] = x0[q];
+        s = s0;
+        if (ISO) rates_at<OTS>(T, log10(s.T), s.nH, Riso);
+        t = 0.0;
+        h = 1.0;
+        rejected = false;
+        need_fj = true;
+        nsteps = 0;
+    }
+
+    // returns true when the integration has ended (t = 1 or the step limit)
+    __device__ bool attempt(const DevTables &T, const DevPhys &P, double rtol) {
+        const double atol[7] = {1e-7, 1e-7, 1e-7, 1e-7, 1e-7, 1e-7, 1.};
+        double f1[7], fp[7], xt[7], M[N * N], k1[N], k2[N];
+        int piv[N];
+        if (need_fj) {
+            if (!(t < 1.0 && nsteps < ROS2_MAXSTEPS)) return true;
+            if (h > 1.0 - t) h = 1.0 - t;
+            rhs<ISO, ADI, OTS, HONLY>(x, f0, s, T, P, Riso);
 #pragma unroll
-            for (int q = 0; q < 7; q++) xt[q] = x[q];
-            const double dl = 1.4901161193847656e-08 * fmax(fabs(x[j]), 1e-8);
-            xt[j] = x[j] + dl;
-            const double dlr = xt[j] - x[j];
-            rhs<ISO, ADI, OTS, HONLY>(xt, fp, s, T, P, Riso);
-#pragma unroll
-            for (int ii = 0; ii < N; ii++) J[ii * N + jj] = (fp[S::act(ii)] - f0[S::act(ii)]) / dlr;
-        }
-        for (;;) {
-            nsteps++;
-            const double gh = ROS2_GAMMA * h;
-#pragma unroll
-            for (int ii = 0; ii < N; ii++)
-#pragma unroll
-                for (int jj = 0; jj < N; jj++) M[ii * N + jj] = (ii == jj ? 1.0 : 0.0) - gh * J[ii * N + jj];
-            const bool ok = lu_factor<N>(M, piv);
-            double E = INFINITY;
-            double xn[7];
-            if (ok) {
-#pragma unroll
-                for (int ii = 0; ii < N; ii++) k1[ii] = f0[S::act(ii)];
-                lu_solve<N>(M, piv, k1);
+            for (int jj = 0; jj < N; jj++) {
+                const int j = S::act(jj);
 #pragma unroll
                 for (int q = 0; q < 7; q++) xt[q] = x[q];
+                const double dl = 1.4901161193847656e-08 * fmax(fabs(x[j]), 1e-8);
+                xt[j] = x[j] + dl;
+                const double dlr = xt[j] - x[j];
+                rhs<ISO, ADI, OTS, HONLY>(xt, fp, s, T, P, Riso);
 #pragma unroll
-                for (int ii = 0; ii < N; ii++) xt[S::act(ii)] = x[S::act(ii)] + h * k1[ii];
-                rhs<ISO, ADI, OTS, HONLY>(xt, f1, s, T, P, Riso);
-#pragma unroll
-                for (int ii = 0; ii < N; ii++) k2[ii] = f1[S::act(ii)] - 2.0 * k1[ii];
-                lu_solve<N>(M, piv, k2);
-#pragma unroll
-                for (int q = 0; q < 7; q++) xn[q] = x[q];
-                double acc = 0.0;
-#pragma unroll
-                for (int ii = 0; ii < N; ii++) {
-                    const int i = S::act(ii);
-                    xn[i] = x[i] + 1.5 * h * k1[ii] + 0.5 * h * k2[ii];
-                    const double e = 0.5 * h * (k1[ii] + k2[ii]);
-                    const double sc = atol[i] + rtol * fmax(fabs(x[i]), fabs(xn[i]));
-                    acc += (e / sc) * (e / sc);
-                }
-                E = sqrt(acc / N);
-                if (!(E == E)) E = INFINITY;
+                for (int ii = 0; ii < N; ii++) J[ii * N + jj] = (fp[S::act(ii)] - f0[S::act(ii)]) / dlr;
             }
-            if (E <= 1.0 || h < 1e-14) {
-                t += h;
-#pragma unroll
-                for (int q = 0; q < 7; q++) x[q] = xn[q];
-                double fac = 0.9 / sqrt(fmax(E, 1e-10));
-                fac = fmin(5.0, fmax(0.2, fac));
-                if (rejected) fac = fmin(fac, 1.0);
-                rejected = false;
-                h *= fac;
-                if (1.0 - t < 1e-13) t = 1.0;
-                break;
-            }
-            rejected = true;
-            const double fac = (E == INFINITY) ? 0.2 : fmax(0.2, 0.9 / sqrt(E));
-            h *= fac;
+            need_fj = false;
         }
+        nsteps++;
+        const double gh = ROS2_GAMMA * h;
+#pragma unroll
+        for (int ii = 0; ii < N; ii++)
+#pragma unroll
+            for (int jj = 0; jj < N; jj++) M[ii * N + jj] = (ii == jj ? 1.0 : 0.0) - gh * J[ii * N + jj];
+        const bool ok = lu_factor<N>(M, piv);
+        double E = INFINITY;
+        double xn[7];
+        if (ok) {
+#pragma unroll
+            for (int ii = 0; ii < N; ii++) k1[ii] = f0[S::act(ii)];
+            lu_solve<N>(M, piv, k1);
+#pragma unroll
+            for (int q = 0; q < 7; q++) xt[q] = x[q];
+#pragma unroll
+            for (int ii = 0; ii < N; ii++) xt[S::act(ii)] = x[S::act(ii)] + h * k1[ii];
+            rhs<ISO, ADI, OTS, HONLY>(xt, f1, s, T, P, Riso);
+#pragma unroll
+            for (int ii = 0; ii < N; ii++) k2[ii] = f1[S::act(ii)] - 2.0 * k1[ii];
+            lu_solve<N>(M, piv, k2);
+#pragma unroll
+            for (int q = 0; q < 7; q++) xn[q] = x[q];
+            double acc = 0.0;
+#pragma unroll
+            for (int ii = 0; ii < N; ii++) {
+                const int i = S::act(ii);
+                xn[i] = x[i] + 1.5 * h * k1[ii] + 0.5 * h * k2[ii];
+                const double e = 0.5 * h * (k1[ii] + k2[ii]);
+                const double sc = atol[i] + rtol * fmax(fabs(x[i]), fabs(xn[i]));
+                acc += (e / sc) * (e / sc);
+            }
+            E = sqrt(acc / N);
+            if (!(E == E)) E = INFINITY;
+        }
+        if (E <= 1.0 || h < 1e-14) {
+            t += h;
+#pragma unroll
+            for (int q = 0; q < 7; q++) x[q] = xn[q];
+            double fac = 0.9 / sqrt(fmax(E, 1e-10));
+            fac = fmin(5.0, fmax(0.2, fac));
+            if (rejected) fac = fmin(fac, 1.0);
+            rejected = false;
+            h *= fac;
+            if (1.0 - t < 1e-13) t = 1.0;
+            need_fj = true;
+            return !(t < 1.0 && nsteps < ROS2_MAXSTEPS);
+        }
+        rejected = true;
+        const double fac = (E == INFINITY) ? 0.2 : fmax(0.2, 0.9 / sqrt(E));
+        h *= fac;
+        return false;
     }
+
+    // state after an accepted step: the evolving variables, t, h, steps
+    static constexpr int SAVE_WORDS = N + 3;
+    __device__ void save(double *w) const {
+#pragma unroll
+        for (int jj = 0; jj < N; jj++) w[jj] = x[S::act(jj)];
+        w[N] = t;
+        w[N + 1] = h;
+        w[N + 2] = (double)nsteps;
+    }
+    __device__ void restore(const double *w) {
+#pragma unroll
+        for (int jj = 0; jj < N; jj++) x[S::act(jj)] = w[jj];
+        t = w[N];
+        h = w[N + 1];
+        nsteps = (int)w[N + 2];
+        rejected = false;
+        need_fj = true;
+    }
+
     // clamps of src/solve.c:208-213
-    x[0] = fmin(1., fmax(0., x[0]));
-    x[1] = fmin(1., fmax(0., x[1]));
-    x[2] = fmin(1. - x[1], fmax(0., x[2]));
-    x[3] = fmax(0., x[3]);
-    x[4] = fmax(0., x[4]);
-    x[5] = fmax(0., x[5]);
-    return t >= 1.0 ? nsteps : -nsteps;
-}
+    __device__ void clamp() {
+        x[0] = fmin(1., fmax(0., x[0]));
+        x[1] = fmin(1., fmax(0., x[1]));
+        x[2] = fmin(1. - x[1], fmax(0., x[2]));
+        x[3] = fmax(0., x[3]);
+        x[4] = fmax(0., x[4]);
+        x[5] = fmax(0., x[5]);
+    }
+};
 
 __device__ __forceinline__ double ieye2T(const DevPhys &P, double ie, double ye) {
     return P.U_MPROTON_OVER_C_BOLTZMANN * ie / (P.C_HEMF * 0.25 + P.C_HMF + P.C_HMF * ye) * 2.0 / 3.0;
@@ -314,66 +359,145 @@ __device__ __forceinline__ double lambda_ye(const DevPhys &P, double lH, double 
     return (1.0 - lH) + P.C_HEMF / P.C_HMF * 0.25 * (lHeII + 2 * (1.0 - lHeII - lHeI));
 }
 
-template <bool ISO, bool ADI, bool OTS, bool HONLY>
-__global__ __launch_bounds__(256) void k_update(DevState S, int64_t npar, int64_t tick, double tick_time,
-                                                DevTables T, DevPhys P, DevStats *__restrict__ st) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    bool act = false;
-    int steps = 0, fail = 0;
-    if (i < npar && S.active[i]) {
-        act = true;
-        // Step set-up, src/loop.c:662-711
-        const double lH = S.lH[i], lHeI = S.lHeI[i], lHeII = S.lHeII[i];
-        const double rho = S.rho[i] * P.scaling_fac3_inv;
-        StepIn s;
-        s.time = (tick - S.lasthit[i]) * tick_time;
-        const double ie0 = S.ie[i];
-        s.T = P.fake_temperature > 0 ? P.fake_temperature : ieye2T(P, ie0, lambda_ye(P, lH, lHeI, lHeII));
-        s.yGdepHI = S.yGdepHI[i];
-        s.yGdepHeI = S.yGdepHeI[i];
-        s.yGdepHeII = S.yGdepHeII[i];
-        s.heat = S.heat[i];
-        s.yeMET = S.yeMET[i];
-        s.nH = P.C_H_PER_MASS * rho;
-        double x[7] = {lH, lHeI, lHeII, (double)S.yGrecHII[i], (double)S.yGrecHeII[i],
-                       (double)S.yGrecHeIII[i], ie0};
-        const int ns = evolve<ISO, ADI, OTS, HONLY>(x, s, T, P, 1e-4);
-        steps = ns < 0 ? -ns : ns;
-        fail = ns < 0;
-        // write-back, src/loop.c:745-766 (step_remain = 0 on this path)
-        S.yGdepHI[i] = 0.0f;
-        S.yGdepHeI[i] = 0.0f;
-        S.yGdepHeII[i] = 0.0f;
-        S.yGrecHII[i] = (float)x[3];
-        S.yGrecHeII[i] = (float)x[4];
-        S.yGrecHeIII[i] = (float)x[5];
-        const double nlH = fmax(0.0, fmin(1.0, x[0]));
-        const double nlHeI = fmax(0.0, fmin(1.0, x[1]));
-        const double nlHeII = fmax(0.0, fmin(1.0 - nlHeI, x[2]));
-        S.lH[i] = nlH;
-        S.lHeI[i] = nlHeI;
-        S.lHeII[i] = nlHeII;
-        float ie_new = ADI ? S.ie[i] : (float)x[6];
-        S.lasthit[i] = tick;
-        if (ISO) ie_new = (float)Tye2ie(P, s.T, lambda_ye(P, nlH, nlHeI, nlHeII));
-        S.ie[i] = ie_new;
-        S.heat[i] = 0.0f;
-        S.active[i] = 0;
-    }
-    // counters: one atomic per wavefront
-    const unsigned long long am = __ballot(act);
-    int sv = steps, fv = fail;
+// Step set-up of one particle, src/loop.c:662-711
+__device__ __forceinline__ void load_step(const DevState &S, uint32_t i, int64_t tick, double tick_time,
+                                          const DevPhys &P, double x0[7], StepIn &s) {
+    const double lH = S.lH[i], lHeI = S.lHeI[i], lHeII = S.lHeII[i];
+    const double rho = S.rho[i] * P.scaling_fac3_inv;
+    s.time = (tick - S.lasthit[i]) * tick_time;
+    const double ie0 = S.ie[i];
+    s.T = P.fake_temperature > 0 ? P.fake_temperature : ieye2T(P, ie0, lambda_ye(P, lH, lHeI, lHeII));
+    s.yGdepHI = S.yGdepHI[i];
+    s.yGdepHeI = S.yGdepHeI[i];
+    s.yGdepHeII = S.yGdepHeII[i];
+    s.heat = S.heat[i];
+    s.yeMET = S.yeMET[i];
+    s.nH = P.C_H_PER_MASS * rho;
+    x0[0] = lH;
+    x0[1] = lHeI;
+    x0[2] = lHeII;
+    x0[3] = (double)S.yGrecHII[i];
+    x0[4] = (double)S.yGrecHeII[i];
+    x0[5] = (double)S.yGrecHeIII[i];
+    x0[6] = ie0;
+}
+
+// write-back, src/loop.c:745-766 (step_remain = 0 on this path)
+template <bool ISO, bool ADI>
+__device__ __forceinline__ void store_step(const DevState &S, uint32_t i, int64_t tick, const DevPhys &P,
+                                           const double x[7], double T) {
+    S.yGdepHI[i] = 0.0f;
+    S.yGdepHeI[i] = 0.0f;
+    S.yGdepHeII[i] = 0.0f;
+    S.yGrecHII[i] = (float)x[3];
+    S.yGrecHeII[i] = (float)x[4];
+    S.yGrecHeIII[i] = (float)x[5];
+    const double nlH = fmax(0.0, fmin(1.0, x[0]));
+    const double nlHeI = fmax(0.0, fmin(1.0, x[1]));
+    const double nlHeII = fmax(0.0, fmin(1.0 - nlHeI, x[2]));
+    S.lH[i] = nlH;
+    S.lHeI[i] = nlHeI;
+    S.lHeII[i] = nlHeII;
+    float ie_new = ADI ? S.ie[i] : (float)x[6];
+    S.lasthit[i] = tick;
+    if (ISO) ie_new = (float)Tye2ie(P, T, lambda_ye(P, nlH, nlHeI, nlHeII));
+    S.ie[i] = ie_new;
+    S.heat[i] = 0.0f;
+    S.active[i] = 0;
+}
+
+__device__ __forceinline__ void count_steps(DevStats *st, int steps, int fail, int done) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-        sv += __shfl_xor(sv, o, 64);
-        fv += __shfl_xor(fv, o, 64);
+        steps += __shfl_xor(steps, o, 64);
+        fail += __shfl_xor(fail, o, 64);
+        done += __shfl_xor(done, o, 64);
     }
-    if ((threadIdx.x & 63) == 0 && am) {
-        atomicAdd(&st->active_count, (unsigned long long)__popcll(am));
-        atomicAdd(&st->solver_steps, (unsigned long long)sv);
-        if (fv) atomicAdd(&st->solver_failures, (unsigned long long)fv);
+    if ((threadIdx.x & 63) == 0 && done) {
+        atomicAdd(&st->active_count, (unsigned long long)done);
+        atomicAdd(&st->solver_steps, (unsigned long long)steps);
+        if (fail) atomicAdd(&st->solver_failures, (unsigned long long)fail);
     }
 }
+
+#define PHASE1_ATTEMPTS 4
+
+// Chemistry of the active set, phase 1: one thread per particle in tree
+// order, at most PHASE1_ATTEMPTS step attempts (then up to the next accepted
+// step).  Most particles finish here in one or two steps; the others save
+// their integrator state and are marked active = 2 for phase 2.  The split
+// bounds the wavefront divergence: step counts range from 1 to over 100.
+template <bool ISO, bool ADI, bool OTS, bool HONLY>
+__global__ __launch_bounds__(256) void k_update(DevState S, int64_t npar, int64_t tick, double tick_time,
+                                                DevTables T, DevPhys P, DevStats *__restrict__ st,
+                                                double *__restrict__ saved) {
+    using G = Integ<ISO, ADI, OTS, HONLY>;
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int steps = 0, fail = 0, done = 0;
+    if (i < npar && S.active[i]) {
+        double x0[7];
+        StepIn s;
+        load_step(S, (uint32_t)i, tick, tick_time, P, x0, s);
+        G g;
+        g.init(x0, s, T);
+        int att = 0;
+        bool fin;
+        while (!(fin = g.attempt(T, P, 1e-4))) {
+            if (++att >= PHASE1_ATTEMPTS && g.need_fj) break;
+        }
+        if (fin) {
+            g.clamp();
+            steps = g.nsteps;
+            fail = g.t < 1.0;
+            done = 1;
+            store_step<ISO, ADI>(S, (uint32_t)i, tick, P, g.x, s.T);
+        } else {
+            g.save(saved + (size_t)i * G::SAVE_WORDS);
+            S.active[i] = 2;
+        }
+    }
+    count_steps(st, steps, fail, done);
+}
+
+// phase 2: the particles left over, run to the end (grid-stride over the
+// compacted list whose length the selection wrote to device memory)
+template <bool ISO, bool ADI, bool OTS, bool HONLY>
+__global__ __launch_bounds__(256) void k_update_rest(DevState S, const uint32_t *__restrict__ list,
+                                                     const uint32_t *__restrict__ nlist, int64_t tick,
+                                                     double tick_time, DevTables T, DevPhys P,
+                                                     DevStats *__restrict__ st, const double *__restrict__ saved) {
+    using G = Integ<ISO, ADI, OTS, HONLY>;
+    const uint32_t n = *nlist;
+    int steps = 0, fail = 0, done = 0;
+    // uniform trip count so every lane reaches the wavefront reduction
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t k0 = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t kb = blockIdx.x * blockDim.x; kb < n; kb += stride) {
+        const uint32_t k = kb + (k0 - blockIdx.x * blockDim.x);
+        if (k < n) {
+            const uint32_t i = list[k];
+            double x0[7];
+            StepIn s;
+            load_step(S, i, tick, tick_time, P, x0, s);
+            G g;
+            g.init(x0, s, T);
+            g.restore(saved + (size_t)i * G::SAVE_WORDS);
+            while (!g.attempt(T, P, 1e-4)) {
+            }
+            g.clamp();
+            steps += g.nsteps;
+            fail += g.t < 1.0;
+            done++;
+            store_step<ISO, ADI>(S, i, tick, P, g.x, s.T);
+        }
+    }
+    count_steps(st, steps, fail, done);
+}
+
+struct IsLeftOver {
+    const uint8_t *active;
+    __host__ __device__ bool operator()(uint32_t i) const { return active[i] == 2; }
+};
 
 template <bool ISO, bool ADI, bool OTS, bool HONLY>
 __global__ __launch_bounds__(256) void k_solve(psr_step *__restrict__ steps, int64_t n, DevTables T, DevPhys P,
@@ -390,8 +514,15 @@ __global__ __launch_bounds__(256) void k_solve(psr_step *__restrict__ steps, int
     s.nH = st.nH;
     s.T = st.T;
     s.yeMET = st.yeMET;
-    double x[7] = {st.lambdaH, st.lambdaHeI, st.lambdaHeII, st.yGrecHII, st.yGrecHeII, st.yGrecHeIII, st.ie};
-    const int ns = evolve<ISO, ADI, OTS, HONLY>(x, s, T, P, rtol);
+    const double x0[7] = {st.lambdaH, st.lambdaHeI, st.lambdaHeII, st.yGrecHII, st.yGrecHeII, st.yGrecHeIII,
+                          st.ie};
+    Integ<ISO, ADI, OTS, HONLY> g;
+    g.init(x0, s, T);
+    while (!g.attempt(T, P, rtol)) {
+    }
+    g.clamp();
+    const double *x = g.x;
+    const int ns = g.t >= 1.0 ? g.nsteps : -g.nsteps;
     st.lambdaH = x[0];
     st.lambdaHeI = x[1];
     st.lambdaHeII = x[2];
@@ -430,9 +561,31 @@ int psr_launch_update(psr_ctx *c, int64_t tick, double tick_time, int32_t) {
     if (c->npar == 0) return PSR_OK;
     const DevPhys P = psr_dev_phys(c);
     const DevState S = c->state();
-    const unsigned grid = (unsigned)((c->npar + 255) / 256);
+    const int64_t n = c->npar;
+    if (c->qlist_cap < n) {
+        if (c->d_qlist) PSR_HIP(c, hipFree(c->d_qlist));
+        if (c->d_saved) PSR_HIP(c, hipFree(c->d_saved));
+        c->d_qlist = nullptr;
+        c->d_saved = nullptr;
+        c->qlist_cap = 0;
+        PSR_HIP(c, hipMalloc(&c->d_qlist, sizeof(uint32_t) * n));
+        PSR_HIP(c, hipMalloc(&c->d_saved, sizeof(double) * 10 * n));  // SAVE_WORDS <= 10
+        c->qlist_cap = n;
+    }
+    if (!c->d_q) PSR_HIP(c, hipMalloc(&c->d_q, sizeof(uint32_t) * 2));
     PSR_EV(c, 3, 0);
-    CHEM_DISPATCH(k_update, S, c->npar, tick, tick_time, c->T, P, c->d_stats);
+    unsigned grid = (unsigned)((n + 255) / 256);
+    CHEM_DISPATCH(k_update, S, n, tick, tick_time, c->T, P, c->d_stats, c->d_saved);
+    PSR_HIP(c, hipGetLastError());
+    // leftovers: compact with a device-wide selection (scan based)
+    hipcub::CountingInputIterator<uint32_t> it(0);
+    IsLeftOver pred{c->d_active};
+    size_t tb = 0;
+    PSR_HIP(c, hipcub::DeviceSelect::If(nullptr, tb, it, c->d_qlist, c->d_q, (int)n, pred, c->stream));
+    if (psr_ensure_scratch(c, tb)) return PSR_ERR_HIP;
+    PSR_HIP(c, hipcub::DeviceSelect::If(c->d_scratch, tb, it, c->d_qlist, c->d_q, (int)n, pred, c->stream));
+    grid = (unsigned)std::max(1, c->cu_count * 8);
+    CHEM_DISPATCH(k_update_rest, S, c->d_qlist, c->d_q, tick, tick_time, c->T, P, c->d_stats, c->d_saved);
     PSR_HIP(c, hipGetLastError());
     PSR_EV(c, 3, 1);
     return PSR_OK;

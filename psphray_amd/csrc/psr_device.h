@@ -1,0 +1,74 @@
+// psr_device.h -- device helpers shared by the trace and deposit kernels:
+// table lookups of the reference (src/tabfun.c, src/atomic-rate.c,
+// src/sph-depth.c) and wavefront reductions.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "psr_internal.h"
+
+// clamped linear interpolation of src/tabfun.c:38-58
+__device__ __forceinline__ double tab_lin(const double *xcol, const double *ycol, int nrows, double min,
+                                          double step_inv, double v) {
+    const long long idx = (long long)((v - min) * step_inv);
+    if (idx < 0) return ycol[0];
+    if (idx >= nrows - 1) return ycol[nrows - 1];
+    const double left = xcol[idx], right = xcol[idx + 1];
+    const double lw = (right - v), rw = (v - left);
+    return (lw * ycol[idx] + rw * ycol[idx + 1]) * step_inv;
+}
+
+// cross section with the threshold cut of src/atomic-rate.c:313-318
+__device__ __forceinline__ double xs_get(const DevTables &T, int id, double eng) {
+    if (eng < T.xs_cut[id]) return 0.0;
+    return tab_lin(T.xs, T.xs + (size_t)(id + 1) * T.xs_nrows, T.xs_nrows, T.xs_min, T.xs_step_inv, eng);
+}
+
+// src/sph-depth.c:224-231: nearest-lower entry, 0 outside the table
+__device__ __forceinline__ double sph_depth(const DevTables &T, double r_h) {
+    const int n = (int)(T.kernel_n * r_h);
+    return (n < T.kernel_n && n >= 0) ? T.kernel[n] : 0.0;
+}
+
+// bilinear TabFun2 lookup of src/tabfun.c:173-223
+__device__ inline double tab2_get(const DevTables &T, const double *d, double x, double y) {
+    const int n1 = T.si_n1, n2 = T.si_n2;
+    const int xind = (int)((x - T.si_min1) * T.si_step1_inv);
+    const int yind = (int)((y - T.si_min2) * T.si_step2_inv);
+    int linoffset = -1;
+    if (xind < 0) linoffset = 0;
+    if (xind >= n1 - 1) linoffset = (n1 - 1) * n2;
+    if (linoffset != -1) {
+        if (yind < 0) return d[linoffset];
+        if (yind >= n2 - 1) return d[linoffset + n2 - 1];
+        const double left = T.si_y[yind], right = T.si_y[yind + 1];
+        return ((right - y) * d[linoffset + yind] + (y - left) * d[linoffset + yind + 1]) * T.si_step2_inv;
+    }
+    int lin = -1;
+    if (yind < 0) lin = 0;
+    if (yind >= n2 - 1) lin = n2 - 1;
+    if (lin != -1) {
+        const double left = T.si_x[xind], right = T.si_x[xind + 1];
+        return ((right - x) * d[(size_t)xind * n2 + lin] + (x - left) * d[(size_t)(xind + 1) * n2 + lin]) *
+               T.si_step1_inv;
+    }
+    const double x1 = T.si_x[xind], x2 = T.si_x[xind + 1];
+    const double y1 = T.si_y[yind], y2 = T.si_y[yind + 1];
+    const double fac = T.si_step1_inv * T.si_step2_inv;
+    const double f11 = d[(size_t)xind * n2 + yind];
+    const double f12 = d[(size_t)xind * n2 + yind + 1];
+    const double f21 = d[(size_t)(xind + 1) * n2 + yind];
+    const double f22 = d[(size_t)(xind + 1) * n2 + yind + 1];
+    return fac * (f11 * (x2 - x) * (y2 - y) + f21 * (x - x1) * (y2 - y) + f12 * (x2 - x) * (y - y1) +
+                  f22 * (x - x1) * (y - y1));
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}

@@ -69,12 +69,25 @@ struct DevState {
     uint8_t *active;
 };
 
-// hit slots
+// state, tables and constants as one device-resident block, read by the
+// kernels through a pointer (scalar loads on demand instead of a large
+// by-value argument list)
+struct DevCtx {
+    DevState S;
+    DevTables T;
+    DevPhys P;
+};
+
+// hit slots: trace order (d, b, slot), then the per-ray sort writes the
+// permutation and the slot / b columns in (d, ipar) order
 struct DevHits {
     double *d;
     double *b;
     uint32_t *slot;
     int32_t *count;
+    uint32_t *perm;
+    uint32_t *sslot;
+    double *sb;
     int32_t cap;
 };
 
@@ -87,6 +100,7 @@ struct DevStats {
     unsigned long long solver_failures;
     unsigned long long surrogate_count;
     unsigned long long surrogate_fallbacks;
+    unsigned long long deposit_processed;
     double lost_photon_count_sum;
     double first_ionization_HI, first_ionization_HeI, first_ionization_HeII;
     double secondary_ionization_HI, secondary_ionization_HeI;
@@ -147,9 +161,18 @@ struct psr_ctx {
     int ev_mask = 0;
     double t_ms[5] = {0, 0, 0, 0, 0};
     int64_t ticks = 0;
+    // chemistry phase 2: compact list of unfinished particles + its length,
+    // and their saved integrator state (per particle index)
+    uint32_t *d_qlist = nullptr;
+    uint32_t *d_q = nullptr;
+    double *d_saved = nullptr;
+    int64_t qlist_cap = 0;
     // scratch
     void *d_scratch = nullptr;
     size_t scratch_bytes = 0;
+    // device copy of DevCtx, refreshed by psr_sync_dev_ctx when dirty
+    DevCtx *d_ctx = nullptr;
+    bool ctx_dirty = true;
     // surrogate
     MlpState *mlp = nullptr;
     double mlp_residual = 1e-4;
@@ -164,6 +187,7 @@ int psr_launch_deposit(psr_ctx *c, int64_t ray_first, int64_t nray);
 int psr_launch_update(psr_ctx *c, int64_t tick, double tick_time, int32_t chem_mode);
 int psr_solve_steps_dev(psr_ctx *c, psr_step *d_steps, int64_t n, double rtol);
 DevPhys psr_dev_phys(const psr_ctx *c);
+int psr_sync_dev_ctx(psr_ctx *c);
 int psr_ensure_scratch(psr_ctx *c, size_t bytes);
 int psr_mlp_update_dev(psr_ctx *c, int64_t tick, double tick_time);
 void psr_mlp_free(psr_ctx *c);

@@ -1,5 +1,6 @@
-// psr_trace.hip -- per-tick hot path on gfx950: traversal, per-ray sort,
-// photon deposition (src/raytrace.c:153-230, src/loop.c:375-608).
+// psr_trace.hip -- per-tick hot path on gfx950: traversal and per-ray sort
+// (src/raytrace.c:153-230, src/loop.c:375-386), the tick driver and the
+// parity export; the deposition kernel is in psr_deposit.hip.
 //
 // k_trace   one 256-thread workgroup per ray.  A LIFO stack in LDS holds
 //           the first-child index of hit internal nodes; each step expands
@@ -9,18 +10,15 @@
 //           gets the reference's float Pluecker test and every particle the
 //           reference's float/double impact test, so the hit set is the
 //           reference's exactly.  Hits go to the ray's slot in HBM.
-// k_sort    one workgroup per ray: bitonic sort of (d, ipar) in LDS, writes
-//           the sorted order as a permutation of the slot.
-// k_deposit one wavefront per ray: lanes gather and precompute 64 hits at a
-//           time (tau, expm1); the transmission recurrence TM -= absorb with
-//           the saturation clamp and the 1e-10 cut-off runs lane-serially
-//           over readlane'd values; lanes then scatter with f32 atomics.
+// k_sort    one workgroup per ray: O(n) bucket sort of (d, ipar) in LDS;
+//           writes the permutation and the slot / b columns in that order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstring>
 #include <vector>
 
+#include "psr_device.h"
 #include "psr_internal.h"
 
 #define TR_THREADS 256
@@ -172,55 +170,140 @@ __global__ __launch_bounds__(TR_THREADS) void k_trace(
 }
 
 // ------------------------------------------------------------ sort
-template <int MAXN>
+// (d, ipar) order of one ray's hits (the qsort of src/loop.c:383 with ties
+// broken by particle index).  Bucket sort in LDS: uniform buckets over
+// [dmin, dmax]; the bucket index (int)((d - dmin) * scale) is monotone in d,
+// so buckets are ordered and equal distances share a bucket; each bucket
+// (about 4 hits) is then insertion-sorted.  O(n) LDS work per ray.
+__device__ __forceinline__ bool hit_less(unsigned long long ka, unsigned long long kb, uint32_t ia,
+                                         uint32_t ib, const DevHits &H, int64_t base,
+                                         const uint32_t *__restrict__ orig) {
+    if (ka != kb) return ka < kb;
+    const uint32_t pa = orig[H.slot[base + ia]] & ~PSR_INVALID_BIT;
+    const uint32_t pb = orig[H.slot[base + ib]] & ~PSR_INVALID_BIT;
+    return pa < pb;
+}
+
+__device__ __forceinline__ double block_reduce(double v, bool is_max, double *red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_xor(v, o, 64);
+        v = is_max ? fmax(v, w) : fmin(v, w);
+    }
+    const int wid = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[wid] = v;
+    __syncthreads();
+    double r = red[0];
+    for (int k = 1; k < (int)(blockDim.x >> 6); k++) r = is_max ? fmax(r, red[k]) : fmin(r, red[k]);
+    return r;
+}
+
+template <int MAXN, int MAXB>
 __global__ __launch_bounds__(256) void k_sort(int64_t nray, DevHits H, const uint32_t *__restrict__ orig,
-                                              uint32_t *__restrict__ perm, int32_t lo, int32_t hi) {
+                                              int32_t lo, int32_t hi) {
     __shared__ unsigned long long key[MAXN];
-    __shared__ uint32_t sec[MAXN];
-    __shared__ uint16_t pos[MAXN];
+    __shared__ uint16_t idx[MAXN];
+    __shared__ int32_t start[MAXB];
+    __shared__ int32_t fill[MAXB];
+    __shared__ double red[4];
+    __shared__ int32_t wsum[4];
     const int tid = threadIdx.x;
     for (int64_t r = blockIdx.x; r < nray; r += gridDim.x) {
         const int n = H.count[r];
         if (n <= lo || n > hi) continue;
         const int64_t base = r * (int64_t)H.cap;
-        int P = 1;
-        while (P < n) P <<= 1;
-        for (int i = tid; i < P; i += 256) {
-            if (i < n) {
-                key[i] = (unsigned long long)__double_as_longlong(H.d[base + i]);
-                sec[i] = orig[H.slot[base + i]] & ~PSR_INVALID_BIT;
-            } else {
-                key[i] = ~0ull;
-                sec[i] = ~0u;
-            }
-            pos[i] = (uint16_t)i;
+        double mn = INFINITY, mx = -INFINITY;
+        for (int i = tid; i < n; i += 256) {
+            const double d = H.d[base + i];
+            mn = fmin(mn, d);
+            mx = fmax(mx, d);
+        }
+        const double dmin = block_reduce(mn, false, red);
+        const double dmax = block_reduce(mx, true, red);
+        int B = 1;
+        while (B * 4 < n && B < MAXB) B <<= 1;
+        const double scale = dmax > dmin ? (double)B / (dmax - dmin) : 0.0;
+        for (int b = tid; b < B; b += 256) {
+            start[b] = 0;
+            fill[b] = 0;
         }
         __syncthreads();
-        for (int k = 2; k <= P; k <<= 1) {
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                for (int i = tid; i < P; i += 256) {
-                    const int ixj = i ^ j;
-                    if (ixj > i) {
-                        const unsigned long long ka = key[i], kb = key[ixj];
-                        const uint32_t sa = sec[i], sb = sec[ixj];
-                        const bool gt = ka > kb || (ka == kb && sa > sb);
-                        const bool up = (i & k) == 0;
-                        if (gt == up) {
-                            key[i] = kb;
-                            key[ixj] = ka;
-                            sec[i] = sb;
-                            sec[ixj] = sa;
-                            const uint16_t t = pos[i];
-                            pos[i] = pos[ixj];
-                            pos[ixj] = t;
-                        }
-                    }
+        for (int i = tid; i < n; i += 256) {
+            const double d = H.d[base + i];
+            const int b = min(B - 1, (int)((d - dmin) * scale));
+            atomicAdd(&start[b], 1);
+        }
+        __syncthreads();
+        // exclusive scan of the counts: each thread owns a contiguous run
+        {
+            const int per = (B + 255) / 256;
+            const int b0 = tid * per, b1 = min(B, b0 + per);
+            int s = 0;
+            for (int b = b0; b < b1; b++) s += start[b];
+            int incl = s;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int v = __shfl_up(incl, o, 64);
+                if ((tid & 63) >= o) incl += v;
+            }
+            if ((tid & 63) == 63) wsum[tid >> 6] = incl;
+            __syncthreads();
+            int off = 0;
+            for (int w = 0; w < (tid >> 6); w++) off += wsum[w];
+            int run = off + incl - s;
+            for (int b = b0; b < b1; b++) {
+                const int c = start[b];
+                start[b] = run;
+                run += c;
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < n; i += 256) {
+            const double d = H.d[base + i];
+            const int b = min(B - 1, (int)((d - dmin) * scale));
+            const int p = start[b] + atomicAdd(&fill[b], 1);
+            key[p] = (unsigned long long)__double_as_longlong(d);
+            idx[p] = (uint16_t)i;
+        }
+        __syncthreads();
+        for (int b = tid; b < B; b += 256) {
+            const int s0 = start[b], e0 = s0 + fill[b];
+            for (int j = s0 + 1; j < e0; j++) {
+                const unsigned long long kj = key[j];
+                const uint16_t ij = idx[j];
+                int q = j - 1;
+                while (q >= s0 && hit_less(kj, key[q], ij, idx[q], H, base, orig)) {
+                    key[q + 1] = key[q];
+                    idx[q + 1] = idx[q];
+                    q--;
                 }
-                __syncthreads();
+                key[q + 1] = kj;
+                idx[q + 1] = ij;
             }
         }
-        for (int i = tid; i < n; i += 256) perm[base + i] = pos[i];
         __syncthreads();
+        for (int i = tid; i < n; i += 256) {
+            const uint32_t q = idx[i];
+            H.perm[base + i] = q;
+            H.sslot[base + i] = H.slot[base + q];
+            H.sb[base + i] = H.b[base + q];
+        }
+        __syncthreads();
+    }
+}
+
+// sorted slot / b columns from a permutation (rays of the global fallback)
+__global__ void k_apply_perm(const int32_t *__restrict__ rays_big, int32_t nbig, DevHits H) {
+    for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
+        const int64_t r = rays_big[b];
+        const int n = H.count[r];
+        const int64_t base = r * (int64_t)H.cap;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            const uint32_t q = H.perm[base + i];
+            H.sslot[base + i] = H.slot[base + q];
+            H.sb[base + i] = H.b[base + q];
+        }
     }
 }
 
@@ -228,7 +311,6 @@ __global__ __launch_bounds__(256) void k_sort(int64_t nray, DevHits H, const uin
 // workgroup per listed ray, scratch of P entries per ray
 __global__ __launch_bounds__(256) void k_sort_global(const int32_t *__restrict__ rays_big, int32_t nbig,
                                                      DevHits H, const uint32_t *__restrict__ orig,
-                                                     uint32_t *__restrict__ perm,
                                                      unsigned long long *__restrict__ gkey,
                                                      uint32_t *__restrict__ gsec, int32_t pmax) {
     const int tid = threadIdx.x;
@@ -238,7 +320,7 @@ __global__ __launch_bounds__(256) void k_sort_global(const int32_t *__restrict__
         const int64_t base = r * (int64_t)H.cap;
         unsigned long long *key = gkey + (int64_t)b * pmax;
         uint32_t *sec = gsec + (int64_t)b * pmax;
-        uint32_t *pos = perm + base;  // sorted in place over a padded temp
+        uint32_t *pos = H.perm + base;
         int P = 1;
         while (P < n) P <<= 1;
         // pos needs P entries; the slot holds cap >= n; pad lives in gsec's upper half
@@ -283,277 +365,6 @@ __global__ __launch_bounds__(256) void k_sort_global(const int32_t *__restrict__
     }
 }
 
-// ------------------------------------------------------------ tables
-// clamped linear interpolation of src/tabfun.c:38-58
-__device__ __forceinline__ double tab_lin(const double *xcol, const double *ycol, int nrows,
-                                          double min, double step_inv, double v) {
-    const long long idx = (long long)((v - min) * step_inv);
-    if (idx < 0) return ycol[0];
-    if (idx >= nrows - 1) return ycol[nrows - 1];
-    const double left = xcol[idx], right = xcol[idx + 1];
-    const double lw = (right - v), rw = (v - left);
-    return (lw * ycol[idx] + rw * ycol[idx + 1]) * step_inv;
-}
-
-__device__ __forceinline__ double xs_get(const DevTables &T, int id, double eng) {
-    if (eng < T.xs_cut[id]) return 0.0;
-    return tab_lin(T.xs, T.xs + (size_t)(id + 1) * T.xs_nrows, T.xs_nrows, T.xs_min, T.xs_step_inv, eng);
-}
-
-// src/sph-depth.c:224-231
-__device__ __forceinline__ double sph_depth(const DevTables &T, double r_h) {
-    const int n = (int)(T.kernel_n * r_h);
-    return (n < T.kernel_n && n >= 0) ? T.kernel[n] : 0.0;
-}
-
-// bilinear TabFun2 lookup of src/tabfun.c:173-223
-__device__ double tab2_get(const DevTables &T, const double *d, double x, double y) {
-    const int n1 = T.si_n1, n2 = T.si_n2;
-    const int xind = (int)((x - T.si_min1) * T.si_step1_inv);
-    const int yind = (int)((y - T.si_min2) * T.si_step2_inv);
-    int linoffset = -1;
-    if (xind < 0) linoffset = 0;
-    if (xind >= n1 - 1) linoffset = (n1 - 1) * n2;
-    if (linoffset != -1) {
-        if (yind < 0) return d[linoffset];
-        if (yind >= n2 - 1) return d[linoffset + n2 - 1];
-        const double left = T.si_y[yind], right = T.si_y[yind + 1];
-        return ((right - y) * d[linoffset + yind] + (y - left) * d[linoffset + yind + 1]) * T.si_step2_inv;
-    }
-    int lin = -1;
-    if (yind < 0) lin = 0;
-    if (yind >= n2 - 1) lin = n2 - 1;
-    if (lin != -1) {
-        const double left = T.si_x[xind], right = T.si_x[xind + 1];
-        return ((right - x) * d[(size_t)xind * n2 + lin] + (x - left) * d[(size_t)(xind + 1) * n2 + lin]) *
-               T.si_step1_inv;
-    }
-    const double x1 = T.si_x[xind], x2 = T.si_x[xind + 1];
-    const double y1 = T.si_y[yind], y2 = T.si_y[yind + 1];
-    const double fac = T.si_step1_inv * T.si_step2_inv;
-    const double f11 = d[(size_t)xind * n2 + yind];
-    const double f12 = d[(size_t)xind * n2 + yind + 1];
-    const double f21 = d[(size_t)(xind + 1) * n2 + yind];
-    const double f22 = d[(size_t)(xind + 1) * n2 + yind + 1];
-    return fac * (f11 * (x2 - x) * (y2 - y) + f21 * (x - x1) * (y2 - y) + f12 * (x2 - x) * (y - y1) +
-                  f22 * (x - x1) * (y - y1));
-}
-
-// ------------------------------------------------------------ deposit
-__device__ __forceinline__ double readlane_f64(double v, int lane) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ __forceinline__ int wave_sum_i(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-template <bool HE>
-__global__ __launch_bounds__(256) void k_deposit(const psr_ray *__restrict__ rays, int64_t nray,
-                                                 DevHits H, const uint32_t *__restrict__ perm,
-                                                 DevState S, DevTables T, DevPhys P,
-                                                 DevStats *__restrict__ st, double *__restrict__ ray_len,
-                                                 double *__restrict__ ray_tm) {
-    const int lane = threadIdx.x & 63;
-    const int64_t wave0 = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int64_t nwaves = gridDim.x * (int64_t)(blockDim.x >> 6);
-    for (int64_t r = wave0; r < nray; r += nwaves) {
-        const psr_ray R = rays[r];
-        const int n = H.count[r];
-        if (lane == 0) {
-            ray_len[r] = R.length;
-            ray_tm[r] = R.Nph;
-        }
-        if (n == 0 || R.Nph == 0) continue;
-        const int64_t base = r * (int64_t)H.cap;
-        double TM = R.Nph;
-        if (P.no_photon && R.type < 0) TM = 0.0;
-        const double Nph = R.Nph;
-        const double freq = R.freq;
-        const double sigmaHI = xs_get(T, 0, freq);
-        const double sigmaHeI = HE ? xs_get(T, 1, freq) : 0.0;
-        const double sigmaHeII = HE ? xs_get(T, 2, freq) : 0.0;
-        const double dEHI = fdim(freq, P.C_HI_ENERGY);
-        const double dEHeI = fdim(freq, P.C_HEI_ENERGY);
-        const double dEHeII = fdim(freq, P.C_HEII_ENERGY);
-        const double hf_HI = P.C_H_PER_MASS * dEHI;
-        const double hf_HeI = P.C_HE_PER_MASS * dEHeI;
-        const double hf_HeII = P.C_HE_PER_MASS * dEHeII;
-        const bool secion_terms = P.secondary_ionization && (dEHI > 0 || dEHeI > 0) && T.si_n1 > 0;
-        int last = n - 1;
-        bool done = false;
-        int nsat = 0;
-        double fiHI = 0, fiHeI = 0, fiHeII = 0, siHI = 0, siHeI = 0;
-        for (int c0 = 0; c0 < n && !done; c0 += 64) {
-            const int i = c0 + lane;
-            const bool valid = i < n;
-            uint32_t slot = 0;
-            double b = 0, sml = 1, mass = 0, xHI = 0, xHeI = 0, xHeII = 0;
-            if (valid) {
-                const uint32_t q = perm[base + i];
-                slot = H.slot[base + q];
-                b = H.b[base + q];
-                sml = S.ps[slot].w;
-                mass = S.mass[slot];
-                xHI = S.lH[slot];
-                xHeI = S.lHeI[slot];
-                xHeII = S.lHeII[slot];
-            }
-            // src/loop.c:427-477
-            const double sml_inv = 1.0 / sml;
-            const double NH = mass * P.C_H_PER_MASS;
-            const double NHe = mass * P.C_HE_PER_MASS;
-            const double NH_inv = 1.0 / NH;
-            const double NHe_inv = 1.0 / NHe;
-            const double kernel = sph_depth(T, b * sml_inv) * (sml_inv * sml_inv) * P.scaling_fac2_inv;
-            const double NHI = xHI * NH;
-            const double tauHI = sigmaHI * (kernel * NHI);
-            double tausum = tauHI;
-            double tauHeI = 0, tauHeII = 0;
-            const bool he = HE && NHe != 0.0;
-            if (he) {
-                tauHeI = sigmaHeI * (kernel * (xHeI * NHe));
-                tausum += tauHeI;
-                tauHeII = sigmaHeII * (kernel * (xHeII * NHe));
-                tausum += tauHeII;
-            }
-            const bool skip = !valid || tausum == 0.0;
-            const double em = skip ? 0.0 : expm1(-tausum);
-            const double fHI = skip ? 0.0 : tauHI / tausum;
-            const double fHeI = (skip || !he) ? 0.0 : tauHeI / tausum;
-            const double fHeII = (skip || !he) ? 0.0 : tauHeII / tausum;
-            const double NHeI = xHeI * NHe, NHeII = xHeII * NHe;
-            const unsigned long long skipmask = __ballot(skip);
-            const unsigned long long hemask = __ballot(he);
-            const int m = min(64, n - c0);
-            double my_aHI = 0, my_aHeI = 0, my_aHeII = 0;
-            bool mine = false;
-            // transmission recurrence, src/loop.c:479-504, 580-589
-            for (int e = 0; e < m; e++) {
-                if ((skipmask >> e) & 1ull) continue;
-                const double absorb_est = -TM * readlane_f64(em, e);
-                double aHI = readlane_f64(fHI, e) * absorb_est;
-                const double nhi = readlane_f64(NHI, e);
-                if (aHI > nhi) {
-                    aHI = nhi;
-                    nsat++;
-                }
-                double absorb = aHI;
-                double aHeI = 0, aHeII = 0;
-                if (HE && ((hemask >> e) & 1ull)) {
-                    aHeI = readlane_f64(fHeI, e) * absorb_est;
-                    const double n1 = readlane_f64(NHeI, e);
-                    if (aHeI > n1) {
-                        aHeI = n1;
-                        nsat++;
-                    }
-                    aHeII = readlane_f64(fHeII, e) * absorb_est;
-                    const double n2 = readlane_f64(NHeII, e);
-                    if (aHeII > n2) {
-                        aHeII = n2;
-                        nsat++;
-                    }
-                    absorb += (aHeI + aHeII);
-                }
-                if (lane == e) {
-                    my_aHI = aHI;
-                    my_aHeI = aHeI;
-                    my_aHeII = aHeII;
-                    mine = true;
-                }
-                TM -= absorb;
-                if (TM < 0.0 || TM / Nph < 1e-10) {
-                    last = c0 + e;
-                    done = true;
-                    break;
-                }
-            }
-            if (mine) {
-                // src/loop.c:486-573
-                const double deltaHI = my_aHI * NH_inv;
-                const double deltaHeI = he ? my_aHeI * NHe_inv : 0.0;
-                const double deltaHeII = he ? my_aHeII * NHe_inv : 0.0;
-                const double ye = (1.0 - xHI) + P.C_HEMF / P.C_HMF * 0.25 * (xHeII + 2 * (1.0 - xHeII - xHeI));
-                double x = ye * NH / (NH + NHe);
-                if (x > 1.0) x = 1.0;
-                if (P.secondary_ionization && x > 0.0) {
-                    atomicAdd(&S.yGdepHI[slot], (float)deltaHI);
-                    fiHI += deltaHI * NH;
-                    if (secion_terms) {
-                        const double log10x = log10(x);
-                        const double sec =
-                            deltaHI * dEHI / P.C_HI_ENERGY * tab2_get(T, T.si_phi_hi, dEHI, log10x) +
-                            deltaHeI * dEHeI / P.C_HI_ENERGY * tab2_get(T, T.si_phi_hi, dEHeI, log10x);
-                        atomicAdd(&S.yGdepHI[slot], (float)sec);
-                        siHI += sec * NH;
-                        atomicAdd(&S.heat[slot], (float)(hf_HI * deltaHI * tab2_get(T, T.si_eh, dEHI, log10x)));
-                        if (he) {
-                            const double sec2 =
-                                deltaHI * dEHI / P.C_HEI_ENERGY * tab2_get(T, T.si_phi_hei, dEHI, log10x) +
-                                deltaHeI * dEHeI / P.C_HEI_ENERGY * tab2_get(T, T.si_phi_hei, dEHeI, log10x);
-                            atomicAdd(&S.yGdepHeI[slot], (float)(deltaHeI + sec2));
-                            siHeI += sec2 * NHe;
-                            fiHeI += deltaHeI * NHe;
-                            atomicAdd(&S.heat[slot],
-                                      (float)(hf_HeI * deltaHeI * tab2_get(T, T.si_eh, dEHeI, log10x)));
-                            atomicAdd(&S.yGdepHeII[slot], (float)deltaHeII);
-                            fiHeII += deltaHeII * NHe;
-                        }
-                    } else if (he) {
-                        atomicAdd(&S.yGdepHeI[slot], (float)deltaHeI);
-                        fiHeI += deltaHeI * NHe;
-                        atomicAdd(&S.yGdepHeII[slot], (float)deltaHeII);
-                        fiHeII += deltaHeII * NHe;
-                    }
-                } else {
-                    atomicAdd(&S.yGdepHI[slot], (float)deltaHI);
-                    if (hf_HI != 0.0) atomicAdd(&S.heat[slot], (float)(hf_HI * deltaHI));
-                    if (he) {
-                        atomicAdd(&S.yGdepHeI[slot], (float)deltaHeI);
-                        atomicAdd(&S.yGdepHeII[slot], (float)deltaHeII);
-                        if (hf_HeI != 0.0) atomicAdd(&S.heat[slot], (float)(deltaHeI * hf_HeI));
-                        if (hf_HeII != 0.0) atomicAdd(&S.heat[slot], (float)(deltaHeII * hf_HeII));
-                    }
-                }
-                atomicAdd(&S.hits[slot], 1);
-            }
-        }
-        // ray results and counters
-        const double dl = H.d[base + perm[base + last]];
-        fiHI = wave_sum(fiHI);
-        siHI = wave_sum(siHI);
-        if (HE) {
-            fiHeI = wave_sum(fiHeI);
-            fiHeII = wave_sum(fiHeII);
-            siHeI = wave_sum(siHeI);
-        }
-        if (lane == 0) {
-            ray_len[r] = dl;
-            ray_tm[r] = TM;
-            atomicAdd(&st->total_deposit_count, (unsigned long long)n);
-            if (nsat) atomicAdd(&st->saturated_deposit_count, (unsigned long long)nsat);
-            atomicAdd(&st->lost_photon_count_sum, TM);
-            atomicAdd(&st->first_ionization_HI, fiHI);
-            atomicAdd(&st->secondary_ionization_HI, siHI);
-            if (HE) {
-                atomicAdd(&st->first_ionization_HeI, fiHeI);
-                atomicAdd(&st->first_ionization_HeII, fiHeII);
-                atomicAdd(&st->secondary_ionization_HeI, siHeI);
-            }
-        }
-    }
-}
-
 // total_deposit_count when deposit is skipped (traceOnly)
 __global__ void k_count_hits(const int32_t *__restrict__ count, int64_t nray, DevStats *st) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -587,6 +398,9 @@ static int ensure_hits(psr_ctx *c, int64_t nray, int32_t cap) {
     if (c->H.b) (void)hipFree(c->H.b);
     if (c->H.slot) (void)hipFree(c->H.slot);
     if (c->H.count) (void)hipFree(c->H.count);
+    if (c->H.perm) (void)hipFree(c->H.perm);
+    if (c->H.sslot) (void)hipFree(c->H.sslot);
+    if (c->H.sb) (void)hipFree(c->H.sb);
     if (c->d_ray_len) (void)hipFree(c->d_ray_len);
     if (c->d_ray_tm) (void)hipFree(c->d_ray_tm);
     c->H = DevHits{};
@@ -595,8 +409,10 @@ static int ensure_hits(psr_ctx *c, int64_t nray, int32_t cap) {
     const size_t tot = (size_t)ns * cp;
     PSR_HIP(c, hipMalloc(&c->H.d, sizeof(double) * tot));
     PSR_HIP(c, hipMalloc(&c->H.b, sizeof(double) * tot));
-    // slot array followed by the sort permutation (same size)
-    PSR_HIP(c, hipMalloc(&c->H.slot, sizeof(uint32_t) * tot * 2));
+    PSR_HIP(c, hipMalloc(&c->H.slot, sizeof(uint32_t) * tot));
+    PSR_HIP(c, hipMalloc(&c->H.perm, sizeof(uint32_t) * tot));
+    PSR_HIP(c, hipMalloc(&c->H.sslot, sizeof(uint32_t) * tot));
+    PSR_HIP(c, hipMalloc(&c->H.sb, sizeof(double) * tot));
     PSR_HIP(c, hipMalloc(&c->H.count, sizeof(int32_t) * ns));
     PSR_HIP(c, hipMalloc(&c->d_ray_len, sizeof(double) * ns));
     PSR_HIP(c, hipMalloc(&c->d_ray_tm, sizeof(double) * ns));
@@ -604,8 +420,6 @@ static int ensure_hits(psr_ctx *c, int64_t nray, int32_t cap) {
     c->hit_slots = ns;
     return PSR_OK;
 }
-
-static inline uint32_t *perm_of(psr_ctx *c) { return c->H.slot + (size_t)c->hit_slots * c->H.cap; }
 
 static int read_flags(psr_ctx *c, int32_t *flags) {
     PSR_HIP(c, hipMemcpyAsync(flags, &c->d_stats->flags, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
@@ -652,13 +466,17 @@ int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray) {
 }
 
 int psr_launch_sort(psr_ctx *c, int64_t nray) {
-    uint32_t *perm = perm_of(c);
     const unsigned grid = (unsigned)std::min<int64_t>(nray, 65535 * 4);
     PSR_EV(c, 1, 0);
-    k_sort<2048><<<grid, 256, 0, c->stream>>>(nray, c->H, c->d_orig, perm, -1, 2048);
+    // LDS tiers by list length: 10 B per hit + 8 B per bucket
+    k_sort<1024, 256><<<grid, 256, 0, c->stream>>>(nray, c->H, c->d_orig, -1, 1024);
     PSR_HIP(c, hipGetLastError());
-    if (c->H.cap > 2048) {
-        k_sort<8192><<<grid, 256, 0, c->stream>>>(nray, c->H, c->d_orig, perm, 2048, 8192);
+    if (c->H.cap > 1024) {
+        k_sort<4096, 1024><<<grid, 256, 0, c->stream>>>(nray, c->H, c->d_orig, 1024, 4096);
+        PSR_HIP(c, hipGetLastError());
+    }
+    if (c->H.cap > 4096) {
+        k_sort<8192, 2048><<<grid, 256, 0, c->stream>>>(nray, c->H, c->d_orig, 4096, 8192);
         PSR_HIP(c, hipGetLastError());
     }
     if (c->H.cap > 8192) {
@@ -683,29 +501,15 @@ int psr_launch_sort(psr_ctx *c, int64_t nray) {
             uint32_t *gsec = (uint32_t *)(gkey + (size_t)nb * pmax);
             PSR_HIP(c, hipMemcpyAsync(d_big, big.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, c->stream));
             PSR_HIP(c, hipStreamSynchronize(c->stream));  // 'big' is released on return
-            k_sort_global<<<std::min(nb, 1024), 256, 0, c->stream>>>(d_big, nb, c->H, c->d_orig, perm, gkey,
-                                                                     gsec, pmax);
+            k_sort_global<<<std::min(nb, 1024), 256, 0, c->stream>>>(d_big, nb, c->H, c->d_orig, gkey, gsec,
+                                                                     pmax);
             PSR_HIP(c, hipGetLastError());
+            k_apply_perm<<<std::min(nb, 1024), 256, 0, c->stream>>>(d_big, nb, c->H);
+            PSR_HIP(c, hipGetLastError());
+            PSR_HIP(c, hipStreamSynchronize(c->stream));  // scratch is reused by the next call
         }
     }
     PSR_EV(c, 1, 1);
-    return PSR_OK;
-}
-
-int psr_launch_deposit(psr_ctx *c, int64_t ray_first, int64_t nray) {
-    const DevPhys P = psr_dev_phys(c);
-    const DevState S = c->state();
-    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nray + 3) / 4, 65535 * 4));
-    const bool he = !c->F.h_only;
-    PSR_EV(c, 2, 0);
-    if (he)
-        k_deposit<true><<<grid, 256, 0, c->stream>>>(c->d_rays + ray_first, nray, c->H, perm_of(c), S, c->T,
-                                                     P, c->d_stats, c->d_ray_len, c->d_ray_tm);
-    else
-        k_deposit<false><<<grid, 256, 0, c->stream>>>(c->d_rays + ray_first, nray, c->H, perm_of(c), S, c->T,
-                                                      P, c->d_stats, c->d_ray_len, c->d_ray_tm);
-    PSR_HIP(c, hipGetLastError());
-    PSR_EV(c, 2, 1);
     return PSR_OK;
 }
 
@@ -756,7 +560,7 @@ extern "C" int psr_trace_lists(psr_ctx *c, int64_t ray_first, int64_t nray, int6
     }
     hipError_t e = hipMemcpyAsync(d_off, offsets, sizeof(int64_t) * (nray + 1), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
-        k_export_lists<<<(unsigned)nray, 256, 0, c->stream>>>(nray, c->H, perm_of(c), c->d_orig, d_off, d_i, d_d, d_b);
+        k_export_lists<<<(unsigned)nray, 256, 0, c->stream>>>(nray, c->H, c->H.perm, c->d_orig, d_off, d_i, d_d, d_b);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(ipar, d_i, sizeof(int64_t) * total, hipMemcpyDeviceToHost, c->stream);

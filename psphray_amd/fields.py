@@ -34,7 +34,7 @@ def grf(n, slope, rng):
     amp = np.zeros_like(kk)
     amp[kk > 0] = kk[kk > 0] ** (slope / 2.0)
     noise = np.fft.rfftn(rng.standard_normal((n, n, n)))
-    g = np.fft.irfftn(noise * amp, s=(n, n, n))
+    g = np.fft.irfftn(noise * amp, s=(n, n, n), axes=(0, 1, 2))
     g -= g.mean()
     return g / g.std()
 
@@ -50,7 +50,7 @@ def zeldovich_displacement(n, slope, rng):
     dk = np.fft.rfftn(rng.standard_normal((n, n, n))) * amp
     inv = np.zeros_like(kk2)
     inv[kk2 > 0] = 1.0 / kk2[kk2 > 0]
-    psi = [np.fft.irfftn(1j * kv * inv * dk, s=(n, n, n)) for kv in (kx, ky, kzz)]
+    psi = [np.fft.irfftn(1j * kv * inv * dk, s=(n, n, n), axes=(0, 1, 2)) for kv in (kx, ky, kzz)]
     return np.stack(psi, axis=-1)
 
 
