@@ -12,16 +12,22 @@ HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
 HIPFLAGS=(--offload-arch=gfx950 -O3 -std=c++17 -fPIC -ffp-contract=off -munsafe-fp-atomics
           -Xclang -target-feature -Xclang -packed-fp32-ops
           -I"$ROOT/include" -I"$HERE/csrc" -Wno-unused-result)
+# PSR_FORCE=1 recompiles every object; otherwise only stale ones
+FORCE="${PSR_FORCE:-0}"
 pids=()
 for src in "$HERE"/csrc/*.hip; do
   obj="$OUT/$(basename "${src%.hip}").o"
-  if [[ ! -f "$obj" || "$src" -nt "$obj" || "$HERE/csrc/psr_internal.h" -nt "$obj" || "$ROOT/include/psphray_amd.h" -nt "$obj" ]]; then
+  stale=0
+  for dep in "$src" "$HERE"/csrc/*.h "$ROOT/include/psphray_amd.h"; do
+    [[ "$dep" -nt "$obj" ]] && stale=1
+  done
+  if [[ "$FORCE" == 1 || ! -f "$obj" || "$stale" == 1 ]]; then
     "$HIPCC" "${HIPFLAGS[@]}" -c "$src" -o "$obj" &
     pids+=($!)
   fi
 done
 gcc -std=gnu99 -O2 -fPIC -ffp-contract=off -I"$ROOT/include" -c "$HERE/host/psr_host.c" -o "$OUT/psr_host.o" &
 pids+=($!)
-for p in "${pids[@]}"; do wait "$p"; done
+for p in "${pids[@]}"; do wait "$p" || { echo "build.sh: a compile step failed" >&2; exit 1; }; done
 "$HIPCC" --offload-arch=gfx950 -shared -fPIC -o "$HERE/libpsphray_amd.so" "$OUT"/*.o -lm
 make -s -C "$ROOT/oracle" all

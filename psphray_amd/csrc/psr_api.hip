@@ -116,8 +116,8 @@ extern "C" int psr_create(psr_ctx **out, int device) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
-    if (hipMalloc(&c->d_stats, sizeof(DevStats)) != hipSuccess ||
-        hipMemset(c->d_stats, 0, sizeof(DevStats)) != hipSuccess) {
+    if (hipMalloc(&c->d_stats, sizeof(DevStats) * PSR_NSTAT) != hipSuccess ||
+        hipMemset(c->d_stats, 0, sizeof(DevStats) * PSR_NSTAT) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
         delete c;
         return PSR_ERR_HIP;
@@ -804,9 +804,27 @@ extern "C" int psr_rays_upload(psr_ctx *c, const psr_ray *rays, int64_t nray) {
 // ------------------------------------------------------------ stats
 extern "C" int psr_get_stats(psr_ctx *c, psr_stats *out) {
     if (!c || !out) return PSR_ERR_ARG;
-    DevStats h;
+    std::vector<DevStats> hs(PSR_NSTAT);
     PSR_HIP(c, hipStreamSynchronize(c->stream));
-    PSR_HIP(c, hipMemcpy(&h, c->d_stats, sizeof(h), hipMemcpyDeviceToHost));
+    PSR_HIP(c, hipMemcpy(hs.data(), c->d_stats, sizeof(DevStats) * PSR_NSTAT, hipMemcpyDeviceToHost));
+    DevStats h = hs[0];
+    for (int k = 1; k < PSR_NSTAT; k++) {
+        const DevStats &o = hs[k];
+        h.total_deposit_count += o.total_deposit_count;
+        h.saturated_deposit_count += o.saturated_deposit_count;
+        h.active_count += o.active_count;
+        h.solver_steps += o.solver_steps;
+        h.solver_failures += o.solver_failures;
+        h.surrogate_count += o.surrogate_count;
+        h.surrogate_fallbacks += o.surrogate_fallbacks;
+        h.deposit_processed += o.deposit_processed;
+        h.lost_photon_count_sum += o.lost_photon_count_sum;
+        h.first_ionization_HI += o.first_ionization_HI;
+        h.first_ionization_HeI += o.first_ionization_HeI;
+        h.first_ionization_HeII += o.first_ionization_HeII;
+        h.secondary_ionization_HI += o.secondary_ionization_HI;
+        h.secondary_ionization_HeI += o.secondary_ionization_HeI;
+    }
     memset(out, 0, sizeof(*out));
     out->ticks = c->ticks;
     out->total_deposit_count = (int64_t)h.total_deposit_count;
@@ -828,7 +846,7 @@ extern "C" int psr_get_stats(psr_ctx *c, psr_stats *out) {
 
 extern "C" int psr_reset_stats(psr_ctx *c) {
     if (!c) return PSR_ERR_ARG;
-    PSR_HIP(c, hipMemsetAsync(c->d_stats, 0, sizeof(DevStats), c->stream));
+    PSR_HIP(c, hipMemsetAsync(c->d_stats, 0, sizeof(DevStats) * PSR_NSTAT, c->stream));
     PSR_HIP(c, hipStreamSynchronize(c->stream));
     c->ticks = 0;
     return PSR_OK;

@@ -406,17 +406,33 @@ __device__ __forceinline__ void store_step(const DevState &S, uint32_t i, int64_
     S.active[i] = 0;
 }
 
+// block totals of the solver counters, then one atomic per counter and block
+// (every thread of the 256-thread block must call this)
 __device__ __forceinline__ void count_steps(DevStats *st, int steps, int fail, int done) {
+    __shared__ int part[3][4];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         steps += __shfl_xor(steps, o, 64);
         fail += __shfl_xor(fail, o, 64);
         done += __shfl_xor(done, o, 64);
     }
-    if ((threadIdx.x & 63) == 0 && done) {
-        atomicAdd(&st->active_count, (unsigned long long)done);
-        atomicAdd(&st->solver_steps, (unsigned long long)steps);
-        if (fail) atomicAdd(&st->solver_failures, (unsigned long long)fail);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        part[0][w] = steps;
+        part[1][w] = fail;
+        part[2][w] = done;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        steps = part[0][0] + part[0][1] + part[0][2] + part[0][3];
+        fail = part[1][0] + part[1][1] + part[1][2] + part[1][3];
+        done = part[2][0] + part[2][1] + part[2][2] + part[2][3];
+        if (done) {
+            DevStats *s = stat_slot(st);
+            atomicAdd(&s->active_count, (unsigned long long)done);
+            atomicAdd(&s->solver_steps, (unsigned long long)steps);
+            if (fail) atomicAdd(&s->solver_failures, (unsigned long long)fail);
+        }
     }
 }
 

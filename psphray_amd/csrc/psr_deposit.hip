@@ -81,7 +81,7 @@ struct DepLds {
 
 template <bool HE>
 __global__ __launch_bounds__(256) void k_deposit(const psr_ray *__restrict__ rays, int64_t nray, DevHits H,
-                                                 const DevCtx *__restrict__ C, DevStats *__restrict__ st,
+                                                 const DevCtx *__restrict__ C, DevStats *__restrict__ st0,
                                                  double *__restrict__ ray_len, double *__restrict__ ray_tm) {
     __shared__ DepLds<HE> L;
     const DevState &S = C->S;
@@ -290,6 +290,7 @@ __global__ __launch_bounds__(256) void k_deposit(const psr_ray *__restrict__ ray
         if (lane == 0) {
             ray_len[r] = dl;
             ray_tm[r] = TM;
+            DevStats *st = stat_slot(st0);
             atomicAdd(&st->total_deposit_count, (unsigned long long)n);
             atomicAdd(&st->deposit_processed, (unsigned long long)(last + 1));
             if (nsat) atomicAdd(&st->saturated_deposit_count, (unsigned long long)nsat);

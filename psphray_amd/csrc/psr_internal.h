@@ -91,8 +91,12 @@ struct DevHits {
     int32_t cap;
 };
 
-// device counters (one struct, atomics)
-struct DevStats {
+// device counters: PSR_NSTAT copies, one 128-byte line each; a block adds
+// into copy blockIdx.x % PSR_NSTAT and the host folds the copies.  Device-
+// scope atomics on one address serialize beyond the XCD's L2, so a single
+// copy costs several ns per wavefront atomic once there are 1e4-1e5 of them.
+#define PSR_NSTAT 256
+struct __attribute__((aligned(128))) DevStats {
     unsigned long long total_deposit_count;
     unsigned long long saturated_deposit_count;
     unsigned long long active_count;
@@ -104,8 +108,12 @@ struct DevStats {
     double lost_photon_count_sum;
     double first_ionization_HI, first_ionization_HeI, first_ionization_HeII;
     double secondary_ionization_HI, secondary_ionization_HeI;
-    int32_t flags;  // PSR_DF_* below
+    int32_t flags;  // PSR_DF_* below (copy 0 only)
 };
+static_assert(sizeof(DevStats) == 128, "DevStats must fill one 128-byte line");
+
+// the counter copy of the calling block
+__device__ __forceinline__ DevStats *stat_slot(DevStats *st) { return st + (blockIdx.x & (PSR_NSTAT - 1)); }
 #define PSR_DF_HITCAP 1
 #define PSR_DF_STACK 2
 #define PSR_DF_SORTCAP 4

@@ -370,7 +370,7 @@ __global__ void k_count_hits(const int32_t *__restrict__ count, int64_t nray, De
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     int v = i < nray ? count[i] : 0;
     v = wave_sum_i(v);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd(&st->total_deposit_count, (unsigned long long)v);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(&stat_slot(st)->total_deposit_count, (unsigned long long)v);
 }
 
 // parity export: CSR lists with caller indices
