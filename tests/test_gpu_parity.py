@@ -185,6 +185,43 @@ def test_deposit_parity(ctx, cfg, n, nray):
     assert s["saturated_deposit_count"] == st.saturated_deposit_count
 
 
+@pytest.mark.parametrize("he", [False, True])
+def test_deposit_no_photon_and_cutoff(ctx, he):
+    """rays of negative type start with TM = 0 under no_photon (src/loop.c:411);
+    the first live hit then ends the ray; other rays run into the cut-off"""
+    print_case("deposit-no-photon", he)
+    wl = make_workload(3 if he else 1, RATES, n=16 if he else 24, nray=64)
+    f = wl.physics.flags
+    flags = psr.make_flags(isothermal=bool(f.isothermal), on_the_spot=bool(f.on_the_spot),
+                           h_only=bool(f.h_only), secondary_ionization=bool(f.secondary_ionization),
+                           no_photon=True)
+    wl.physics = psr.Physics(RATES, wl.physics.units, flags, with_secion=he)
+    orc = oracle_for(wl)
+    load(ctx, wl)
+    ctx.reset_stats()
+    rays = wl.rays(1)
+    rays["type"][::3] = -1
+    rays["Nph"][1::3] *= 1e-6                         # weak rays: cut-off early
+    ctx.rays_upload(rays)
+    ctx.tick(1, wl.tick_time, phases=psr.PSR_PHASE_TRACE | psr.PSR_PHASE_DEPOSIT)
+    out = wl.ps.copy()
+    ctx.state_download(out)
+    ref = wl.ps.copy()
+    t = po.Tree(ref.pos, ref.sml, wl.boxsize)
+    off, hits = po.trace_rays(t, ref, rays)
+    ln, tm, st = po.deposit(orc, ref, rays, off, hits)
+    assert np.array_equal(out.hits, ref.hits)
+    for fld in ("yGdepHI", "yGdepHeI", "yGdepHeII", "heat"):
+        a, r = getattr(out, fld).astype(np.float64), getattr(ref, fld).astype(np.float64)
+        assert np.allclose(a, r, rtol=1e-5, atol=1e-6 * max(np.abs(r).max(), 1e-300)), fld
+    dln, dtm = ctx.ray_results(len(rays))
+    assert np.array_equal(dln, ln)
+    assert np.allclose(dtm, tm, rtol=1e-9, atol=1e-12 * rays["Nph"].max())
+    s = ctx.stats()
+    assert s["deposit_processed"] < s["total_deposit_count"]   # some rays were cut
+    assert s["saturated_deposit_count"] == st.saturated_deposit_count
+
+
 @pytest.mark.parametrize("cfg,n,nray", [(1, 32, 32), (2, 24, 256), (3, 16, 128)])
 def test_full_tick_parity(ctx, cfg, n, nray):
     print_case("tick", cfg, n, nray)
