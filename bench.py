@@ -92,7 +92,9 @@ def main():
     wl = make_workload(args.config, rates, n=args.n, nray=args.nray)
     log(f"workload {wl.name}: {wl.ps.npar} particles, {wl.nray} rays/tick "
         f"({time.perf_counter() - t0:.1f}s)")
-    nt = args.warmup + args.steps
+    # warmup, the timed ticks, then as many again with per-phase event timers
+    # (those synchronise every tick, so they are kept out of the timed run)
+    nt = args.warmup + 2 * args.steps
     ray_ticks = [wl.rays(t) for t in range(1, nt + 1)]
     firsts = np.cumsum([0] + [len(r) for r in ray_ticks])
     pool = np.concatenate(ray_ticks)
@@ -110,16 +112,19 @@ def main():
         ctx.tick(t, wl.tick_time, ray_first=int(firsts[t - 1]), nray=len(ray_ticks[t - 1]), chem=chem)
     ctx.sync()
     ctx.reset_stats()
-    ctx.set_timing(True)
     ctx.sync()
+    t_end = args.warmup + args.steps
     t0 = time.perf_counter()
-    for t in range(args.warmup + 1, nt + 1):
+    for t in range(args.warmup + 1, t_end + 1):
         ctx.tick(t, wl.tick_time, ray_first=int(firsts[t - 1]), nray=len(ray_ticks[t - 1]), chem=chem)
-        if (t - args.warmup) % 5 == 0:
-            log(f"tick {t - args.warmup}/{args.steps}")
     ctx.sync()
     wall = time.perf_counter() - t0
+    log(f"{args.steps} ticks in {wall * 1e3:.2f} ms")
     st = ctx.stats()
+    ctx.set_timing(True)
+    for t in range(t_end + 1, nt + 1):
+        ctx.tick(t, wl.tick_time, ray_first=int(firsts[t - 1]), nray=len(ray_ticks[t - 1]), chem=chem)
+    ctx.sync()
     tm = ctx.timers()
     ctx.set_timing(False)
     K = args.steps

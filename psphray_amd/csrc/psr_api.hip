@@ -86,6 +86,7 @@ int psr_sync_dev_ctx(psr_ctx *c) {
 
 int psr_ensure_scratch(psr_ctx *c, size_t bytes) {
     if (bytes <= c->scratch_bytes) return PSR_OK;
+    PSR_HIP(c, hipStreamSynchronize(c->stream));  // queued kernels may use the old block
     if (c->d_scratch) PSR_HIP(c, hipFree(c->d_scratch));
     c->d_scratch = nullptr;
     c->scratch_bytes = 0;
@@ -148,6 +149,7 @@ extern "C" void psr_destroy(psr_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    if (c->h_ring) (void)hipHostFree(c->h_ring);
     free_tree(c);
     free_state(c);
     psr_mlp_free(c);
@@ -180,6 +182,9 @@ extern "C" const char *psr_last_error(const psr_ctx *c) { return c ? c->err.c_st
 
 extern "C" int psr_sync(psr_ctx *c) {
     if (!c) return PSR_ERR_ARG;
+    PSR_HIP(c, hipSetDevice(c->device));
+    int rc = psr_resolve(c);
+    if (rc) return rc;
     PSR_HIP(c, hipStreamSynchronize(c->stream));
     return PSR_OK;
 }
@@ -192,6 +197,7 @@ extern "C" int psr_set_physics(psr_ctx *c, const psr_units *u, const psr_flags *
         PSR_FAIL(c, PSR_ERR_ARG, "psr_set_physics: incomplete tables");
     if (f->comoving && f->redshift < 0) PSR_FAIL(c, PSR_ERR_ARG, "negative redshift");
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc = psr_resolve(c)) return rc;
     dfree(c->d_tables);
     const size_t nar = 16 * (size_t)t->ar_nrows, nxs = 4 * (size_t)t->xs_nrows,
                  nk = (size_t)t->kernel_n;
@@ -444,6 +450,7 @@ extern "C" int psr_tree_build(psr_ctx *c, const float *pos, const float *sml, co
         if (!(boxsizes[f] > 0)) PSR_FAIL(c, PSR_ERR_ARG, "boxsize must be > 0");
     }
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc = psr_resolve(c)) return rc;
     hipStream_t st = c->stream;
     free_tree(c);
     free_state(c);
@@ -715,6 +722,7 @@ extern "C" int psr_state_upload(psr_ctx *c, const psr_state *s) {
     if (!c->have_tree) PSR_FAIL(c, PSR_ERR_STATE, "psr_state_upload before psr_tree_build");
     if (s->npar != c->npar) PSR_FAIL(c, PSR_ERR_ARG, "psr_state_upload: npar differs from the tree");
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc = psr_resolve(c)) return rc;
     const int64_t n = c->npar, nal = n > 0 ? n : 1;
     if (!c->have_state) {
         PSR_HIP(c, hipMalloc(&c->d_f32, sizeof(float) * 11 * nal));
@@ -758,6 +766,7 @@ extern "C" int psr_state_download(psr_ctx *c, psr_state *s) {
     if (!c->have_state) PSR_FAIL(c, PSR_ERR_STATE, "no particle state");
     if (s->npar != c->npar) PSR_FAIL(c, PSR_ERR_ARG, "psr_state_download: npar differs");
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc = psr_resolve(c)) return rc;
     const int64_t nal = c->npar > 0 ? c->npar : 1;
     if (psr_ensure_scratch(c, sizeof(double) * 3 * nal)) return PSR_ERR_HIP;
     void *tmp = c->d_scratch;
@@ -789,6 +798,8 @@ extern "C" int psr_state_download(psr_ctx *c, psr_state *s) {
 extern "C" int psr_rays_upload(psr_ctx *c, const psr_ray *rays, int64_t nray) {
     if (!c || nray < 0 || (nray > 0 && !rays)) return PSR_ERR_ARG;
     PSR_HIP(c, hipSetDevice(c->device));
+    // a replay of a pending tick needs the pool it was launched with
+    if (int rc = psr_resolve(c)) return rc;
     if (nray > c->ray_cap) {
         dfree(c->d_rays);
         PSR_HIP(c, hipMalloc(&c->d_rays, sizeof(psr_ray) * nray));
@@ -807,6 +818,8 @@ extern "C" int psr_rays_upload(psr_ctx *c, const psr_ray *rays, int64_t nray) {
 extern "C" int psr_get_stats(psr_ctx *c, psr_stats *out) {
     if (!c || !out) return PSR_ERR_ARG;
     std::vector<DevStats> hs(PSR_NSTAT);
+    PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc = psr_resolve(c)) return rc;
     PSR_HIP(c, hipStreamSynchronize(c->stream));
     PSR_HIP(c, hipMemcpy(hs.data(), c->d_stats, sizeof(DevStats) * PSR_NSTAT, hipMemcpyDeviceToHost));
     DevStats h = hs[0];
@@ -848,6 +861,8 @@ extern "C" int psr_get_stats(psr_ctx *c, psr_stats *out) {
 
 extern "C" int psr_reset_stats(psr_ctx *c) {
     if (!c) return PSR_ERR_ARG;
+    PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc = psr_resolve(c)) return rc;
     PSR_HIP(c, hipMemsetAsync(c->d_stats, 0, sizeof(DevStats) * PSR_NSTAT, c->stream));
     PSR_HIP(c, hipStreamSynchronize(c->stream));
     c->ticks = 0;

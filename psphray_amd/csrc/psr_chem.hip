@@ -448,6 +448,7 @@ __global__ __launch_bounds__(256) void k_update(DevState S, int64_t npar, int64_
                                                 DevTables T, DevPhys P, DevStats *__restrict__ st,
                                                 double *__restrict__ saved) {
     using G = Integ<ISO, ADI, OTS, HONLY>;
+    if (halted(&st->flags)) return;  // this tick's trace overflowed: replayed later
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     int steps = 0, fail = 0, done = 0;
     if (i < npar && S.active[i]) {
@@ -483,6 +484,7 @@ __global__ __launch_bounds__(256) void k_update_rest(DevState S, const uint32_t 
                                                      double tick_time, DevTables T, DevPhys P,
                                                      DevStats *__restrict__ st, const double *__restrict__ saved) {
     using G = Integ<ISO, ADI, OTS, HONLY>;
+    if (halted(&st->flags)) return;
     const uint32_t n = *nlist;
     int steps = 0, fail = 0, done = 0;
     // uniform trip count so every lane reaches the wavefront reduction
@@ -579,6 +581,7 @@ int psr_launch_update(psr_ctx *c, int64_t tick, double tick_time, int32_t) {
     const DevState S = c->state();
     const int64_t n = c->npar;
     if (c->qlist_cap < n) {
+        PSR_HIP(c, hipStreamSynchronize(c->stream));  // earlier ticks may still use them
         if (c->d_qlist) PSR_HIP(c, hipFree(c->d_qlist));
         if (c->d_saved) PSR_HIP(c, hipFree(c->d_saved));
         c->d_qlist = nullptr;

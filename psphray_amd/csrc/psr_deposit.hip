@@ -66,6 +66,7 @@ constexpr int DEP_THREADS = 256;
 template <bool HE>
 __global__ __launch_bounds__(DEP_THREADS) void k_dep_prep(const psr_ray *__restrict__ rays, int64_t nray,
                                                           DevHits H, DevDep D, const DevCtx *__restrict__ C) {
+    if (halted(D.halt)) return;
     const DevState &S = C->S;
     const DevTables &T = C->T;
     const DevPhys &P = C->P;
@@ -257,6 +258,7 @@ __global__ __launch_bounds__(DEP_THREADS) void k_dep_walk(const psr_ray *__restr
     constexpr int WAVES = DEP_THREADS / 64;
     __shared__ WalkRec<HE> rec[WAVES][SEG];
     __shared__ double tml[WAVES][SEG + 1];
+    if (halted(D.halt)) return;
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int wave0 = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * WAVES + w));
@@ -355,6 +357,7 @@ __global__ __launch_bounds__(DEP_THREADS) void k_dep_apply(const psr_ray *__rest
     const DevState &S = C->S;
     const DevTables &T = C->T;
     const DevPhys &P = C->P;
+    if (halted(D.halt)) return;
     ApplySums acc{0, 0, 0, 0, 0, 0};
     for (int64_t r = blockIdx.x; r < nray; r += gridDim.x) {
         const int last = D.last[r];
@@ -506,6 +509,10 @@ static int ensure_dep(psr_ctx *c, bool he, int64_t nray) {
     const int64_t slots = c->hit_slots * (int64_t)c->H.cap;
     const int32_t words = he ? 8 : 3;
     if (c->H.cap % 64) PSR_FAIL(c, PSR_ERR_STATE, "deposit: hit capacity must be a multiple of 64");
+    if (slots > c->dep_slots || words > c->dep_words || nray > c->dep_rays) {
+        // kernels of earlier ticks may still use the old arrays
+        PSR_HIP(c, hipStreamSynchronize(c->stream));
+    }
     if (slots > c->dep_slots || words > c->dep_words) {
         if (c->d_dep) PSR_HIP(c, hipFree(c->d_dep));
         c->d_dep = nullptr;
@@ -522,6 +529,7 @@ static int ensure_dep(psr_ctx *c, bool he, int64_t nray) {
         PSR_HIP(c, hipMalloc(&c->D.last, sizeof(int32_t) * std::max<int64_t>(nray, c->hit_slots)));
         c->dep_rays = std::max<int64_t>(nray, c->hit_slots);
     }
+    c->D.halt = &c->d_stats->flags;
     double *p = c->d_dep;
     const size_t s = (size_t)c->dep_slots;
     c->D.em = p;

@@ -14,6 +14,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <deque>
 #include <string>
 
 #include "psphray_amd.h"
@@ -100,6 +102,7 @@ struct DevDep {
     double *em, *nhi, *tmb;
     double *fhi, *fhei, *fheii, *nhei, *nheii;
     int32_t *last;
+    const int32_t *halt;  // the counters' flag word (PSR_DF_HALT)
 };
 
 // device counters: PSR_NSTAT copies, one 128-byte line each; a block adds
@@ -128,6 +131,21 @@ __device__ __forceinline__ DevStats *stat_slot(DevStats *st) { return st + (bloc
 #define PSR_DF_HITCAP 1
 #define PSR_DF_STACK 2
 #define PSR_DF_SORTCAP 4
+// a tick whose trace overflowed stops every later kernel until the host has
+// replayed it (the flag stays set until then)
+#define PSR_DF_HALT (PSR_DF_HITCAP | PSR_DF_STACK)
+
+__device__ __forceinline__ bool halted(const int32_t *halt) { return halt && (*(volatile const int32_t *)halt & PSR_DF_HALT); }
+
+// a tick launched without waiting for its trace's capacity check; validated
+// later from its published flag word (host ring slot)
+struct PendingTick {
+    int64_t ray_first, nray, tick;
+    double tick_time;
+    int32_t phases, chem_mode;
+    uint32_t slot;
+};
+#define PSR_RING 256
 
 struct MlpState;
 
@@ -202,15 +220,24 @@ struct psr_ctx {
     // surrogate
     MlpState *mlp = nullptr;
     double mlp_residual = 1e-4;
+    // ticks launched but not yet validated, and the pinned ring of flag words
+    // the device publishes at the end of each tick (0 = not yet written)
+    std::deque<PendingTick> pending;
+    int32_t *h_ring = nullptr;
+    uint32_t seq = 0;
 
     DevState state() const;
 };
 
 // ---- launchers implemented in the .hip files ----
-int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray);
+// sync: the trace waits for its capacity check (and retries); otherwise the
+// kernels read the halt flag and the tick is validated later
+int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray, bool sync);
 int psr_launch_sort(psr_ctx *c, int64_t nray);
 int psr_launch_deposit(psr_ctx *c, int64_t ray_first, int64_t nray);
 int psr_launch_update(psr_ctx *c, int64_t tick, double tick_time, int32_t chem_mode);
+// validate (and if needed replay) every pending tick; blocks
+int psr_resolve(psr_ctx *c);
 int psr_solve_steps_dev(psr_ctx *c, psr_step *d_steps, int64_t n, double rtol);
 DevPhys psr_dev_phys(const psr_ctx *c);
 int psr_sync_dev_ctx(psr_ctx *c);

@@ -65,10 +65,11 @@ __global__ __launch_bounds__(TR_THREADS) void k_trace(
     const psr_ray *__restrict__ rays, int64_t nray, const Node *__restrict__ nodes,
     const int32_t *__restrict__ froot, const float4 *__restrict__ ps,
     const uint32_t *__restrict__ orig, DevHits H, uint8_t *__restrict__ active,
-    DevStats *__restrict__ st) {
+    DevStats *__restrict__ st, const int32_t *halt) {
     __shared__ int32_t stack[TR_SCAP];
     __shared__ int32_t leafq[TR_LCAP];
     __shared__ int32_t s_sp, s_lq, s_nh, s_cs, s_cl, s_ovf;
+    if (halted(halt)) return;
     const int tid = threadIdx.x;
     for (int64_t r = blockIdx.x; r < nray; r += gridDim.x) {
         const psr_ray R = rays[r];
@@ -201,13 +202,14 @@ __device__ __forceinline__ double block_reduce(double v, bool is_max, double *re
 
 template <int MAXN, int MAXB>
 __global__ __launch_bounds__(256) void k_sort(int64_t nray, DevHits H, const uint32_t *__restrict__ orig,
-                                              int32_t lo, int32_t hi) {
+                                              int32_t lo, int32_t hi, const int32_t *halt) {
     __shared__ unsigned long long key[MAXN];
     __shared__ uint16_t idx[MAXN];
     __shared__ int32_t start[MAXB];
     __shared__ int32_t fill[MAXB];
     __shared__ double red[4];
     __shared__ int32_t wsum[4];
+    if (halted(halt)) return;
     const int tid = threadIdx.x;
     for (int64_t r = blockIdx.x; r < nray; r += gridDim.x) {
         const int n = H.count[r];
@@ -294,7 +296,8 @@ __global__ __launch_bounds__(256) void k_sort(int64_t nray, DevHits H, const uin
 }
 
 // sorted slot / b columns from a permutation (rays of the global fallback)
-__global__ void k_apply_perm(const int32_t *__restrict__ rays_big, int32_t nbig, DevHits H) {
+__global__ void k_apply_perm(const int32_t *__restrict__ rays_big, int32_t nbig, DevHits H, const int32_t *halt) {
+    if (halted(halt)) return;
     for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
         const int64_t r = rays_big[b];
         const int n = H.count[r];
@@ -312,7 +315,9 @@ __global__ void k_apply_perm(const int32_t *__restrict__ rays_big, int32_t nbig,
 __global__ __launch_bounds__(256) void k_sort_global(const int32_t *__restrict__ rays_big, int32_t nbig,
                                                      DevHits H, const uint32_t *__restrict__ orig,
                                                      unsigned long long *__restrict__ gkey,
-                                                     uint32_t *__restrict__ gsec, int32_t pmax) {
+                                                     uint32_t *__restrict__ gsec, int32_t pmax,
+                                                     const int32_t *halt) {
+    if (halted(halt)) return;
     const int tid = threadIdx.x;
     for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
         const int64_t r = rays_big[b];
@@ -366,7 +371,8 @@ __global__ __launch_bounds__(256) void k_sort_global(const int32_t *__restrict__
 }
 
 // total_deposit_count when deposit is skipped (traceOnly)
-__global__ void k_count_hits(const int32_t *__restrict__ count, int64_t nray, DevStats *st) {
+__global__ void k_count_hits(const int32_t *__restrict__ count, int64_t nray, DevStats *st, const int32_t *halt) {
+    if (halted(halt)) return;
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     int v = i < nray ? count[i] : 0;
     v = wave_sum_i(v);
@@ -394,6 +400,8 @@ static int ensure_hits(psr_ctx *c, int64_t nray, int32_t cap) {
     if (nray <= c->hit_slots && cap <= c->H.cap && c->H.d) return PSR_OK;
     const int64_t ns = std::max(nray, c->hit_slots);
     const int32_t cp = std::max(cap, c->H.cap);
+    // kernels of earlier ticks may still use the old arrays
+    PSR_HIP(c, hipStreamSynchronize(c->stream));
     if (c->H.d) (void)hipFree(c->H.d);
     if (c->H.b) (void)hipFree(c->H.b);
     if (c->H.slot) (void)hipFree(c->H.slot);
@@ -431,17 +439,29 @@ static int clear_flags(psr_ctx *c) {
     return PSR_OK;
 }
 
-int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray) {
+int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray, bool sync) {
     int32_t cap0 = c->H.cap > 0 ? c->H.cap : 2048;
     int rc = ensure_hits(c, nray, cap0);
     if (rc) return rc;
-    for (int attempt = 0; attempt < 8; attempt++) {
-        if ((rc = clear_flags(c))) return rc;
-        const unsigned grid = (unsigned)std::min<int64_t>(nray, 65535 * 4);
+    const unsigned grid = (unsigned)std::min<int64_t>(nray, 65535 * 4);
+    if (!sync) {
+        // the flag stays as the previous ticks left it: set only if one of
+        // them overflowed, and then this tick is replayed as well
         PSR_EV(c, 0, 0);
         k_trace<<<grid, TR_THREADS, 0, c->stream>>>(c->d_rays + ray_first, nray, c->d_nodes,
                                                     c->d_field_root, c->d_ps, c->d_orig, c->H,
-                                                    c->d_active, c->d_stats);
+                                                    c->d_active, c->d_stats, &c->d_stats->flags);
+        PSR_HIP(c, hipGetLastError());
+        PSR_EV(c, 0, 1);
+        c->last_nray = nray;
+        return PSR_OK;
+    }
+    for (int attempt = 0; attempt < 8; attempt++) {
+        if ((rc = clear_flags(c))) return rc;
+        PSR_EV(c, 0, 0);
+        k_trace<<<grid, TR_THREADS, 0, c->stream>>>(c->d_rays + ray_first, nray, c->d_nodes,
+                                                    c->d_field_root, c->d_ps, c->d_orig, c->H,
+                                                    c->d_active, c->d_stats, nullptr);
         PSR_HIP(c, hipGetLastError());
         PSR_EV(c, 0, 1);
         int32_t fl = 0;
@@ -455,6 +475,7 @@ int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray) {
         std::vector<int32_t> cnt(nray);
         PSR_HIP(c, hipMemcpy(cnt.data(), c->H.count, sizeof(int32_t) * nray, hipMemcpyDeviceToHost));
         const int32_t mx = *std::max_element(cnt.begin(), cnt.end());
+        // capacities stay 2048 * 2^k (a multiple of 64, as the deposit walk needs)
         int32_t ncap = c->H.cap;
         while (ncap < mx) ncap *= 2;
         c->H.cap = 0;  // force reallocation at the new size
@@ -467,23 +488,32 @@ int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray) {
 
 int psr_launch_sort(psr_ctx *c, int64_t nray) {
     const unsigned grid = (unsigned)std::min<int64_t>(nray, 65535 * 4);
+    const int32_t *halt = &c->d_stats->flags;
     PSR_EV(c, 1, 0);
     // LDS tiers by list length: 10 B per hit + 8 B per bucket
-    k_sort<1024, 256><<<grid, 256, 0, c->stream>>>(nray, c->H, c->d_orig, -1, 1024);
+    k_sort<1024, 256><<<grid, 256, 0, c->stream>>>(nray, c->H, c->d_orig, -1, 1024, halt);
     PSR_HIP(c, hipGetLastError());
     if (c->H.cap > 1024) {
-        k_sort<4096, 1024><<<grid, 256, 0, c->stream>>>(nray, c->H, c->d_orig, 1024, 4096);
+        k_sort<4096, 1024><<<grid, 256, 0, c->stream>>>(nray, c->H, c->d_orig, 1024, 4096, halt);
         PSR_HIP(c, hipGetLastError());
     }
     if (c->H.cap > 4096) {
-        k_sort<8192, 2048><<<grid, 256, 0, c->stream>>>(nray, c->H, c->d_orig, 4096, 8192);
+        k_sort<8192, 2048><<<grid, 256, 0, c->stream>>>(nray, c->H, c->d_orig, 4096, 8192, halt);
         PSR_HIP(c, hipGetLastError());
     }
     if (c->H.cap > 8192) {
         std::vector<int32_t> cnt(nray), big;
+        int32_t fl = 0;
         PSR_HIP(c, hipMemcpyAsync(cnt.data(), c->H.count, sizeof(int32_t) * nray, hipMemcpyDeviceToHost,
                                   c->stream));
+        PSR_HIP(c, hipMemcpyAsync(&fl, &c->d_stats->flags, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
         PSR_HIP(c, hipStreamSynchronize(c->stream));
+        // an overflowed trace leaves counts beyond the capacity: its tick is
+        // replayed, nothing to sort now
+        if (fl & PSR_DF_HALT) {
+            PSR_EV(c, 1, 1);
+            return PSR_OK;
+        }
         int32_t pmax = 1;
         for (int64_t r = 0; r < nray; r++)
             if (cnt[r] > 8192) {
@@ -502,9 +532,9 @@ int psr_launch_sort(psr_ctx *c, int64_t nray) {
             PSR_HIP(c, hipMemcpyAsync(d_big, big.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, c->stream));
             PSR_HIP(c, hipStreamSynchronize(c->stream));  // 'big' is released on return
             k_sort_global<<<std::min(nb, 1024), 256, 0, c->stream>>>(d_big, nb, c->H, c->d_orig, gkey, gsec,
-                                                                     pmax);
+                                                                     pmax, halt);
             PSR_HIP(c, hipGetLastError());
-            k_apply_perm<<<std::min(nb, 1024), 256, 0, c->stream>>>(d_big, nb, c->H);
+            k_apply_perm<<<std::min(nb, 1024), 256, 0, c->stream>>>(d_big, nb, c->H, halt);
             PSR_HIP(c, hipGetLastError());
             PSR_HIP(c, hipStreamSynchronize(c->stream));  // scratch is reused by the next call
         }
@@ -527,9 +557,10 @@ extern "C" int psr_trace_lists(psr_ctx *c, int64_t ray_first, int64_t nray, int6
     int rc = check_ready(c, ray_first, nray);
     if (rc) return rc;
     PSR_HIP(c, hipSetDevice(c->device));
+    if ((rc = psr_resolve(c))) return rc;
     offsets[0] = 0;
     if (nray == 0) return PSR_OK;
-    if ((rc = psr_launch_trace(c, ray_first, nray))) return rc;
+    if ((rc = psr_launch_trace(c, ray_first, nray, true))) return rc;
     // parity mode marks particles active like the tick does; clear them
     PSR_HIP(c, hipMemsetAsync(c->d_active, 0, c->npar > 0 ? c->npar : 1, c->stream));
     if ((rc = psr_launch_sort(c, nray))) return rc;
@@ -572,6 +603,73 @@ extern "C" int psr_trace_lists(psr_ctx *c, int64_t ray_first, int64_t nray, int6
     return PSR_OK;
 }
 
+// the kernels of one tick; sync = wait for the trace's capacity check
+static int tick_body(psr_ctx *c, const PendingTick &t, bool sync) {
+    int rc;
+    const int32_t *halt = sync ? nullptr : &c->d_stats->flags;
+    if (t.nray > 0) {
+        if ((rc = psr_launch_trace(c, t.ray_first, t.nray, sync))) return rc;
+        if (t.phases & PSR_PHASE_DEPOSIT) {
+            if ((rc = psr_launch_sort(c, t.nray))) return rc;
+            if ((rc = psr_launch_deposit(c, t.ray_first, t.nray))) return rc;
+        } else {
+            k_count_hits<<<(unsigned)((t.nray + 255) / 256), 256, 0, c->stream>>>(c->H.count, t.nray,
+                                                                                  c->d_stats, halt);
+            PSR_HIP(c, hipGetLastError());
+        }
+    }
+    if (t.phases & PSR_PHASE_UPDATE) {
+        if (t.chem_mode == PSR_CHEM_SURROGATE) rc = psr_mlp_update_dev(c, t.tick, t.tick_time);
+        else rc = psr_launch_update(c, t.tick, t.tick_time, t.chem_mode);
+        if (rc) return rc;
+    } else {
+        // merge without update: the active set is dropped (traceOnly); an
+        // overflowed tick's replay marks its particles again
+        PSR_HIP(c, hipMemsetAsync(c->d_active, 0, c->npar > 0 ? c->npar : 1, c->stream));
+    }
+    return PSR_OK;
+}
+
+// end of a tick: its flag word (with bit 8 set, so 0 means "not yet") into
+// the pinned ring slot the host polls
+__global__ void k_publish(const int32_t *__restrict__ flags, int32_t *slot) {
+    if (threadIdx.x == 0) *(volatile int32_t *)slot = 0x100 | *(volatile const int32_t *)flags;
+}
+
+int psr_resolve(psr_ctx *c) {
+    if (c->pending.empty()) return PSR_OK;
+    PSR_HIP(c, hipStreamSynchronize(c->stream));
+    std::deque<PendingTick> todo;
+    todo.swap(c->pending);
+    // ticks before the first overflow were applied; from it on, every kernel
+    // stopped at the halt flag and the state is as that tick found it
+    while (!todo.empty()) {
+        const int32_t v = ((volatile int32_t *)c->h_ring)[todo.front().slot];
+        if (v & PSR_DF_HALT) break;
+        todo.pop_front();
+    }
+    if (todo.empty()) return PSR_OK;
+    PSR_HIP(c, hipMemsetAsync(&c->d_stats->flags, 0, sizeof(int32_t), c->stream));
+    for (const PendingTick &t : todo) {
+        const int rc = tick_body(c, t, true);
+        if (rc) return rc;
+    }
+    PSR_HIP(c, hipStreamSynchronize(c->stream));
+    return PSR_OK;
+}
+
+// drop validated ticks from the front of the queue without blocking; an
+// overflow found there is replayed at once
+static int poll_pending(psr_ctx *c) {
+    while (!c->pending.empty()) {
+        const int32_t v = ((volatile int32_t *)c->h_ring)[c->pending.front().slot];
+        if (v == 0) return PSR_OK;
+        if (v & PSR_DF_HALT) return psr_resolve(c);
+        c->pending.pop_front();
+    }
+    return PSR_OK;
+}
+
 extern "C" int psr_tick(psr_ctx *c, int64_t ray_first, int64_t nray, int64_t tick, double tick_time,
                         int32_t phases, int32_t chem_mode) {
     if (!c) return PSR_ERR_ARG;
@@ -582,26 +680,20 @@ extern "C" int psr_tick(psr_ctx *c, int64_t ray_first, int64_t nray, int64_t tic
     if (chem_mode != PSR_CHEM_EXACT && chem_mode != PSR_CHEM_SURROGATE)
         PSR_FAIL(c, PSR_ERR_ARG, "unknown chemistry mode");
     PSR_HIP(c, hipSetDevice(c->device));
+    if (!c->h_ring) {
+        PSR_HIP(c, hipHostMalloc((void **)&c->h_ring, sizeof(int32_t) * PSR_RING, hipHostMallocCoherent));
+        memset(c->h_ring, 0, sizeof(int32_t) * PSR_RING);
+    }
+    if ((rc = poll_pending(c))) return rc;
+    if (c->pending.size() >= PSR_RING - 1 && (rc = psr_resolve(c))) return rc;
+    PendingTick t{ray_first, nray, tick, tick_time, phases, chem_mode, c->seq++ % PSR_RING};
+    ((volatile int32_t *)c->h_ring)[t.slot] = 0;
     c->ev_mask = 0;
     PSR_EV(c, 4, 0);
-    if (nray > 0) {
-        if ((rc = psr_launch_trace(c, ray_first, nray))) return rc;
-        if (phases & PSR_PHASE_DEPOSIT) {
-            if ((rc = psr_launch_sort(c, nray))) return rc;
-            if ((rc = psr_launch_deposit(c, ray_first, nray))) return rc;
-        } else {
-            k_count_hits<<<(unsigned)((nray + 255) / 256), 256, 0, c->stream>>>(c->H.count, nray, c->d_stats);
-            PSR_HIP(c, hipGetLastError());
-        }
-    }
-    if (phases & PSR_PHASE_UPDATE) {
-        if (chem_mode == PSR_CHEM_SURROGATE) rc = psr_mlp_update_dev(c, tick, tick_time);
-        else rc = psr_launch_update(c, tick, tick_time, chem_mode);
-        if (rc) return rc;
-    } else {
-        // merge without update: the active set is dropped (traceOnly)
-        PSR_HIP(c, hipMemsetAsync(c->d_active, 0, c->npar > 0 ? c->npar : 1, c->stream));
-    }
+    if ((rc = tick_body(c, t, false))) return rc;
+    k_publish<<<1, 64, 0, c->stream>>>(&c->d_stats->flags, c->h_ring + t.slot);
+    PSR_HIP(c, hipGetLastError());
+    c->pending.push_back(t);
     PSR_EV(c, 4, 1);
     if (c->timing) {
         PSR_HIP(c, hipEventSynchronize(c->ev[9]));
@@ -635,6 +727,8 @@ extern "C" int psr_get_timers(psr_ctx *c, double *ms) {
 extern "C" int psr_ray_results(psr_ctx *c, int64_t nray, double *length, double *tm) {
     if (!c) return PSR_ERR_ARG;
     if (nray > c->last_nray) PSR_FAIL(c, PSR_ERR_ARG, "more rays than the last tick traced");
+    PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc = psr_resolve(c)) return rc;
     PSR_HIP(c, hipStreamSynchronize(c->stream));
     if (length) PSR_HIP(c, hipMemcpy(length, c->d_ray_len, sizeof(double) * nray, hipMemcpyDeviceToHost));
     if (tm) PSR_HIP(c, hipMemcpy(tm, c->d_ray_tm, sizeof(double) * nray, hipMemcpyDeviceToHost));

@@ -222,6 +222,46 @@ def test_deposit_no_photon_and_cutoff(ctx, he):
     assert s["saturated_deposit_count"] == st.saturated_deposit_count
 
 
+def test_overflowed_tick_is_replayed():
+    """ticks are launched without waiting for the trace's capacity check; a
+    tick whose lists outgrow the capacity (2048 hits per ray at first) stops
+    every later kernel and is replayed, with the ticks after it, at the next
+    call -- the result equals three sequential oracle ticks"""
+    print_case("overflow-replay")
+    wl = make_workload(1, RATES, n=104, nray=16)
+    box = wl.boxsize
+    r1, r2, r3 = wl.rays(1), wl.rays(2), wl.rays(3)
+    rng = np.random.default_rng(7)
+    d = np.array([1.0, 1.0, 1.0]) + rng.normal(0, 0.02, (len(r2), 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    r2["s"] = np.float32(0.01 * box)
+    r2["dir"] = d.astype(np.float32)
+    r2["length"] = box * np.sqrt(3.0)
+    pool = np.concatenate([r1, r2, r3])
+    orc = oracle_for(wl)
+    with psr.Context(0) as c2:
+        load(c2, wl)
+        c2.rays_upload(pool)
+        for t, (a, rr) in enumerate(((0, r1), (len(r1), r2), (len(r1) + len(r2), r3)), start=1):
+            c2.tick(t, wl.tick_time, ray_first=a, nray=len(rr))
+        out = wl.ps.copy()
+        c2.state_download(out)
+        s = c2.stats()
+    ref = wl.ps.copy()
+    tree = po.Tree(ref.pos, ref.sml, box)
+    total = 0
+    for t, rr in enumerate((r1, r2, r3), start=1):
+        off, _ = po.trace_rays(tree, ref, rr)
+        if t == 2:
+            assert np.diff(off).max() > 2048          # the case this test is for
+        total += int(off[-1])
+        rc, _ = po.tick(orc, tree, ref, rr, t, wl.tick_time)
+        assert rc == 0
+    assert s["total_deposit_count"] == total
+    assert np.array_equal(out.lasthit, ref.lasthit)
+    assert np.abs(out.lambdaH - ref.lambdaH).max() < 1e-6
+
+
 @pytest.mark.parametrize("cfg,n,nray", [(1, 32, 32), (2, 24, 256), (3, 16, 128)])
 def test_full_tick_parity(ctx, cfg, n, nray):
     print_case("tick", cfg, n, nray)
