@@ -1084,6 +1084,40 @@ static void lu_solve(const double *a, int n, const int *piv, double *b) {
 #define ROS2_GAMMA 1.7071067811865475 /* 1 + 1/sqrt(2) */
 #define ROS2_MAXSTEPS 100000
 
+/* Isothermal, hydrogen-only, on-the-spot: the rates are fixed over the step
+ * and src/solve.c:97-99 reduces to the Riccati equation (x = x_HI, tau = t /
+ * time, G = gamma_HI time, A = alpha_B time, g = yGdepHI, m = yeMET)
+ *     dx/dtau = F(x) = -g + (1 + m - x)(A - c x),   c = A + G,
+ * i.e. F = a + b x + c x^2 with a = (1+m)A - g, b = -((1+m)c + A).  Its roots
+ * are real, lambda^2 = b^2 - 4ac = ((1+m)G + mA)^2 + 4cg >= 0, r1 = a/q <= r2
+ * = q/c with q = (lambda - b)/2.  With u = x - r1 the solution is
+ *     u(1) = u0 e^-lambda / (e^-lambda + w (1 - e^-lambda)),  w = c (r2 - x0) / lambda,
+ * evaluated as x(1) = x0 + F(x0) phi / D, phi = (1 - e^-lambda) / lambda,
+ * D = e^-lambda + w (1 - e^-lambda); w is taken from F(x0) = -c u0 (r2 - x0)
+ * when x0 is nearer r2 (no cancellation in r2 - x0).  Exact in one step: it
+ * replaces the integrator for this flag set (the state is still clamped). */
+double or_riccati_tick(double x0, double g, double m, double G, double A) {
+    const double c = A + G;
+    const double F0 = -g + (1.0 + m - x0) * (A - c * x0);
+    if (F0 == 0.0) return x0;
+    const double b = -((1.0 + m) * c + A);
+    const double a = (1.0 + m) * A - g;
+    const double dd = (1.0 + m) * G + m * A;
+    const double lam = sqrt(dd * dd + 4.0 * c * g);
+    const double q = 0.5 * (lam - b);
+    if (lam < 1e-8) {
+        /* F nearly constant: phi = 1 - lambda/2 + O(lambda^2), D = 1 - c u0 phi */
+        const double cu0 = q > 0 ? c * x0 - a * c / q : 0.0;
+        const double phi = 1.0 - 0.5 * lam;
+        return x0 + F0 * phi / (1.0 - cu0 * phi);
+    }
+    const double u0 = x0 - a / q;
+    const double s = -expm1(-lam);
+    const double w = (c * u0 > 0.5 * lam) ? -F0 / (u0 * lam) : (q - c * x0) / lam;
+    const double D = (1.0 - s) + fmax(w, 0.0) * s;
+    return x0 + F0 * (s / lam) / D;
+}
+
 /* Two-stage L-stable Rosenbrock (Verwer, Spee, Blom & Hundsdorfer 1999):
  *   (I - g h J) k1 = F(y),  (I - g h J) k2 = F(y + h k1) - 2 k1,
  *   y' = y + 1.5 h k1 + 0.5 h k2,  err = 0.5 h (k1 + k2).
@@ -1102,6 +1136,14 @@ int32_t or_step_evolve(const or_ctx *c, or_step *s, double rtol, const double *a
     int nsteps = 0, rejected = 0;
     double f0[7], f1[7], fp[7], xt[7], k1[7], k2[7], J[49], M[49];
     int piv[7];
+    if (c->f.isothermal && c->f.h_only && c->f.on_the_spot) {
+        const double logT = log10(s->T);
+        const double G = or_ar_get(c, AR_HI_CI, logT) * s->nH * s->time;
+        const double A = or_ar_get(c, AR_HII_RC_B, logT) * s->nH * s->time;
+        x[0] = or_riccati_tick(x[0], s->yGdepHI, s->yeMET, G, A);
+        nsteps = 1;
+        t = 1.0;
+    }
     while (t < 1.0 && nsteps < ROS2_MAXSTEPS) {
         if (h > 1.0 - t) h = 1.0 - t;
         or_chem_rhs(t, x, f0, &cd);

@@ -191,9 +191,13 @@ typedef struct {
 } or_chem_data;
 int or_chem_rhs(double t, double *x, double *dxdt, void *data);
 
+/* exact one-step solution of the isothermal hydrogen-only on-the-spot rate
+ * equation dx/dtau = -g + (1 + m - x)(A - (A + G) x) over tau in [0, 1] */
+double or_riccati_tick(double x0, double g, double m, double G, double A);
 /* the oracle's implicit FP64 solver (2-stage L-stable Rosenbrock with
  * embedded error estimate, same algorithm as the device), integrating
- * normalized time 0..1 and applying the clamps of src/solve.c:208-222 */
+ * normalized time 0..1 and applying the clamps of src/solve.c:208-222; the
+ * isothermal hydrogen-only on-the-spot case takes or_riccati_tick instead */
 int32_t or_step_evolve(const or_ctx *c, or_step *s, double rtol,
                        const double *atol7);
 /* chemistry in bulk (for fixtures): n steps in place */

@@ -87,6 +87,7 @@ _SIG = {
     "or_tick": (_I32, [_VP, _VP, C.POINTER(OrPsys), _VP, _I64, _I64, _D, C.POINTER(OrStats), _I32]),
     "or_chem_rhs": (C.c_int, [_D, _VP, _VP, _VP]),
     "or_step_evolve": (_I32, [_VP, _VP, _D, _VP]),
+    "or_riccati_tick": (_D, [_D, _D, _D, _D, _D]),
     "or_step_evolve_many": (_I32, [_VP, _VP, _I64, _D, _I32]),
     "or_ieye2T": (_D, [_VP, _D, _D]),
     "or_Tye2ie": (_D, [_VP, _D, _D]),

@@ -152,6 +152,60 @@ __device__ void rhs(const double x[7], double f[7], const StepIn &s, const DevTa
 #define ROS2_GAMMA 1.7071067811865475
 #define ROS2_MAXSTEPS 100000
 
+// Isothermal hydrogen-only on-the-spot: with the rates fixed over the step
+// the rate equation (x = x_HI, tau = t / time) is the Riccati equation
+//     dx/dtau = F(x) = -g + (1 + m - x)(A - c x),  c = A + G,
+// G = gamma_HI time, A = alpha_B time, g = yGdepHI, m = yeMET; F = a + b x +
+// c x^2 with real roots r1 = a/q <= r2 = q/c, q = (lambda - b)/2, lambda^2 =
+// ((1+m)G + mA)^2 + 4cg.  Exact solution over the step:
+//     x(1) = x0 + F(x0) phi / D,  phi = (1 - e^-lambda)/lambda,
+//     D = e^-lambda + w (1 - e^-lambda),  w = c (r2 - x0) / lambda,
+// w from F(x0) = -c u0 (r2 - x0) (u0 = x0 - r1) when x0 is nearer r2.  The
+// oracle's or_riccati_tick evaluates the same expressions in the same order.
+template <bool ISO, bool OTS, bool HONLY>
+constexpr bool closed_form() {
+    return ISO && OTS && HONLY;
+}
+
+__device__ double riccati_tick(double x0, double g, double m, double G, double A) {
+    const double c = A + G;
+    const double F0 = -g + (1.0 + m - x0) * (A - c * x0);
+    if (F0 == 0.0) return x0;
+    const double b = -((1.0 + m) * c + A);
+    const double a = (1.0 + m) * A - g;
+    const double dd = (1.0 + m) * G + m * A;
+    const double lam = sqrt(dd * dd + 4.0 * c * g);
+    const double q = 0.5 * (lam - b);
+    if (lam < 1e-8) {
+        const double cu0 = q > 0 ? c * x0 - a * c / q : 0.0;
+        const double phi = 1.0 - 0.5 * lam;
+        return x0 + F0 * phi / (1.0 - cu0 * phi);
+    }
+    const double u0 = x0 - a / q;
+    const double s = -expm1(-lam);
+    const double w = (c * u0 > 0.5 * lam) ? -F0 / (u0 * lam) : (q - c * x0) / lam;
+    const double D = (1.0 - s) + fmax(w, 0.0) * s;
+    return x0 + F0 * (s / lam) / D;
+}
+
+// one step in closed form, then the clamps of src/solve.c:208-213
+__device__ __forceinline__ void riccati_step(const double x0[7], const StepIn &s, const DevTables &T,
+                                             double x[7]) {
+    Rates R;
+    rates_at<true>(T, log10(s.T), s.nH, R);
+    const double G = R.gamma_HI * s.time;
+    const double A = R.alpha_HII_B * s.time;
+#pragma unroll
+    for (int q = 0; q < 7; q++) x[q] = x0[q];
+    x[0] = riccati_tick(x0[0], s.yGdepHI, s.yeMET, G, A);
+    x[0] = fmin(1., fmax(0., x[0]));
+    x[1] = fmin(1., fmax(0., x[1]));
+    x[2] = fmin(1. - x[1], fmax(0., x[2]));
+    x[3] = fmax(0., x[3]);
+    x[4] = fmax(0., x[4]);
+    x[5] = fmax(0., x[5]);
+}
+
 // LU with partial pivoting (first strictly larger pivot, as the oracle);
 // every index is static after unrolling so the matrix stays in registers
 template <int N>
@@ -451,7 +505,17 @@ __global__ __launch_bounds__(256) void k_update(DevState S, int64_t npar, int64_
     if (halted(&st->flags)) return;  // this tick's trace overflowed: replayed later
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     int steps = 0, fail = 0, done = 0;
-    if (i < npar && S.active[i]) {
+    if (closed_form<ISO, OTS, HONLY>()) {
+        if (i < npar && S.active[i]) {
+            double x0[7], x[7];
+            StepIn s;
+            load_step(S, (uint32_t)i, tick, tick_time, P, x0, s);
+            riccati_step(x0, s, T, x);
+            steps = 1;
+            done = 1;
+            store_step<ISO, ADI>(S, (uint32_t)i, tick, P, x, s.T);
+        }
+    } else if (i < npar && S.active[i]) {
         double x0[7];
         StepIn s;
         load_step(S, (uint32_t)i, tick, tick_time, P, x0, s);
@@ -534,13 +598,20 @@ __global__ __launch_bounds__(256) void k_solve(psr_step *__restrict__ steps, int
     s.yeMET = st.yeMET;
     const double x0[7] = {st.lambdaH, st.lambdaHeI, st.lambdaHeII, st.yGrecHII, st.yGrecHeII, st.yGrecHeIII,
                           st.ie};
+    double xc[7];
+    const double *x = xc;
+    int ns = 1;
     Integ<ISO, ADI, OTS, HONLY> g;
-    g.init(x0, s, T);
-    while (!g.attempt(T, P, rtol)) {
+    if (closed_form<ISO, OTS, HONLY>()) {
+        riccati_step(x0, s, T, xc);
+    } else {
+        g.init(x0, s, T);
+        while (!g.attempt(T, P, rtol)) {
+        }
+        g.clamp();
+        x = g.x;
+        ns = g.t >= 1.0 ? g.nsteps : -g.nsteps;
     }
-    g.clamp();
-    const double *x = g.x;
-    const int ns = g.t >= 1.0 ? g.nsteps : -g.nsteps;
     st.lambdaH = x[0];
     st.lambdaHeI = x[1];
     st.lambdaHeII = x[2];
@@ -596,6 +667,11 @@ int psr_launch_update(psr_ctx *c, int64_t tick, double tick_time, int32_t) {
     unsigned grid = (unsigned)((n + 255) / 256);
     CHEM_DISPATCH(k_update, S, n, tick, tick_time, c->T, P, c->d_stats, c->d_saved);
     PSR_HIP(c, hipGetLastError());
+    if (c->F.isothermal && c->F.on_the_spot && c->F.h_only) {
+        // closed form: every particle finished in phase 1
+        PSR_EV(c, 3, 1);
+        return PSR_OK;
+    }
     // leftovers: compact with a device-wide selection (scan based)
     hipcub::CountingInputIterator<uint32_t> it(0);
     IsLeftOver pred{c->d_active};

@@ -203,6 +203,41 @@ def test_chemistry_riccati_closed_form():
         assert 1 - o["lambdaH"][0] == pytest.approx(x, abs=1e-6)
 
 
+def test_riccati_tick_matches_tight_integration():
+    """the one-step solution of dx/dtau = -g + (1 + m - x)(A - (A + G) x)
+    against an implicit integrator at rtol 1e-12, over rates spanning
+    lambda = 1e-12 .. 30 (both branches), deposits below, at and above x0,
+    and the exact-equilibrium and no-rate limits"""
+    from scipy.integrate import solve_ivp
+    L = po.lib()
+    rng = np.random.default_rng(11)
+    cases = []
+    for _ in range(300):
+        x0 = rng.choice([rng.uniform(0, 1), 1.0, 10 ** rng.uniform(-8, 0)])
+        A = 10 ** rng.uniform(-12, 1.2)
+        G = rng.choice([0.0, 10 ** rng.uniform(-14, 1.0)])
+        m = rng.choice([0.0, 10 ** rng.uniform(-6, -2)])
+        g = rng.choice([0.0, x0 * rng.uniform(0, 0.99), x0, x0 * rng.uniform(1, 50)])
+        cases.append((x0, g, m, G, A))
+    for x0, g, m, G, A in cases:
+        c = A + G
+        f = lambda t, x: [-g + (1 + m - x[0]) * (A - c * x[0])]
+        jac = lambda t, x: [[-(A - c * x[0]) - c * (1 + m - x[0])]]
+        ref = solve_ivp(f, (0, 1), [x0], method="Radau", jac=jac, rtol=1e-12, atol=1e-15).y[0, -1]
+        got = L.or_riccati_tick(x0, g, m, G, A)
+        assert abs(got - ref) <= 1e-9 * max(1.0, abs(ref)), (x0, g, m, G, A, got, ref)
+    # no rates: the deposit is taken whole; neutral, no electrons, no deposit: stays
+    assert L.or_riccati_tick(0.7, 0.2, 0.0, 0.0, 0.0) == pytest.approx(0.5, abs=1e-15)
+    assert L.or_riccati_tick(1.0, 0.0, 0.0, 3.0, 50.0) == 1.0
+    # strongly relaxing (lambda ~ 200): the stable root r1 = a / q
+    A, G, m, g = 900.0, 200.0, 0.0, 0.5
+    c = A + G
+    a = A - g
+    b = -(c + A)
+    q = 0.5 * (np.sqrt(b * b - 4 * a * c) - b)
+    assert L.or_riccati_tick(0.9, g, m, G, A) == pytest.approx(a / q, rel=1e-12)
+
+
 def test_chemistry_equilibrium_case_a_formula():
     """src/test-solve.c:41-47: with case A and no photons the neutral fraction
     relaxes to P/q = alpha/(gamma + alpha)"""
