@@ -164,8 +164,6 @@ extern "C" void psr_destroy(psr_ctx *c) {
     dfree(c->H.sb);
     dfree(c->d_ray_len);
     dfree(c->d_ray_tm);
-    dfree(c->d_dep);
-    dfree(c->D.last);
     dfree(c->d_stats);
     dfree(c->d_qlist);
     dfree(c->d_q);

@@ -1,36 +1,35 @@
 // psr_deposit.hip -- photon deposition along the sorted hit lists on gfx950
 // (deposit, src/loop.c:388-608).
 //
-// Three kernels per tick, so that the one serial part carries no memory
-// latency beyond its own streamed inputs:
+// One 3-wavefront workgroup per ray, pipelined over segments of 128 sorted
+// hits held in an LDS ring of three:
 //
-//   k_dep_prep   one block per ray, one thread per hit: gathers the particle
-//                (sml, mass, x_HI[, x_HeI, x_HeII]) of each sorted hit,
-//                computes the optical depths, em = expm1(-tau_sum) and N_HI,
-//                and writes them to per-hit arrays.  em = +0 marks a hit with
-//                tau_sum = 0, which the reference skips (src/loop.c:475-477);
-//                a live hit has em <= -0 (sign bit set).
-//   k_dep_walk   one wavefront per ray: the transmission recurrence
-//                    TM -= min(-TM * em, N_HI)            (hydrogen only)
-//                walked serially, 64 hits per chunk, the chunk's inputs taken
-//                from the lanes with readlane (the next chunks' loads are in
-//                flight meanwhile).  For hydrogen only the step is evaluated
-//                as TM' = max(TM + TM * em, TM - N_HI): correctly rounded
-//                subtraction is monotone in the subtrahend, so this is the
-//                reference's value bit for bit, with three dependent
-//                operations instead of five.  Each lane keeps the TM its hit
-//                saw (written to tmb) and the TM after it; the cut-off
-//                "TM < 0 || TM / Nph < 1e-10" (src/loop.c:585) is one compare
-//                against the threshold T0, tested once per chunk (TM never
-//                increases) and resolved to the first live hit by a ballot.
-//   k_dep_apply  one block per ray, one thread per hit up to the walk's end:
-//                recomputes the hit's absorbed photons from tmb exactly as the
-//                walk did, and scatters the ionization / heating deposits with
-//                f32 atomics (src/loop.c:486-573).
+//   wave 1 (prep)   gathers the particles of segment s + 1 (sml, mass, x_HI
+//                   [, x_HeI, x_HeII]), computes the optical depths, em =
+//                   expm1(-tau_sum) and N_HI (src/loop.c:427-477) into the
+//                   ring.  em = +0 marks a hit with tau_sum = 0, which the
+//                   reference skips (src/loop.c:475-477); a live hit has em
+//                   <= -0 (sign bit set).  Slots past the list end get no-op
+//                   records (em = +0, N = 0, N_HeI = -1).
+//   wave 0 (walk)   one lane runs the transmission recurrence over segment s,
+//                       TM -= min(-TM * em, N_HI)          (hydrogen only)
+//                   evaluated as TM' = max(TM + TM * em, TM - N_HI): correctly
+//                   rounded subtraction is monotone in the subtrahend, so this
+//                   is the reference's value bit for bit, three dependent
+//                   operations per hit.  It stores TM only every 8 hits (a
+//                   checkpoint), and stops at the first group of 8 after which
+//                   TM < T0 and which holds a live hit; the cut-off "TM < 0 ||
+//                   TM / Nph < 1e-10" (src/loop.c:585) is exactly "TM < T0".
+//   wave 2 (apply)  for segment s - 1, each lane recomputes the TM its hit saw
+//                   from the checkpoint (at most 7 steps, the same operations,
+//                   so the same bits), resolves the exact cut hit in the
+//                   stopping group, and scatters the ionization / heating
+//                   deposits with f32 atomics (src/loop.c:486-573).
 //
-// Keeping the atomics out of the walk matters on gfx9: stores and atomics
-// share the in-order vmcnt counter with loads, so a wave that waits for its
-// next chunk's loads also waits for every earlier atomic to reach memory.
+// The walk is the critical path (about 35 cycles per hit at 2.4 GHz); prep
+// and apply run beside it.  The per-hit values never leave LDS, and the wave
+// that issues atomics never waits on global loads (stores and atomics share
+// the in-order vmcnt counter with loads on gfx9).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -59,160 +58,35 @@ __device__ __forceinline__ double readlane_d(double v, int j) {
 
 __device__ __forceinline__ bool is_live(double em) { return signbit(em); }
 
-constexpr int DEP_THREADS = 256;
+constexpr int FW = 3;                // wavefronts: walk, prep, apply
+constexpr int FTHREADS = 64 * FW;
+constexpr int FSEG = 128;            // hits per segment
+constexpr int FG = 8;                // checkpoint interval of the walk
+constexpr int FNCK = FSEG / FG;
 
-// ---------------------------------------------------------------- prep
-// src/loop.c:427-477 per hit
+// walk record of one hit (16-byte aligned: one LDS b128 read per pair)
 template <bool HE>
-__global__ __launch_bounds__(DEP_THREADS) void k_dep_prep(const psr_ray *__restrict__ rays, int64_t nray,
-                                                          DevHits H, DevDep D, const DevCtx *__restrict__ C) {
-    if (halted(D.halt)) return;
-    const DevState &S = C->S;
-    const DevTables &T = C->T;
-    const DevPhys &P = C->P;
-    for (int64_t r = blockIdx.x; r < nray; r += gridDim.x) {
-        const int n = H.count[r];
-        const double Nph = rays[r].Nph;
-        if (n == 0 || Nph == 0) continue;
-        const double freq = rays[r].freq;
-        const double sigmaHI = xs_get(T, 0, freq);
-        const double sigmaHeI = HE ? xs_get(T, 1, freq) : 0.0;
-        const double sigmaHeII = HE ? xs_get(T, 2, freq) : 0.0;
-        const int64_t base = r * (int64_t)H.cap;
-        // no-op entries up to the next multiple of 64 (inside the capacity)
-        const int npad = (n + 63) & ~63;
-        for (int k = n + threadIdx.x; k < npad; k += DEP_THREADS) {
-            const int64_t q = base + k;
-            D.em[q] = 0.0;
-            D.nhi[q] = 0.0;
-            if (HE) {
-                D.fhi[q] = 0.0;
-                D.fhei[q] = 0.0;
-                D.fheii[q] = 0.0;
-                D.nhei[q] = -1.0;
-                D.nheii[q] = 0.0;
-            }
-        }
-        for (int k = threadIdx.x; k < n; k += DEP_THREADS) {
-            const int64_t q = base + k;
-            const uint32_t slot = H.sslot[q];
-            const double b = H.sb[q];
-            const double sml_inv = 1.0 / (double)S.ps[slot].w;
-            const double mass = (double)S.mass[slot];
-            const double xHI = S.lH[slot];
-            const double NH = mass * P.C_H_PER_MASS;
-            const double NHe = mass * P.C_HE_PER_MASS;
-            const double kernel = sph_depth(T, b * sml_inv) * (sml_inv * sml_inv) * P.scaling_fac2_inv;
-            const double NHI = xHI * NH;
-            const double tauHI = sigmaHI * (kernel * NHI);
-            double tausum = tauHI;
-            double tauHeI = 0, tauHeII = 0, NHeI = 0, NHeII = 0;
-            const bool he = HE && NHe != 0.0;
-            if (he) {
-                NHeI = S.lHeI[slot] * NHe;
-                NHeII = S.lHeII[slot] * NHe;
-                tauHeI = sigmaHeI * (kernel * NHeI);
-                tausum += tauHeI;
-                tauHeII = sigmaHeII * (kernel * NHeII);
-                tausum += tauHeII;
-            }
-            const bool live = tausum != 0.0;
-            // expm1 of a negative argument is <= -0, so the sign bit marks live
-            D.em[q] = live ? expm1(-tausum) : 0.0;
-            D.nhi[q] = live ? NHI : 0.0;
-            if (HE) {
-                D.fhi[q] = live ? tauHI / tausum : 0.0;
-                D.fhei[q] = (live && he) ? tauHeI / tausum : 0.0;
-                D.fheii[q] = (live && he) ? tauHeII / tausum : 0.0;
-                D.nhei[q] = (live && he) ? NHeI : -1.0;  // -1: no helium terms
-                D.nheii[q] = (live && he) ? NHeII : 0.0;
-            }
-        }
-    }
-}
-
-// ---------------------------------------------------------------- walk
-template <bool HE>
-struct WalkChunk {
-    double em, nhi;
-    double fhi, fhei, fheii, nhei, nheii;
-};
-
-// plain loads, no branch or select next to them (either makes the compiler
-// wait for the load at once); the index is clamped into the ray's slots, and
-// a chunk that is walked lies below the padded list end, where prep wrote
-// no-op entries past n
-template <bool HE>
-__device__ __forceinline__ WalkChunk<HE> walk_load(const DevDep &D, int64_t base, int k, int cap) {
-    WalkChunk<HE> w;
-    const int64_t q = base + min(k, cap - 1);
-    w.em = D.em[q];
-    w.nhi = D.nhi[q];
-    if (HE) {
-        w.fhi = D.fhi[q];
-        w.fhei = D.fhei[q];
-        w.fheii = D.fheii[q];
-        w.nhei = D.nhei[q];
-        w.nheii = D.nheii[q];
-    } else {
-        w.fhi = w.fhei = w.fheii = w.nheii = 0.0;
-        w.nhei = -1.0;
-    }
-    return w;
-}
-
-// block sum of (count, walked, lost) into this block's counter copy
-__device__ __forceinline__ void walk_stats(DevStats *st0, unsigned long long cnt, unsigned long long walked,
-                                           double lost) {
-    __shared__ unsigned long long pc[DEP_THREADS / 64], pw[DEP_THREADS / 64];
-    __shared__ double pl[DEP_THREADS / 64];
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        pc[w] = cnt;
-        pw[w] = walked;
-        pl[w] = lost;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long c = 0, k = 0;
-        double l = 0;
-        for (int i = 0; i < DEP_THREADS / 64; i++) {
-            c += pc[i];
-            k += pw[i];
-            l += pl[i];
-        }
-        if (c) {
-            DevStats *st = stat_slot(st0);
-            atomicAdd(&st->total_deposit_count, c);
-            atomicAdd(&st->deposit_processed, k);
-            atomicAdd(&st->lost_photon_count_sum, l);
-        }
-    }
-}
-
-// LDS record of one hit for the walk
-template <bool HE>
-struct WalkRec {
+struct __attribute__((aligned(16))) WalkRec {
     double em, nhi;
 };
 template <>
-struct WalkRec<true> {
+struct __attribute__((aligned(16))) WalkRec<true> {
     double em, nhi, fhi, fhei, fheii, nhei, nheii, pad;
 };
 
-// a segment of SEG hits, 64 per register set, lane l holding hit q * 64 + l
+// what apply needs besides the walk record
 template <bool HE>
-struct WalkSeg {
-    static constexpr int K = HE ? 2 : 4;
-    static constexpr int SEG = 64 * K;
-    WalkChunk<HE> c[K];
+struct HitAux {
+    uint32_t slot;
+    float mass;
+    double xHI;
 };
-
-template <bool HE>
-__device__ __forceinline__ void seg_load(WalkSeg<HE> &s, const DevDep &D, int64_t base, int k0, int lane, int cap) {
-#pragma unroll
-    for (int q = 0; q < WalkSeg<HE>::K; q++) s.c[q] = walk_load<HE>(D, base, k0 + q * 64 + lane, cap);
-}
+template <>
+struct HitAux<true> {
+    uint32_t slot;
+    float mass;
+    double xHI, xHeI, xHeII;
+};
 
 // one step of the transmission recurrence, src/loop.c:479-504, 587
 template <bool HE>
@@ -236,338 +110,470 @@ __device__ __forceinline__ double walk_step<true>(double TM, const WalkRec<true>
 }
 
 template <bool HE>
-__device__ __forceinline__ void put_rec(WalkRec<HE> &r, const WalkChunk<HE> &c) {
-    r.em = c.em;
-    r.nhi = c.nhi;
+__device__ __forceinline__ void noop_rec(WalkRec<HE> &r, HitAux<HE> &a) {
+    r.em = 0.0;
+    r.nhi = 0.0;
     if constexpr (HE) {
-        r.fhi = c.fhi;
-        r.fhei = c.fhei;
-        r.fheii = c.fheii;
-        r.nhei = c.nhei;
-        r.nheii = c.nheii;
+        r.fhi = r.fhei = r.fheii = r.nheii = 0.0;
+        r.nhei = -1.0;
+        a.xHeI = a.xHeII = 0.0;
     }
+    a.slot = 0;
+    a.mass = 0.0f;
+    a.xHI = 0.0;
 }
+
+// per-ray constants
+struct RayConst {
+    double sigmaHI, sigmaHeI, sigmaHeII, dEHI, dEHeI, dEHeII, hf_HI, hf_HeI, hf_HeII;
+    bool secion_terms;
+};
 
 template <bool HE>
-__global__ __launch_bounds__(DEP_THREADS) void k_dep_walk(const psr_ray *__restrict__ rays, int64_t nray,
-                                                          DevHits H, DevDep D, const DevCtx *__restrict__ C,
-                                                          DevStats *__restrict__ st0, double *__restrict__ ray_len,
-                                                          double *__restrict__ ray_tm) {
-    using Seg = WalkSeg<HE>;
-    constexpr int SEG = Seg::SEG;
-    constexpr int WAVES = DEP_THREADS / 64;
-    __shared__ WalkRec<HE> rec[WAVES][SEG];
-    __shared__ double tml[WAVES][SEG + 1];
-    if (halted(D.halt)) return;
-    const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const int wave0 = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * WAVES + w));
-    const int nwaves = gridDim.x * WAVES;
-    const int no_photon = C->P.no_photon;
-    // this wave's totals, kept by lane 0
-    unsigned long long cnt = 0, walked = 0;
-    double lost = 0.0;
-    for (int64_t r = wave0; r < nray; r += nwaves) {
-        const int n = __builtin_amdgcn_readfirstlane(H.count[r]);
-        const double Nph = rays[r].Nph;
-        if (n == 0 || Nph == 0) {
-            if (lane == 0) {
-                ray_len[r] = rays[r].length;
-                ray_tm[r] = Nph;
-                D.last[r] = -1;
-            }
-            continue;
-        }
-        const int64_t base = r * (int64_t)H.cap;
-        double TM = (no_photon && rays[r].type < 0) ? 0.0 : Nph;
-        const double T0 = cutoff_threshold(Nph);
-        int last = n - 1;
-        double *tmb = D.tmb + base;
-        const int cap = H.cap;
-        const int npad = (n + 63) & ~63;
-        Seg cur, nxt;
-        seg_load<HE>(cur, D, base, 0, lane, cap);
-        bool cut = false;
-        for (int s0 = 0; s0 < npad && !cut; s0 += SEG) {
-            // stage the segment in LDS, then start loading the next one
-#pragma unroll
-            for (int q = 0; q < Seg::K; q++) put_rec<HE>(rec[w][q * 64 + lane], cur.c[q]);
-            seg_load<HE>(nxt, D, base, s0 + SEG, lane, cap);
-            __builtin_amdgcn_wave_barrier();
-            // the serial recurrence on one lane: an LDS broadcast read, the
-            // step, and an LDS write of the TM the hit saw
-            const int m = min(SEG, npad - s0);
-            if (lane == 0) {
-                double t = TM;
-#pragma unroll 8
-                for (int j = 0; j < m; j++) {
-                    const WalkRec<HE> v = rec[w][j];
-                    tml[w][j] = t;
-                    t = walk_step<HE>(t, v);
-                }
-                tml[w][m] = t;
-            }
-            __builtin_amdgcn_wave_barrier();
-            TM = tml[w][m];
-            // TM after hit j is TM before hit j + 1 and never increases: the
-            // cut is the first live hit after which TM < T0
-#pragma unroll
-            for (int q = 0; q < Seg::K; q++) {
-                if (q * 64 >= m) break;
-                const int c0 = s0 + q * 64;
-                const double tb = tml[w][q * 64 + lane];
-                const double ta = tml[w][q * 64 + lane + 1];
-                // the hit capacity is a multiple of 64 and c0 < n <= capacity,
-                // so the chunk lies inside this ray's slots
-                tmb[c0 + lane] = tb;
-                const unsigned long long cm = __ballot(ta < T0 && c0 + lane < n && is_live(cur.c[q].em));
-                if (cm) {
-                    const int j = __ffsll((long long)cm) - 1;
-                    last = c0 + j;
-                    TM = readlane_d(ta, j);
-                    cut = true;
-                    break;
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
-            cur = nxt;
-        }
-        if (lane == 0) {
-            ray_len[r] = H.d[base + H.perm[base + last]];
-            ray_tm[r] = TM;
-            D.last[r] = last;
-            cnt += (unsigned long long)n;
-            walked += (unsigned long long)(last + 1);
-            lost += TM;
-        }
-    }
-    walk_stats(st0, cnt, walked, lost);
+__device__ __forceinline__ RayConst ray_const(double freq, const DevTables &T, const DevPhys &P) {
+    RayConst k;
+    k.sigmaHI = xs_get(T, 0, freq);
+    k.sigmaHeI = HE ? xs_get(T, 1, freq) : 0.0;
+    k.sigmaHeII = HE ? xs_get(T, 2, freq) : 0.0;
+    k.dEHI = fdim(freq, P.C_HI_ENERGY);
+    k.dEHeI = fdim(freq, P.C_HEI_ENERGY);
+    k.dEHeII = fdim(freq, P.C_HEII_ENERGY);
+    k.hf_HI = P.C_H_PER_MASS * k.dEHI;
+    k.hf_HeI = P.C_HE_PER_MASS * k.dEHeI;
+    k.hf_HeII = P.C_HE_PER_MASS * k.dEHeII;
+    k.secion_terms = P.secondary_ionization && (k.dEHI > 0 || k.dEHeI > 0) && T.si_n1 > 0;
+    return k;
 }
 
-// ---------------------------------------------------------------- apply
+// particle data of one hit, gathered one pipeline stage before its use
+template <bool HE>
+struct HitGather {
+    uint32_t slot;
+    float sml, mass;
+    double b, xHI, xHeI, xHeII;
+};
+
+template <bool HE>
+__device__ __forceinline__ void gather_hit(HitGather<HE> &g, const DevState &S) {
+    g.sml = gmem(S.ps)[g.slot].w;
+    g.mass = gmem(S.mass)[g.slot];
+    g.xHI = gmem(S.lH)[g.slot];
+    if constexpr (HE) {
+        g.xHeI = gmem(S.lHeI)[g.slot];
+        g.xHeII = gmem(S.lHeII)[g.slot];
+    }
+}
+
+// src/loop.c:427-477 for one hit
+// (sdepth = sph_depth(b / sml), looked up by the caller so that the table
+// loads of several hits are in flight together)
+template <bool HE>
+__device__ __forceinline__ void prep_hit(const HitGather<HE> &g, double sdepth, const DevPhys &P,
+                                         const RayConst &k, WalkRec<HE> &r, HitAux<HE> &a) {
+    const uint32_t slot = g.slot;
+    const double sml_inv = 1.0 / (double)g.sml;
+    const float massf = g.mass;
+    const double mass = (double)massf;
+    const double xHI = g.xHI;
+    const double NH = mass * P.C_H_PER_MASS;
+    const double NHe = mass * P.C_HE_PER_MASS;
+    const double kernel = sdepth * (sml_inv * sml_inv) * P.scaling_fac2_inv;
+    const double NHI = xHI * NH;
+    const double tauHI = k.sigmaHI * (kernel * NHI);
+    double tausum = tauHI;
+    double tauHeI = 0, tauHeII = 0, NHeI = 0, NHeII = 0, xHeI = 0, xHeII = 0;
+    const bool he = HE && NHe != 0.0;
+    if (HE) {
+        xHeI = g.xHeI;
+        xHeII = g.xHeII;
+    }
+    if (he) {
+        NHeI = xHeI * NHe;
+        NHeII = xHeII * NHe;
+        tauHeI = k.sigmaHeI * (kernel * NHeI);
+        tausum += tauHeI;
+        tauHeII = k.sigmaHeII * (kernel * NHeII);
+        tausum += tauHeII;
+    }
+    const bool live = tausum != 0.0;
+    // expm1 of a negative argument is <= -0, so the sign bit marks live
+    r.em = live ? expm1(-tausum) : 0.0;
+    r.nhi = live ? NHI : 0.0;
+    if constexpr (HE) {
+        r.fhi = live ? tauHI / tausum : 0.0;
+        r.fhei = (live && he) ? tauHeI / tausum : 0.0;
+        r.fheii = (live && he) ? tauHeII / tausum : 0.0;
+        r.nhei = (live && he) ? NHeI : -1.0;  // -1: no helium terms
+        r.nheii = (live && he) ? NHeII : 0.0;
+        a.xHeI = xHeI;
+        a.xHeII = xHeII;
+    }
+    a.slot = slot;
+    a.mass = massf;
+    a.xHI = xHI;
+}
+
 struct ApplySums {
     int nsat;
     double fiHI, fiHeI, fiHeII, siHI, siHeI;
 };
 
+// src/loop.c:479-573 for one live hit that saw TM = tm
 template <bool HE>
-__global__ __launch_bounds__(DEP_THREADS) void k_dep_apply(const psr_ray *__restrict__ rays, int64_t nray,
-                                                           DevHits H, DevDep D, const DevCtx *__restrict__ C,
-                                                           DevStats *__restrict__ st0) {
+__device__ __forceinline__ void apply_hit(double tm, const WalkRec<HE> &v, const HitAux<HE> &au,
+                                          const DevState &S, const DevTables &T, const DevPhys &P,
+                                          const RayConst &k, ApplySums &acc) {
+    const uint32_t slot = au.slot;
+    const double em = v.em;
+    const double NHI = v.nhi;
+    const double mass = (double)au.mass;
+    const double NH = mass * P.C_H_PER_MASS;
+    const double NHe = mass * P.C_HE_PER_MASS;
+    const double NH_inv = 1.0 / NH;
+    const double NHe_inv = 1.0 / NHe;
+    // the walk's operations on the TM this hit saw
+    const double absorb_est = -tm * em;
+    double fHI = 1.0, fHeI = 0.0, fHeII = 0.0, NHeI = 0.0, NHeII = 0.0;
+    bool he = false;
+    if constexpr (HE) {
+        fHI = v.fhi;
+        NHeI = v.nhei;
+        he = NHeI >= 0.0;
+        if (he) {
+            fHeI = v.fhei;
+            fHeII = v.fheii;
+            NHeII = v.nheii;
+        }
+    }
+    double aHI = HE ? fHI * absorb_est : absorb_est;
+    if (aHI > NHI) {
+        aHI = NHI;
+        acc.nsat++;
+    }
+    double aHeI = 0, aHeII = 0;
+    if (he) {
+        aHeI = fHeI * absorb_est;
+        if (aHeI > NHeI) {
+            aHeI = NHeI;
+            acc.nsat++;
+        }
+        aHeII = fHeII * absorb_est;
+        if (aHeII > NHeII) {
+            aHeII = NHeII;
+            acc.nsat++;
+        }
+    }
+    const double deltaHI = aHI * NH_inv;
+    const double deltaHeI = he ? aHeI * NHe_inv : 0.0;
+    const double deltaHeII = he ? aHeII * NHe_inv : 0.0;
+    double x = 0.0;
+    if (P.secondary_ionization) {
+        const double xHI = au.xHI;
+        double xHeI = 0.0, xHeII = 0.0;
+        if constexpr (HE) {
+            xHeI = au.xHeI;
+            xHeII = au.xHeII;
+        }
+        const double ye = (1.0 - xHI) + P.C_HEMF / P.C_HMF * 0.25 * (xHeII + 2 * (1.0 - xHeII - xHeI));
+        x = ye * NH / (NH + NHe);
+        if (x > 1.0) x = 1.0;
+    }
+    if (P.secondary_ionization && x > 0.0) {
+        atomicAdd(&S.yGdepHI[slot], (float)deltaHI);
+        acc.fiHI += deltaHI * NH;
+        if (k.secion_terms) {
+            const double log10x = log10(x);
+            const double sec = deltaHI * k.dEHI / P.C_HI_ENERGY * tab2_get(T, T.si_phi_hi, k.dEHI, log10x) +
+                               deltaHeI * k.dEHeI / P.C_HI_ENERGY * tab2_get(T, T.si_phi_hi, k.dEHeI, log10x);
+            atomicAdd(&S.yGdepHI[slot], (float)sec);
+            acc.siHI += sec * NH;
+            atomicAdd(&S.heat[slot], (float)(k.hf_HI * deltaHI * tab2_get(T, T.si_eh, k.dEHI, log10x)));
+            if (he) {
+                const double sec2 =
+                    deltaHI * k.dEHI / P.C_HEI_ENERGY * tab2_get(T, T.si_phi_hei, k.dEHI, log10x) +
+                    deltaHeI * k.dEHeI / P.C_HEI_ENERGY * tab2_get(T, T.si_phi_hei, k.dEHeI, log10x);
+                atomicAdd(&S.yGdepHeI[slot], (float)(deltaHeI + sec2));
+                acc.siHeI += sec2 * NHe;
+                acc.fiHeI += deltaHeI * NHe;
+                atomicAdd(&S.heat[slot], (float)(k.hf_HeI * deltaHeI * tab2_get(T, T.si_eh, k.dEHeI, log10x)));
+                atomicAdd(&S.yGdepHeII[slot], (float)deltaHeII);
+                acc.fiHeII += deltaHeII * NHe;
+            }
+        } else if (he) {
+            atomicAdd(&S.yGdepHeI[slot], (float)deltaHeI);
+            acc.fiHeI += deltaHeI * NHe;
+            atomicAdd(&S.yGdepHeII[slot], (float)deltaHeII);
+            acc.fiHeII += deltaHeII * NHe;
+        }
+    } else {
+        atomicAdd(&S.yGdepHI[slot], (float)deltaHI);
+        if (k.hf_HI != 0.0) atomicAdd(&S.heat[slot], (float)(k.hf_HI * deltaHI));
+        if (he) {
+            atomicAdd(&S.yGdepHeI[slot], (float)deltaHeI);
+            atomicAdd(&S.yGdepHeII[slot], (float)deltaHeII);
+            if (k.hf_HeI != 0.0) atomicAdd(&S.heat[slot], (float)(deltaHeI * k.hf_HeI));
+            if (k.hf_HeII != 0.0) atomicAdd(&S.heat[slot], (float)(deltaHeII * k.hf_HeII));
+        }
+    }
+    atomicAdd(&S.hits[slot], 1);
+}
+
+template <bool HE>
+struct DepLds {
+    WalkRec<HE> rec[3][FSEG];
+    HitAux<HE> aux[3][FSEG];
+    double ck[3][FNCK];       // TM before hit FG * g of the segment
+    int32_t stop_seg, stop_grp, last;
+    double tm_end;
+};
+
+template <bool HE>
+__global__ __launch_bounds__(FTHREADS) void k_deposit(const psr_ray *__restrict__ rays, int64_t nray, DevHits H,
+                                                      const DevCtx *__restrict__ C, DevStats *__restrict__ st0,
+                                                      double *__restrict__ ray_len, double *__restrict__ ray_tm,
+                                                      const int32_t *halt) {
+    __shared__ DepLds<HE> L;
+    if (halted(halt)) return;
     const DevState &S = C->S;
     const DevTables &T = C->T;
     const DevPhys &P = C->P;
-    if (halted(D.halt)) return;
+    const int lane = threadIdx.x & 63;
+    // 0 walk, 1 prep, 2 apply; wave-uniform, so each role runs its own loop
+    // (the three loops pass the same number of barriers)
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     ApplySums acc{0, 0, 0, 0, 0, 0};
+    unsigned long long cnt = 0, walked = 0;
+    double lost = 0.0;
+    constexpr int HPL = FSEG / 64;  // hits per lane and segment
     for (int64_t r = blockIdx.x; r < nray; r += gridDim.x) {
-        const int last = D.last[r];
-        if (last < 0) continue;
-        const double freq = rays[r].freq;
-        const double dEHI = fdim(freq, P.C_HI_ENERGY);
-        const double dEHeI = fdim(freq, P.C_HEI_ENERGY);
-        const double dEHeII = fdim(freq, P.C_HEII_ENERGY);
-        const double hf_HI = P.C_H_PER_MASS * dEHI;
-        const double hf_HeI = P.C_HE_PER_MASS * dEHeI;
-        const double hf_HeII = P.C_HE_PER_MASS * dEHeII;
-        const bool secion_terms = P.secondary_ionization && (dEHI > 0 || dEHeI > 0) && T.si_n1 > 0;
-        const int64_t base = r * (int64_t)H.cap;
-        for (int k = threadIdx.x; k <= last; k += DEP_THREADS) {
-            const int64_t q = base + k;
-            const double em = D.em[q];
-            if (!is_live(em)) continue;
-            const uint32_t slot = H.sslot[q];
-            const double NHI = D.nhi[q];
-            const double tm = D.tmb[q];
-            const double mass = (double)S.mass[slot];
-            const double NH = mass * P.C_H_PER_MASS;
-            const double NHe = mass * P.C_HE_PER_MASS;
-            const double NH_inv = 1.0 / NH;
-            const double NHe_inv = 1.0 / NHe;
-            // the walk's operations on the TM this hit saw
-            const double absorb_est = -tm * em;
-            double fHI = 1.0, fHeI = 0.0, fHeII = 0.0, NHeI = 0.0, NHeII = 0.0;
-            bool he = false;
-            if (HE) {
-                fHI = D.fhi[q];
-                NHeI = D.nhei[q];
-                he = NHeI >= 0.0;
-                if (he) {
-                    fHeI = D.fhei[q];
-                    fHeII = D.fheii[q];
-                    NHeII = D.nheii[q];
-                }
+        const int n = H.count[r];
+        const double Nph = rays[r].Nph;
+        if (n == 0 || Nph == 0) {
+            if (threadIdx.x == 0) {
+                ray_len[r] = rays[r].length;
+                ray_tm[r] = Nph;
             }
-            double aHI = HE ? fHI * absorb_est : absorb_est;
-            if (aHI > NHI) {
-                aHI = NHI;
-                acc.nsat++;
-            }
-            double aHeI = 0, aHeII = 0;
-            if (he) {
-                aHeI = fHeI * absorb_est;
-                if (aHeI > NHeI) {
-                    aHeI = NHeI;
-                    acc.nsat++;
-                }
-                aHeII = fHeII * absorb_est;
-                if (aHeII > NHeII) {
-                    aHeII = NHeII;
-                    acc.nsat++;
-                }
-            }
-            const double deltaHI = aHI * NH_inv;
-            const double deltaHeI = he ? aHeI * NHe_inv : 0.0;
-            const double deltaHeII = he ? aHeII * NHe_inv : 0.0;
-            double x = 0.0;
-            if (P.secondary_ionization) {
-                const double xHI = S.lH[slot];
-                const double xHeI = HE ? S.lHeI[slot] : 0.0;
-                const double xHeII = HE ? S.lHeII[slot] : 0.0;
-                const double ye = (1.0 - xHI) + P.C_HEMF / P.C_HMF * 0.25 * (xHeII + 2 * (1.0 - xHeII - xHeI));
-                x = ye * NH / (NH + NHe);
-                if (x > 1.0) x = 1.0;
-            }
-            if (P.secondary_ionization && x > 0.0) {
-                atomicAdd(&S.yGdepHI[slot], (float)deltaHI);
-                acc.fiHI += deltaHI * NH;
-                if (secion_terms) {
-                    const double log10x = log10(x);
-                    const double sec = deltaHI * dEHI / P.C_HI_ENERGY * tab2_get(T, T.si_phi_hi, dEHI, log10x) +
-                                       deltaHeI * dEHeI / P.C_HI_ENERGY * tab2_get(T, T.si_phi_hi, dEHeI, log10x);
-                    atomicAdd(&S.yGdepHI[slot], (float)sec);
-                    acc.siHI += sec * NH;
-                    atomicAdd(&S.heat[slot], (float)(hf_HI * deltaHI * tab2_get(T, T.si_eh, dEHI, log10x)));
-                    if (he) {
-                        const double sec2 =
-                            deltaHI * dEHI / P.C_HEI_ENERGY * tab2_get(T, T.si_phi_hei, dEHI, log10x) +
-                            deltaHeI * dEHeI / P.C_HEI_ENERGY * tab2_get(T, T.si_phi_hei, dEHeI, log10x);
-                        atomicAdd(&S.yGdepHeI[slot], (float)(deltaHeI + sec2));
-                        acc.siHeI += sec2 * NHe;
-                        acc.fiHeI += deltaHeI * NHe;
-                        atomicAdd(&S.heat[slot], (float)(hf_HeI * deltaHeI * tab2_get(T, T.si_eh, dEHeI, log10x)));
-                        atomicAdd(&S.yGdepHeII[slot], (float)deltaHeII);
-                        acc.fiHeII += deltaHeII * NHe;
-                    }
-                } else if (he) {
-                    atomicAdd(&S.yGdepHeI[slot], (float)deltaHeI);
-                    acc.fiHeI += deltaHeI * NHe;
-                    atomicAdd(&S.yGdepHeII[slot], (float)deltaHeII);
-                    acc.fiHeII += deltaHeII * NHe;
-                }
-            } else {
-                atomicAdd(&S.yGdepHI[slot], (float)deltaHI);
-                if (hf_HI != 0.0) atomicAdd(&S.heat[slot], (float)(hf_HI * deltaHI));
-                if (he) {
-                    atomicAdd(&S.yGdepHeI[slot], (float)deltaHeI);
-                    atomicAdd(&S.yGdepHeII[slot], (float)deltaHeII);
-                    if (hf_HeI != 0.0) atomicAdd(&S.heat[slot], (float)(deltaHeI * hf_HeI));
-                    if (hf_HeII != 0.0) atomicAdd(&S.heat[slot], (float)(deltaHeII * hf_HeII));
-                }
-            }
-            atomicAdd(&S.hits[slot], 1);
+            continue;
         }
+        const int64_t base = r * (int64_t)H.cap;
+        const double T0 = cutoff_threshold(Nph);
+        const RayConst kc = ray_const<HE>(rays[r].freq, T, P);
+        double TM = (P.no_photon && rays[r].type < 0) ? 0.0 : Nph;  // carried by the walk lane
+        const int nseg = (n + FSEG - 1) / FSEG;
+        if (threadIdx.x == 0) {
+            L.stop_seg = -1;
+            L.stop_grp = -1;
+            L.last = n - 1;
+            L.tm_end = TM;
+        }
+        __syncthreads();
+        if (wv == 1) {
+            // three-stage pipeline: sorted slot / b of segment i + 2 and the
+            // particle gathers of segment i + 1 are in flight while segment i
+            // is computed; each carried set is used, then reloaded (indexes
+            // clamped into the list, so every load is in bounds)
+            uint32_t a_slot[HPL];
+            double a_b[HPL];
+            HitGather<HE> gc[HPL];
+#pragma unroll
+            for (int h = 0; h < HPL; h++) {
+                const int k0 = min(h * 64 + lane, n - 1);
+                gc[h].slot = H.sslot[base + k0];
+                gc[h].b = H.sb[base + k0];
+                gather_hit<HE>(gc[h], S);
+                const int k1 = min(FSEG + h * 64 + lane, n - 1);
+                a_slot[h] = H.sslot[base + k1];
+                a_b[h] = H.sb[base + k1];
+            }
+            // every iteration runs the same straight-line code (also past the
+            // list end or the cut, on clamped indexes, into the ring slot no
+            // other wave reads in that iteration), so the wait counts stay exact
+            for (int i = 0;; i++) {
+                const int stop = L.stop_seg;
+                if (stop >= 0 ? i > stop + 2 : i > nseg + 1) break;
+                const int ring = i % 3;
+                double sd[HPL];
+#pragma unroll
+                for (int h = 0; h < HPL; h++) sd[h] = sph_depth(T, gc[h].b * (1.0 / (double)gc[h].sml));
+#pragma unroll
+                for (int h = 0; h < HPL; h++) {
+                    const int jj = h * 64 + lane;
+                    WalkRec<HE> rr;
+                    HitAux<HE> au;
+                    prep_hit<HE>(gc[h], sd[h], P, kc, rr, au);
+                    if (i * FSEG + jj >= n) noop_rec<HE>(rr, au);
+                    L.rec[ring][jj] = rr;
+                    L.aux[ring][jj] = au;
+                }
+#pragma unroll
+                for (int h = 0; h < HPL; h++) {
+                    gc[h].slot = a_slot[h];
+                    gc[h].b = a_b[h];
+                    gather_hit<HE>(gc[h], S);
+                    const int k2 = min((i + 2) * FSEG + h * 64 + lane, n - 1);
+                    a_slot[h] = gmem(H.sslot)[base + k2];
+                    a_b[h] = gmem(H.sb)[base + k2];
+                }
+                __syncthreads();
+            }
+        } else if (wv == 0) {
+            // the walk is the critical path: first claim on issue slots
+            __builtin_amdgcn_s_setprio(3);
+            for (int i = 0;; i++) {
+                const int stop = L.stop_seg;
+                if (stop >= 0 ? i > stop + 2 : i > nseg + 1) break;
+                const int s = i - 1;
+                if (s >= 0 && s < nseg && stop < 0 && lane == 0) {
+                    const int ring = s % 3;
+                    const int ngrp = min(FNCK, (n - s * FSEG + FG - 1) / FG);
+                    double t = TM;
+                    // double-buffered groups: the next group's records are
+                    // read (pinned ahead by a scheduling barrier) while this
+                    // group's steps run, so LDS latency stays off the chain.
+                    // TM changes only at live hits, so below T0 after a group
+                    // with a live hit means the cut is in that group.
+                    WalkRec<HE> A[FG], B[FG];
+#pragma unroll
+                    for (int q = 0; q < FG; q++) A[q] = L.rec[ring][q];
+                    for (int g = 0; g < ngrp; g += 2) {
+                        const int g1 = min(g + 1, FNCK - 1);
+#pragma unroll
+                        for (int q = 0; q < FG; q++) B[q] = L.rec[ring][g1 * FG + q];
+                        __builtin_amdgcn_sched_barrier(0);
+                        L.ck[ring][g] = t;
+                        bool seen = false;
+#pragma unroll
+                        for (int q = 0; q < FG; q++) {
+                            seen = seen | is_live(A[q].em);
+                            t = walk_step<HE>(t, A[q]);
+                        }
+                        if (t < T0 && seen) {
+                            L.stop_seg = s;
+                            L.stop_grp = g;
+                            break;
+                        }
+                        if (g + 1 >= ngrp) break;
+                        const int g2 = min(g + 2, FNCK - 1);
+#pragma unroll
+                        for (int q = 0; q < FG; q++) A[q] = L.rec[ring][g2 * FG + q];
+                        __builtin_amdgcn_sched_barrier(0);
+                        L.ck[ring][g + 1] = t;
+                        seen = false;
+#pragma unroll
+                        for (int q = 0; q < FG; q++) {
+                            seen = seen | is_live(B[q].em);
+                            t = walk_step<HE>(t, B[q]);
+                        }
+                        if (t < T0 && seen) {
+                            L.stop_seg = s;
+                            L.stop_grp = g + 1;
+                            break;
+                        }
+                    }
+                    TM = t;
+                    if (s == nseg - 1) L.tm_end = t;
+                }
+                __syncthreads();
+            }
+            __builtin_amdgcn_s_setprio(0);
+        } else {
+            for (int i = 0;; i++) {
+                const int stop = L.stop_seg;
+                if (stop >= 0 ? i > stop + 2 : i > nseg + 1) break;
+                const int s = i - 2;
+                if (s >= 0 && s < nseg && (stop < 0 || s <= stop)) {
+                    const int ring = s % 3;
+                    int lim = min(FSEG, n - s * FSEG) - 1;  // last hit of the segment to apply
+                    if (s == stop) {
+                        // the exact cut: first live hit of the stopping group
+                        // after which TM < T0
+                        const int g = L.stop_grp;
+                        const int j = g * FG + (lane & (FG - 1));
+                        double pre = L.ck[ring][g];
+                        for (int q = g * FG; q < j; q++) pre = walk_step<HE>(pre, L.rec[ring][q]);
+                        const WalkRec<HE> v = L.rec[ring][j];
+                        const double post = walk_step<HE>(pre, v);
+                        const unsigned long long m = __ballot(lane < FG && is_live(v.em) && post < T0);
+                        const int jc = __ffsll((long long)m) - 1;
+                        lim = g * FG + jc;
+                        const double tm_cut = readlane_d(post, jc);
+                        if (lane == 0) {
+                            L.last = s * FSEG + lim;
+                            L.tm_end = tm_cut;
+                        }
+                    }
+#pragma unroll
+                    for (int h = 0; h < FSEG / 64; h++) {
+                        const int jj = h * 64 + lane;
+                        if (jj <= lim) {
+                            const WalkRec<HE> v = L.rec[ring][jj];
+                            if (is_live(v.em)) {
+                                const int g0 = jj & ~(FG - 1);
+                                double pre = L.ck[ring][g0 / FG];
+                                for (int q = g0; q < jj; q++) pre = walk_step<HE>(pre, L.rec[ring][q]);
+                                apply_hit<HE>(pre, v, L.aux[ring][jj], S, T, P, kc, acc);
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (threadIdx.x == 0) {
+            const int last = L.last;
+            const double tm = L.tm_end;
+            ray_len[r] = H.d[base + H.perm[base + last]];
+            ray_tm[r] = tm;
+            cnt += (unsigned long long)n;
+            walked += (unsigned long long)(last + 1);
+            lost += tm;
+        }
+        __syncthreads();
     }
-    // block totals, one atomic per counter and block
-    __shared__ double part[DEP_THREADS / 64][6];
-    const int w = threadIdx.x >> 6;
-    const double ns = (double)wave_sum_i(acc.nsat);
-    const double v1 = wave_sum(acc.fiHI), v2 = wave_sum(acc.siHI);
-    double v3 = 0, v4 = 0, v5 = 0;
-    if (HE) {
-        v3 = wave_sum(acc.fiHeI);
-        v4 = wave_sum(acc.fiHeII);
-        v5 = wave_sum(acc.siHeI);
+    // counters: thread 0 holds the ray totals, wave 2 the deposit sums
+    DevStats *st = stat_slot(st0);
+    if (threadIdx.x == 0 && cnt) {
+        atomicAdd(&st->total_deposit_count, cnt);
+        atomicAdd(&st->deposit_processed, walked);
+        atomicAdd(&st->lost_photon_count_sum, lost);
     }
-    if ((threadIdx.x & 63) == 0) {
-        part[w][0] = ns;
-        part[w][1] = v1;
-        part[w][2] = v2;
-        part[w][3] = v3;
-        part[w][4] = v4;
-        part[w][5] = v5;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t[6] = {0, 0, 0, 0, 0, 0};
-        for (int i = 0; i < DEP_THREADS / 64; i++)
-            for (int k = 0; k < 6; k++) t[k] += part[i][k];
-        DevStats *st = stat_slot(st0);
-        if (t[0] != 0) atomicAdd(&st->saturated_deposit_count, (unsigned long long)t[0]);
-        if (t[1] != 0) atomicAdd(&st->first_ionization_HI, t[1]);
-        if (t[2] != 0) atomicAdd(&st->secondary_ionization_HI, t[2]);
+    if (wv == 2) {
+        const int ns = wave_sum_i(acc.nsat);
+        const double v1 = wave_sum(acc.fiHI), v2 = wave_sum(acc.siHI);
+        double v3 = 0, v4 = 0, v5 = 0;
         if (HE) {
-            if (t[3] != 0) atomicAdd(&st->first_ionization_HeI, t[3]);
-            if (t[4] != 0) atomicAdd(&st->first_ionization_HeII, t[4]);
-            if (t[5] != 0) atomicAdd(&st->secondary_ionization_HeI, t[5]);
+            v3 = wave_sum(acc.fiHeI);
+            v4 = wave_sum(acc.fiHeII);
+            v5 = wave_sum(acc.siHeI);
+        }
+        if (lane == 0) {
+            if (ns) atomicAdd(&st->saturated_deposit_count, (unsigned long long)ns);
+            if (v1 != 0) atomicAdd(&st->first_ionization_HI, v1);
+            if (v2 != 0) atomicAdd(&st->secondary_ionization_HI, v2);
+            if (HE) {
+                if (v3 != 0) atomicAdd(&st->first_ionization_HeI, v3);
+                if (v4 != 0) atomicAdd(&st->first_ionization_HeII, v4);
+                if (v5 != 0) atomicAdd(&st->secondary_ionization_HeI, v5);
+            }
         }
     }
 }
 
 }  // namespace
 
-static int ensure_dep(psr_ctx *c, bool he, int64_t nray) {
-    const int64_t slots = c->hit_slots * (int64_t)c->H.cap;
-    const int32_t words = he ? 8 : 3;
-    if (c->H.cap % 64) PSR_FAIL(c, PSR_ERR_STATE, "deposit: hit capacity must be a multiple of 64");
-    if (slots > c->dep_slots || words > c->dep_words || nray > c->dep_rays) {
-        // kernels of earlier ticks may still use the old arrays
-        PSR_HIP(c, hipStreamSynchronize(c->stream));
-    }
-    if (slots > c->dep_slots || words > c->dep_words) {
-        if (c->d_dep) PSR_HIP(c, hipFree(c->d_dep));
-        c->d_dep = nullptr;
-        c->dep_slots = 0;
-        c->dep_words = 0;
-        PSR_HIP(c, hipMalloc(&c->d_dep, sizeof(double) * (size_t)slots * words));
-        c->dep_slots = slots;
-        c->dep_words = words;
-    }
-    if (nray > c->dep_rays) {
-        if (c->D.last) PSR_HIP(c, hipFree(c->D.last));
-        c->D.last = nullptr;
-        c->dep_rays = 0;
-        PSR_HIP(c, hipMalloc(&c->D.last, sizeof(int32_t) * std::max<int64_t>(nray, c->hit_slots)));
-        c->dep_rays = std::max<int64_t>(nray, c->hit_slots);
-    }
-    c->D.halt = &c->d_stats->flags;
-    double *p = c->d_dep;
-    const size_t s = (size_t)c->dep_slots;
-    c->D.em = p;
-    c->D.nhi = p + s;
-    c->D.tmb = p + 2 * s;
-    if (c->dep_words >= 8) {
-        c->D.fhi = p + 3 * s;
-        c->D.fhei = p + 4 * s;
-        c->D.fheii = p + 5 * s;
-        c->D.nhei = p + 6 * s;
-        c->D.nheii = p + 7 * s;
-    } else {
-        c->D.fhi = c->D.fhei = c->D.fheii = c->D.nhei = c->D.nheii = nullptr;
-    }
-    return PSR_OK;
-}
-
 int psr_launch_deposit(psr_ctx *c, int64_t ray_first, int64_t nray) {
     int rc = psr_sync_dev_ctx(c);
     if (rc) return rc;
-    const bool he = !c->F.h_only;
-    if ((rc = ensure_dep(c, he, nray))) return rc;
     const psr_ray *rays = c->d_rays + ray_first;
-    const unsigned gray = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nray, 65535 * 4));
-    const unsigned gwalk =
-        (unsigned)std::max<int64_t>(1, std::min<int64_t>((nray + DEP_THREADS / 64 - 1) / (DEP_THREADS / 64), 65535 * 4));
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nray, 65535 * 4));
+    const int32_t *halt = &c->d_stats->flags;
     PSR_EV(c, 2, 0);
-    if (he) {
-        k_dep_prep<true><<<gray, DEP_THREADS, 0, c->stream>>>(rays, nray, c->H, c->D, c->d_ctx);
-        k_dep_walk<true><<<gwalk, DEP_THREADS, 0, c->stream>>>(rays, nray, c->H, c->D, c->d_ctx, c->d_stats,
-                                                               c->d_ray_len, c->d_ray_tm);
-        k_dep_apply<true><<<gray, DEP_THREADS, 0, c->stream>>>(rays, nray, c->H, c->D, c->d_ctx, c->d_stats);
-    } else {
-        k_dep_prep<false><<<gray, DEP_THREADS, 0, c->stream>>>(rays, nray, c->H, c->D, c->d_ctx);
-        k_dep_walk<false><<<gwalk, DEP_THREADS, 0, c->stream>>>(rays, nray, c->H, c->D, c->d_ctx, c->d_stats,
-                                                                c->d_ray_len, c->d_ray_tm);
-        k_dep_apply<false><<<gray, DEP_THREADS, 0, c->stream>>>(rays, nray, c->H, c->D, c->d_ctx, c->d_stats);
-    }
+    if (!c->F.h_only)
+        k_deposit<true><<<grid, FTHREADS, 0, c->stream>>>(rays, nray, c->H, c->d_ctx, c->d_stats, c->d_ray_len,
+                                                          c->d_ray_tm, halt);
+    else
+        k_deposit<false><<<grid, FTHREADS, 0, c->stream>>>(rays, nray, c->H, c->d_ctx, c->d_stats, c->d_ray_len,
+                                                           c->d_ray_tm, halt);
     PSR_HIP(c, hipGetLastError());
     PSR_EV(c, 2, 1);
     return PSR_OK;

@@ -23,10 +23,21 @@ __device__ __forceinline__ double xs_get(const DevTables &T, int id, double eng)
     return tab_lin(T.xs, T.xs + (size_t)(id + 1) * T.xs_nrows, T.xs_nrows, T.xs_min, T.xs_step_inv, eng);
 }
 
-// src/sph-depth.c:224-231: nearest-lower entry, 0 outside the table
+// global-memory view of a pointer read from device memory (the compiler
+// cannot prove the address space and would emit flat loads, which count on
+// both the vector-memory and the LDS wait counters)
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gmem(const T *p) {
+    return (const __attribute__((address_space(1))) T *)p;
+}
+
+// src/sph-depth.c:224-231: nearest-lower entry, 0 outside the table; the
+// load is unconditional (clamped index) so no branch hides it
 __device__ __forceinline__ double sph_depth(const DevTables &T, double r_h) {
     const int n = (int)(T.kernel_n * r_h);
-    return (n < T.kernel_n && n >= 0) ? T.kernel[n] : 0.0;
+    const bool in = n < T.kernel_n && n >= 0;
+    const double v = gmem(T.kernel)[in ? n : 0];
+    return in ? v : 0.0;
 }
 
 // bilinear TabFun2 lookup of src/tabfun.c:173-223

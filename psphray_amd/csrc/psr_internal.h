@@ -93,18 +93,6 @@ struct DevHits {
     int32_t cap;
 };
 
-// per-hit deposit values in sorted order (same slot indexing as DevHits):
-// em = expm1(-tau_sum) (+0 marks a hit the reference skips, tau_sum = 0),
-// nhi = N_HI, tmb = transmitted photons before the hit; helium runs add the
-// species fractions tau_s / tau_sum and N_HeI (-1: no helium), N_HeII.
-// last[r] = index of the last hit the walk processed (-1: none).
-struct DevDep {
-    double *em, *nhi, *tmb;
-    double *fhi, *fhei, *fheii, *nhei, *nheii;
-    int32_t *last;
-    const int32_t *halt;  // the counters' flag word (PSR_DF_HALT)
-};
-
 // device counters: PSR_NSTAT copies, one 128-byte line each; a block adds
 // into copy blockIdx.x % PSR_NSTAT and the host folds the copies.  Device-
 // scope atomics on one address serialize beyond the XCD's L2, so a single
@@ -189,13 +177,6 @@ struct psr_ctx {
     int64_t hit_slots = 0;  // number of rays the slot arrays hold
     double *d_ray_len = nullptr, *d_ray_tm = nullptr;
     int64_t last_nray = 0;
-    // deposit buffers: one allocation of dep_words arrays of dep_slots doubles,
-    // and the per-ray walk end
-    DevDep D{};
-    double *d_dep = nullptr;
-    int64_t dep_slots = 0;
-    int32_t dep_words = 0;
-    int64_t dep_rays = 0;
     // stats and phase timers
     DevStats *d_stats = nullptr;
     bool timing = false;
