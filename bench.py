@@ -21,8 +21,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BYTES_TRACE = 16        # pos f32x3 + sml f32 per intersection
-BYTES_TRACE_DEPOSIT = 52  # + mass, x_HI, RMW yGdepHI/heat/hits (H-only)
+# algorithmic bytes per intersection: trace 16 (pos + sml); trace + deposit
+# 52 (H-only) / 84 (H + He).  The deposit kernel's share: mass f32 + x_HI f64
+# + read-modify-write of yGdepHI, heat, hits (3 x 8) = 36; H + He adds x_HeI,
+# x_HeII (16) and yGdepHeI / yGdepHeII (16)
+BYTES_TRACE = 16
+BYTES_DEPOSIT_H = 36
+BYTES_DEPOSIT_HE = 68
+# HBM bytes per k_deposit launch from rocprofv3 FETCH_SIZE + WRITE_SIZE
+# passes, scaled by the same counters on a copy of known size (FETCH_SIZE x 2,
+# WRITE_SIZE x 1): 79,002 KiB x 2 + 34,458 KiB for workload 2
+TRAFFIC_DEPOSIT = {2: (79002 * 2 + 34458) * 1024}
+WORKLOADS = {
+    1: "32^3 jittered lattice, H-only, isothermal 1e4 K, one source",
+    2: "128^3 lognormal, H-only, isothermal 1e4 K, 64 sources",
+    3: "256^3 Zel'dovich, H+He, temperature evolution, 16-bin blackbody, 1024 sources",
+    5: "96 batched lognormal fields n^3, n in {8..64}, H-only, isothermal",
+}
 
 
 def log(*a):
@@ -77,7 +92,20 @@ def main():
     ap.add_argument("--nray", type=int, default=None)
     ap.add_argument("--chem", choices=["exact", "surrogate"], default="exact")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run plain device copies of a known size (scale for the PMC "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        nbytes = 512 << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_read": nbytes, "bytes_written": nbytes, "dispatches": 5}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "design (ray sharding + reduce-scatter of deposits) is described, not built",
@@ -121,19 +149,25 @@ def main():
     wall = time.perf_counter() - t0
     log(f"{args.steps} ticks in {wall * 1e3:.2f} ms")
     st = ctx.stats()
+    # per-phase pass: HIP events on the context stream around each kernel
+    ctx.reset_stats()
     ctx.set_timing(True)
     for t in range(t_end + 1, nt + 1):
         ctx.tick(t, wl.tick_time, ray_first=int(firsts[t - 1]), nray=len(ray_ticks[t - 1]), chem=chem)
     ctx.sync()
     tm = ctx.timers()
+    st_t = ctx.stats()
     ctx.set_timing(False)
     K = args.steps
     inter = st["total_deposit_count"]
     value = inter / wall
     td_ms = tm["trace"] + tm["sort"] + tm["deposit"]
-    trace_launch_s = tm["trace"] / 1e3 / K
     hits_per_tick = inter / K
-    roof_achieved = hits_per_tick * BYTES_TRACE / trace_launch_s / 1e9
+    # dominant kernel: the deposit; its share of the per-intersection bytes
+    he = not wl.physics.flags.h_only
+    bytes_dep = BYTES_DEPOSIT_HE if he else BYTES_DEPOSIT_H
+    dep_launch_s = tm["deposit"] / 1e3 / K
+    roof_achieved = st_t["total_deposit_count"] / K * bytes_dep / dep_launch_s / 1e9
     out = {
         "metric": "ray-particle intersections/s (whole tick, exact FP64 chemistry)",
         "value": value,
@@ -149,7 +183,7 @@ def main():
         "data": "synthetic (seeded generator: jittered lattice, lognormal density, Poisson sources; "
                 "counter-based ray lists)",
         "config": {
-            "workload": f"{wl.name}: {wl.ps.npar} particles (128^3 lognormal, H-only, isothermal 1e4 K), "
+            "workload": f"{wl.name}: {wl.ps.npar} particles ({WORKLOADS.get(args.config, '')}), "
                         f"{wl.nray} rays/tick, {args.chem} chemistry",
             "particles": int(wl.ps.npar),
             "rays_per_tick": int(wl.nray),
@@ -163,14 +197,15 @@ def main():
         "deposit_walked_per_tick": st["deposit_processed"] / K,
         "solver_steps_per_update": st["solver_steps"] / max(st["active_count"], 1),
         "roofline": {
-            "kernel": "k_trace",
+            "kernel": "k_deposit",
             "bound": "hbm",
             "achieved": roof_achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": roof_achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "bytes_per_intersection": BYTES_TRACE,
+            "traffic": TRAFFIC_DEPOSIT.get(args.config),
+            "bytes_per_intersection": bytes_dep,
+            "launch_us": dep_launch_s * 1e6,
         },
     }
     ctx.close()
