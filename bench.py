@@ -86,11 +86,14 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=6,
+                    help="untimed ticks (the first four also time the two deposit accumulations)")
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--n", type=int, default=None, help="override lattice size")
     ap.add_argument("--nray", type=int, default=None)
     ap.add_argument("--chem", choices=["exact", "surrogate"], default="exact")
+    ap.add_argument("--accum", choices=["auto", "atomic", "binned"], default="auto",
+                    help="deposit accumulation (auto: timed on the first ticks)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run plain device copies of a known size (scale for the PMC "
@@ -128,6 +131,8 @@ def main():
     pool = np.concatenate(ray_ticks)
 
     ctx = psr.Context(0)
+    accum_ids = {"auto": psr.PSR_ACCUM_AUTO, "atomic": psr.PSR_ACCUM_ATOMIC, "binned": psr.PSR_ACCUM_BINNED}
+    ctx.set_tuning(psr.PSR_TUNE_DEPOSIT_ACCUM, accum_ids[args.accum])
     ctx.set_physics(wl.physics)
     t0 = time.perf_counter()
     ctx.load(wl.ps, wl.field_offsets, wl.boxsizes, boxsize=wl.boxsize)
@@ -141,6 +146,8 @@ def main():
     ctx.sync()
     ctx.reset_stats()
     ctx.sync()
+    accum = {v: k for k, v in accum_ids.items()}[ctx.tuning()]
+    log(f"deposit accumulation: {accum}")
     t_end = args.warmup + args.steps
     t0 = time.perf_counter()
     for t in range(args.warmup + 1, t_end + 1):
@@ -196,6 +203,7 @@ def main():
         "active_per_tick": st["active_count"] / K,
         "deposit_walked_per_tick": st["deposit_processed"] / K,
         "solver_steps_per_update": st["solver_steps"] / max(st["active_count"], 1),
+        "deposit_accumulation": accum,
         "roofline": {
             "kernel": "k_deposit",
             "bound": "hbm",

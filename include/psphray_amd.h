@@ -220,6 +220,22 @@ int psr_reset_stats(psr_ctx *ctx);
 int psr_set_timing(psr_ctx *ctx, int enable);
 int psr_get_timers(psr_ctx *ctx, double *ms);
 
+/* launch choices of the device path.  No counterpart in the reference (its
+ * deposit adds to the particles in place, src/loop.c:486-573); they change
+ * how the work is laid out, never the values beyond f32 summation order.
+ * PSR_TUNE_DEPOSIT_ACCUM: how a tick's per-particle deposits are summed.
+ *   PSR_ACCUM_AUTO   each choice timed (HIP events) on the first asynchronous
+ *                    ticks after the tree or physics change; the faster kept
+ *   PSR_ACCUM_ATOMIC f32 atomics into the particle arrays
+ *   PSR_ACCUM_BINNED records binned by particle range, summed in LDS
+ * psr_get_tuning returns the choice in use (PSR_ACCUM_AUTO while timing). */
+#define PSR_TUNE_DEPOSIT_ACCUM 1
+#define PSR_ACCUM_AUTO 0
+#define PSR_ACCUM_ATOMIC 1
+#define PSR_ACCUM_BINNED 2
+int psr_set_tuning(psr_ctx *ctx, int32_t param, int32_t value);
+int psr_get_tuning(psr_ctx *ctx, int32_t param, int32_t *value);
+
 /* step_evolve on the device for n independent steps (src/solve.c:317-323),
  * in place; rtol <= 0 selects the reference tolerances (src/solve.c:151) */
 int psr_solve_steps(psr_ctx *ctx, psr_step *steps, int64_t n, double rtol);

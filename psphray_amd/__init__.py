@@ -11,12 +11,14 @@ import ctypes as C
 import numpy as np
 
 from . import _abi
-from ._abi import (PSR_CHEM_EXACT, PSR_CHEM_SURROGATE, PSR_PHASE_ALL, PSR_PHASE_DEPOSIT,
-                   PSR_PHASE_TRACE, PSR_PHASE_UPDATE, RAY_DTYPE, STEP_DTYPE, ptr)
+from ._abi import (PSR_ACCUM_ATOMIC, PSR_ACCUM_AUTO, PSR_ACCUM_BINNED, PSR_CHEM_EXACT,
+                   PSR_CHEM_SURROGATE, PSR_PHASE_ALL, PSR_PHASE_DEPOSIT, PSR_PHASE_TRACE,
+                   PSR_PHASE_UPDATE, PSR_TUNE_DEPOSIT_ACCUM, RAY_DTYPE, STEP_DTYPE, ptr)
 
 __all__ = ["Physics", "Context", "ParticleSystem", "PsrError", "RAY_DTYPE", "STEP_DTYPE",
            "PSR_PHASE_ALL", "PSR_PHASE_TRACE", "PSR_PHASE_DEPOSIT", "PSR_PHASE_UPDATE",
-           "PSR_CHEM_EXACT", "PSR_CHEM_SURROGATE", "default_units", "make_flags"]
+           "PSR_CHEM_EXACT", "PSR_CHEM_SURROGATE", "PSR_TUNE_DEPOSIT_ACCUM", "PSR_ACCUM_AUTO",
+           "PSR_ACCUM_ATOMIC", "PSR_ACCUM_BINNED", "default_units", "make_flags"]
 
 # internal unit defaults, src/config.c:43-51
 C_1_CMH = 3.0835455558318480e+21
@@ -263,6 +265,15 @@ class Context:
 
     def set_timing(self, enable=True):
         self._check(self._lib.psr_set_timing(self.h, int(enable)), "psr_set_timing")
+
+    def set_tuning(self, param, value):
+        self._check(self._lib.psr_set_tuning(self.h, int(param), int(value)), "psr_set_tuning")
+
+    def tuning(self, param=PSR_TUNE_DEPOSIT_ACCUM):
+        """the choice in use (PSR_ACCUM_AUTO while the trial ticks run)"""
+        v = C.c_int32(0)
+        self._check(self._lib.psr_get_tuning(self.h, int(param), C.byref(v)), "psr_get_tuning")
+        return v.value
 
     def timers(self):
         """accumulated ms: trace, sort, deposit, update, whole tick"""

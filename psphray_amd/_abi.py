@@ -19,6 +19,10 @@ PSR_PHASE_UPDATE = 4
 PSR_PHASE_ALL = 7
 PSR_CHEM_EXACT = 0
 PSR_CHEM_SURROGATE = 1
+PSR_TUNE_DEPOSIT_ACCUM = 1
+PSR_ACCUM_AUTO = 0
+PSR_ACCUM_ATOMIC = 1
+PSR_ACCUM_BINNED = 2
 
 _D = C.c_double
 _I32 = C.c_int32
@@ -112,6 +116,8 @@ SIGNATURES = {
     "psr_reset_stats": (C.c_int, [_VP]),
     "psr_set_timing": (C.c_int, [_VP, C.c_int]),
     "psr_get_timers": (C.c_int, [_VP, _VP]),
+    "psr_set_tuning": (C.c_int, [_VP, C.c_int32, C.c_int32]),
+    "psr_get_tuning": (C.c_int, [_VP, C.c_int32, _VP]),
     "psr_solve_steps": (C.c_int, [_VP, _VP, _I64, _D]),
     "psr_mlp_features": (C.c_int, [_VP, C.POINTER(Units), _VP]),
     "psr_mlp_train": (C.c_int, [_VP, _VP, _VP, _I64, _I32, _I32, C.c_float, C.c_uint64, _VP]),

@@ -164,6 +164,8 @@ extern "C" void psr_destroy(psr_ctx *c) {
     dfree(c->H.sb);
     dfree(c->d_ray_len);
     dfree(c->d_ray_tm);
+    dfree(c->d_bin_rec);
+    dfree(c->d_bin_off);
     dfree(c->d_stats);
     dfree(c->d_qlist);
     dfree(c->d_q);
@@ -171,6 +173,8 @@ extern "C" void psr_destroy(psr_ctx *c) {
     dfree(c->d_ctx);
     for (int k = 0; k < 10; k++)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+    for (int k = 0; k < 2 * psr_ctx::TUNE_N; k++)
+        if (c->tune_ev[k]) (void)hipEventDestroy(c->tune_ev[k]);
     if (c->d_scratch) (void)hipFree(c->d_scratch);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -243,6 +247,7 @@ extern "C" int psr_set_physics(psr_ctx *c, const psr_units *u, const psr_flags *
     c->F = *f;
     c->have_physics = true;
     c->ctx_dirty = true;
+    psr_tune_reset(c);
     return PSR_OK;
 }
 
@@ -649,6 +654,7 @@ extern "C" int psr_tree_build(psr_ctx *c, const float *pos, const float *sml, co
 #undef TB_HIP
     c->have_tree = true;
     c->ctx_dirty = true;
+    psr_tune_reset(c);
     return PSR_OK;
 }
 

@@ -23,13 +23,16 @@
 //   wave 2 (apply)  for segment s - 1, each lane recomputes the TM its hit saw
 //                   from the checkpoint (at most 7 steps, the same operations,
 //                   so the same bits), resolves the exact cut hit in the
-//                   stopping group, and scatters the ionization / heating
-//                   deposits with f32 atomics (src/loop.c:486-573).
+//                   stopping group, and computes the ionization / heating
+//                   deposits (src/loop.c:486-573): either added to the
+//                   particles with f32 atomics, or written as records into
+//                   bins by particle range that k_bin_reduce sums in LDS.
 //
-// The walk is the critical path (about 35 cycles per hit at 2.4 GHz); prep
-// and apply run beside it.  The per-hit values never leave LDS, and the wave
-// that issues atomics never waits on global loads (stores and atomics share
-// the in-order vmcnt counter with loads on gfx9).
+// The walk is the critical path (about 35 cycles per hit at 2.4 GHz for
+// hydrogen); prep and apply run beside it.  The per-hit values never leave
+// LDS, and the apply wave never waits on global loads (stores and atomics
+// share the in-order vmcnt counter with loads on gfx9): the bin range and
+// room of its records are fetched one pipeline iteration ahead.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -218,11 +221,26 @@ struct ApplySums {
     double fiHI, fiHeI, fiHeII, siHI, siHeI;
 };
 
+// what one live hit adds to its particle: yGdepHI, heat (and yGdepHeI,
+// yGdepHeII), each summed over the reference's separate increments in double
+// and rounded once; the hit count is one per record
+template <bool HE>
+struct DepRec {
+    uint32_t slot;
+    float dHI, dheat;
+};
+template <>
+struct DepRec<true> {
+    uint32_t slot;
+    float dHI, dHeI, dHeII, dheat;
+};
+
 // src/loop.c:479-573 for one live hit that saw TM = tm
 template <bool HE>
-__device__ __forceinline__ void apply_hit(double tm, const WalkRec<HE> &v, const HitAux<HE> &au,
-                                          const DevState &S, const DevTables &T, const DevPhys &P,
-                                          const RayConst &k, ApplySums &acc) {
+__device__ __forceinline__ DepRec<HE> apply_values(double tm, const WalkRec<HE> &v, const HitAux<HE> &au,
+                                                   const DevTables &T, const DevPhys &P, const RayConst &k,
+                                                   ApplySums &acc) {
+    double yHI = 0.0, yHeI = 0.0, yHeII = 0.0, ht = 0.0;
     const uint32_t slot = au.slot;
     const double em = v.em;
     const double NHI = v.nhi;
@@ -279,44 +297,78 @@ __device__ __forceinline__ void apply_hit(double tm, const WalkRec<HE> &v, const
         if (x > 1.0) x = 1.0;
     }
     if (P.secondary_ionization && x > 0.0) {
-        atomicAdd(&S.yGdepHI[slot], (float)deltaHI);
+        yHI += deltaHI;
         acc.fiHI += deltaHI * NH;
         if (k.secion_terms) {
             const double log10x = log10(x);
             const double sec = deltaHI * k.dEHI / P.C_HI_ENERGY * tab2_get(T, T.si_phi_hi, k.dEHI, log10x) +
                                deltaHeI * k.dEHeI / P.C_HI_ENERGY * tab2_get(T, T.si_phi_hi, k.dEHeI, log10x);
-            atomicAdd(&S.yGdepHI[slot], (float)sec);
+            yHI += sec;
             acc.siHI += sec * NH;
-            atomicAdd(&S.heat[slot], (float)(k.hf_HI * deltaHI * tab2_get(T, T.si_eh, k.dEHI, log10x)));
+            ht += k.hf_HI * deltaHI * tab2_get(T, T.si_eh, k.dEHI, log10x);
             if (he) {
                 const double sec2 =
                     deltaHI * k.dEHI / P.C_HEI_ENERGY * tab2_get(T, T.si_phi_hei, k.dEHI, log10x) +
                     deltaHeI * k.dEHeI / P.C_HEI_ENERGY * tab2_get(T, T.si_phi_hei, k.dEHeI, log10x);
-                atomicAdd(&S.yGdepHeI[slot], (float)(deltaHeI + sec2));
+                yHeI += deltaHeI + sec2;
                 acc.siHeI += sec2 * NHe;
                 acc.fiHeI += deltaHeI * NHe;
-                atomicAdd(&S.heat[slot], (float)(k.hf_HeI * deltaHeI * tab2_get(T, T.si_eh, k.dEHeI, log10x)));
-                atomicAdd(&S.yGdepHeII[slot], (float)deltaHeII);
+                ht += k.hf_HeI * deltaHeI * tab2_get(T, T.si_eh, k.dEHeI, log10x);
+                yHeII += deltaHeII;
                 acc.fiHeII += deltaHeII * NHe;
             }
         } else if (he) {
-            atomicAdd(&S.yGdepHeI[slot], (float)deltaHeI);
+            yHeI += deltaHeI;
             acc.fiHeI += deltaHeI * NHe;
-            atomicAdd(&S.yGdepHeII[slot], (float)deltaHeII);
+            yHeII += deltaHeII;
             acc.fiHeII += deltaHeII * NHe;
         }
     } else {
-        atomicAdd(&S.yGdepHI[slot], (float)deltaHI);
-        if (k.hf_HI != 0.0) atomicAdd(&S.heat[slot], (float)(k.hf_HI * deltaHI));
+        yHI += deltaHI;
+        ht += k.hf_HI * deltaHI;
         if (he) {
-            atomicAdd(&S.yGdepHeI[slot], (float)deltaHeI);
-            atomicAdd(&S.yGdepHeII[slot], (float)deltaHeII);
-            if (k.hf_HeI != 0.0) atomicAdd(&S.heat[slot], (float)(deltaHeI * k.hf_HeI));
-            if (k.hf_HeII != 0.0) atomicAdd(&S.heat[slot], (float)(deltaHeII * k.hf_HeII));
+            yHeI += deltaHeI;
+            yHeII += deltaHeII;
+            ht += deltaHeI * k.hf_HeI;
+            ht += deltaHeII * k.hf_HeII;
         }
     }
-    atomicAdd(&S.hits[slot], 1);
+    DepRec<HE> out;
+    out.slot = slot;
+    out.dHI = (float)yHI;
+    out.dheat = (float)ht;
+    if constexpr (HE) {
+        out.dHeI = (float)yHeI;
+        out.dHeII = (float)yHeII;
+    }
+    return out;
 }
+
+// a record added straight to its particle (records beyond a bin's room)
+template <bool HE>
+__device__ __forceinline__ void rec_atomic(const DevState &S, const DepRec<HE> &r) {
+    atomicAdd(&S.yGdepHI[r.slot], r.dHI);
+    if (r.dheat != 0.0f) atomicAdd(&S.heat[r.slot], r.dheat);
+    if constexpr (HE) {
+        if (r.dHeI != 0.0f) atomicAdd(&S.yGdepHeI[r.slot], r.dHeI);
+        if (r.dHeII != 0.0f) atomicAdd(&S.yGdepHeII[r.slot], r.dHeII);
+    }
+    atomicAdd(&S.hits[r.slot], 1);
+}
+
+// Deposit records are binned by particle range (BIN_SHIFT: 2048 particles in
+// tree order per bin) instead of being added to the particles with scattered
+// atomics, which run at about 23 G/s on this part whatever their scope: the
+// apply wave reserves room with one returning atomic per (bin, instruction),
+// k_bin_reduce then sums each bin in LDS and adds it to the particles.
+constexpr int BIN_SHIFT = 11;
+constexpr int BIN_P = 1 << BIN_SHIFT;
+
+struct DevBins {
+    void *rec;           // records, bin b at [off[b], off[b + 1])
+    const uint32_t *off;  // nbins + 1
+    uint32_t *fill;       // records claimed this tick (may exceed the room)
+};
 
 template <bool HE>
 struct DepLds {
@@ -327,11 +379,74 @@ struct DepLds {
     double tm_end;
 };
 
+// records of one segment in the apply wave, with their room reserved; the
+// reservation's return values are used one pipeline iteration later
+template <bool HE>
+struct RecCarry {
+    static constexpr int HPL = FSEG / 64;
+    DepRec<HE> r[HPL];
+    bool has[HPL];
+    int leader[HPL], rank[HPL];
+    uint32_t old[HPL], o0[HPL], o1[HPL];
+};
+
+// group this instruction's records by bin; one returning atomic per bin
+// (lane `leader`) claims the room, `rank` is the record's place in the group;
+// the bin's range is loaded alongside
+template <bool HE>
+__device__ __forceinline__ void rec_reserve(RecCarry<HE> &c, int h, bool has, const DepRec<HE> &r,
+                                            const DevBins &B, int lane) {
+    const unsigned long long lt = (1ull << lane) - 1;
+    const int b = (int)(r.slot >> BIN_SHIFT);
+    unsigned long long rem = __ballot(has);
+    unsigned long long gm = 0;
+    int leader = lane;
+    while (rem) {
+        const int l = __ffsll((long long)rem) - 1;
+        const int b0 = __builtin_amdgcn_readlane(b, l);
+        const unsigned long long m = __ballot(has && b == b0);
+        if (has && b == b0) {
+            gm = m;
+            leader = l;
+        }
+        rem &= ~m;
+    }
+    uint32_t old = 0, o0 = 0, o1 = 0;
+    if (has && lane == leader) old = atomicAdd(&B.fill[b], (uint32_t)__popcll(gm));
+    if (has) {
+        o0 = B.off[b];
+        o1 = B.off[b + 1];
+    }
+    c.r[h] = r;
+    c.has[h] = has;
+    c.leader[h] = leader;
+    c.rank[h] = __popcll(gm & lt);
+    c.old[h] = old;
+    c.o0[h] = o0;
+    c.o1[h] = o1;
+}
+
+// write the carried records into their bins (or straight to the particles
+// when the bin is full)
+template <bool HE>
+__device__ __forceinline__ void rec_commit(RecCarry<HE> &c, const DevBins &B, const DevState &S) {
+#pragma unroll
+    for (int h = 0; h < RecCarry<HE>::HPL; h++) {
+        const uint32_t base = (uint32_t)__shfl((int)c.old[h], c.leader[h], 64);
+        if (c.has[h]) {
+            const uint32_t pos = base + (uint32_t)c.rank[h];
+            if (pos < c.o1[h] - c.o0[h]) ((DepRec<HE> *)B.rec)[c.o0[h] + pos] = c.r[h];
+            else rec_atomic<HE>(S, c.r[h]);
+        }
+        c.has[h] = false;
+    }
+}
+
 template <bool HE>
 __global__ __launch_bounds__(FTHREADS) void k_deposit(const psr_ray *__restrict__ rays, int64_t nray, DevHits H,
                                                       const DevCtx *__restrict__ C, DevStats *__restrict__ st0,
                                                       double *__restrict__ ray_len, double *__restrict__ ray_tm,
-                                                      const int32_t *halt) {
+                                                      DevBins Bn, const int32_t *halt) {
     __shared__ DepLds<HE> L;
     if (halted(halt)) return;
     const DevState &S = C->S;
@@ -345,6 +460,13 @@ __global__ __launch_bounds__(FTHREADS) void k_deposit(const psr_ray *__restrict_
     unsigned long long cnt = 0, walked = 0;
     double lost = 0.0;
     constexpr int HPL = FSEG / 64;  // hits per lane and segment
+    RecCarry<HE> carry;             // apply wave: records awaiting their room
+#pragma unroll
+    for (int h = 0; h < HPL; h++) {
+        carry.has[h] = false;
+        carry.leader[h] = lane;
+        carry.old[h] = 0;
+    }
     for (int64_t r = blockIdx.x; r < nray; r += gridDim.x) {
         const int n = H.count[r];
         const double Nph = rays[r].Nph;
@@ -432,42 +554,67 @@ __global__ __launch_bounds__(FTHREADS) void k_deposit(const psr_ray *__restrict_
                     // group's steps run, so LDS latency stays off the chain.
                     // TM changes only at live hits, so below T0 after a group
                     // with a live hit means the cut is in that group.
-                    WalkRec<HE> A[FG], B[FG];
+                    if constexpr (HE) {
+                        // 64-byte records: one group in registers, each
+                        // record reloaded with the next group's as soon as
+                        // its step has read it
+                        WalkRec<HE> A[FG];
 #pragma unroll
-                    for (int q = 0; q < FG; q++) A[q] = L.rec[ring][q];
-                    for (int g = 0; g < ngrp; g += 2) {
-                        const int g1 = min(g + 1, FNCK - 1);
+                        for (int q = 0; q < FG; q++) A[q] = L.rec[ring][q];
+                        for (int g = 0; g < ngrp; g++) {
+                            const int g1 = min(g + 1, FNCK - 1);
+                            L.ck[ring][g] = t;
+                            bool seen = false;
 #pragma unroll
-                        for (int q = 0; q < FG; q++) B[q] = L.rec[ring][g1 * FG + q];
-                        __builtin_amdgcn_sched_barrier(0);
-                        L.ck[ring][g] = t;
-                        bool seen = false;
-#pragma unroll
-                        for (int q = 0; q < FG; q++) {
-                            seen = seen | is_live(A[q].em);
-                            t = walk_step<HE>(t, A[q]);
+                            for (int q = 0; q < FG; q++) {
+                                seen = seen | is_live(A[q].em);
+                                t = walk_step<HE>(t, A[q]);
+                                A[q] = L.rec[ring][g1 * FG + q];
+                            }
+                            if (t < T0 && seen) {
+                                L.stop_seg = s;
+                                L.stop_grp = g;
+                                break;
+                            }
                         }
-                        if (t < T0 && seen) {
-                            L.stop_seg = s;
-                            L.stop_grp = g;
-                            break;
-                        }
-                        if (g + 1 >= ngrp) break;
-                        const int g2 = min(g + 2, FNCK - 1);
+                    } else {
+                        WalkRec<HE> A[FG], B[FG];
 #pragma unroll
-                        for (int q = 0; q < FG; q++) A[q] = L.rec[ring][g2 * FG + q];
-                        __builtin_amdgcn_sched_barrier(0);
-                        L.ck[ring][g + 1] = t;
-                        seen = false;
+                        for (int q = 0; q < FG; q++) A[q] = L.rec[ring][q];
+                        for (int g = 0; g < ngrp; g += 2) {
+                            const int g1 = min(g + 1, FNCK - 1);
 #pragma unroll
-                        for (int q = 0; q < FG; q++) {
-                            seen = seen | is_live(B[q].em);
-                            t = walk_step<HE>(t, B[q]);
-                        }
-                        if (t < T0 && seen) {
-                            L.stop_seg = s;
-                            L.stop_grp = g + 1;
-                            break;
+                            for (int q = 0; q < FG; q++) B[q] = L.rec[ring][g1 * FG + q];
+                            __builtin_amdgcn_sched_barrier(0);
+                            L.ck[ring][g] = t;
+                            bool seen = false;
+#pragma unroll
+                            for (int q = 0; q < FG; q++) {
+                                seen = seen | is_live(A[q].em);
+                                t = walk_step<HE>(t, A[q]);
+                            }
+                            if (t < T0 && seen) {
+                                L.stop_seg = s;
+                                L.stop_grp = g;
+                                break;
+                            }
+                            if (g + 1 >= ngrp) break;
+                            const int g2 = min(g + 2, FNCK - 1);
+#pragma unroll
+                            for (int q = 0; q < FG; q++) A[q] = L.rec[ring][g2 * FG + q];
+                            __builtin_amdgcn_sched_barrier(0);
+                            L.ck[ring][g + 1] = t;
+                            seen = false;
+#pragma unroll
+                            for (int q = 0; q < FG; q++) {
+                                seen = seen | is_live(B[q].em);
+                                t = walk_step<HE>(t, B[q]);
+                            }
+                            if (t < T0 && seen) {
+                                L.stop_seg = s;
+                                L.stop_grp = g + 1;
+                                break;
+                            }
                         }
                     }
                     TM = t;
@@ -480,6 +627,8 @@ __global__ __launch_bounds__(FTHREADS) void k_deposit(const psr_ray *__restrict_
             for (int i = 0;; i++) {
                 const int stop = L.stop_seg;
                 if (stop >= 0 ? i > stop + 2 : i > nseg + 1) break;
+                // last iteration's records: their room has arrived by now
+                if (Bn.rec) rec_commit<HE>(carry, Bn, S);
                 const int s = i - 2;
                 if (s >= 0 && s < nseg && (stop < 0 || s <= stop)) {
                     const int ring = s % 3;
@@ -503,17 +652,19 @@ __global__ __launch_bounds__(FTHREADS) void k_deposit(const psr_ray *__restrict_
                         }
                     }
 #pragma unroll
-                    for (int h = 0; h < FSEG / 64; h++) {
+                    for (int h = 0; h < HPL; h++) {
                         const int jj = h * 64 + lane;
-                        if (jj <= lim) {
-                            const WalkRec<HE> v = L.rec[ring][jj];
-                            if (is_live(v.em)) {
-                                const int g0 = jj & ~(FG - 1);
-                                double pre = L.ck[ring][g0 / FG];
-                                for (int q = g0; q < jj; q++) pre = walk_step<HE>(pre, L.rec[ring][q]);
-                                apply_hit<HE>(pre, v, L.aux[ring][jj], S, T, P, kc, acc);
-                            }
+                        const WalkRec<HE> v = L.rec[ring][jj];
+                        const bool has = jj <= lim && is_live(v.em);
+                        DepRec<HE> rr{};
+                        if (has) {
+                            const int g0 = jj & ~(FG - 1);
+                            double pre = L.ck[ring][g0 / FG];
+                            for (int q = g0; q < jj; q++) pre = walk_step<HE>(pre, L.rec[ring][q]);
+                            rr = apply_values<HE>(pre, v, L.aux[ring][jj], T, P, kc, acc);
                         }
+                        if (Bn.rec) rec_reserve<HE>(carry, h, has, rr, Bn, lane);
+                        else if (has) rec_atomic<HE>(S, rr);
                     }
                 }
                 __syncthreads();
@@ -530,6 +681,7 @@ __global__ __launch_bounds__(FTHREADS) void k_deposit(const psr_ray *__restrict_
         }
         __syncthreads();
     }
+    if (wv == 2 && Bn.rec) rec_commit<HE>(carry, Bn, S);  // the last segment's records
     // counters: thread 0 holds the ray totals, wave 2 the deposit sums
     DevStats *st = stat_slot(st0);
     if (threadIdx.x == 0 && cnt) {
@@ -559,22 +711,181 @@ __global__ __launch_bounds__(FTHREADS) void k_deposit(const psr_ray *__restrict_
     }
 }
 
+// one workgroup per bin: the bin's records summed in LDS, then added to its
+// particles (no other kernel touches them meanwhile).  Records are loaded
+// RB at a time per thread so that their latencies overlap.
+constexpr int RED_THREADS = 1024;
+constexpr int RB = 4;
+
+template <bool HE>
+__global__ __launch_bounds__(RED_THREADS) void k_bin_reduce(DevBins B, int64_t nbins, int64_t npar,
+                                                            const DevCtx *__restrict__ C, const int32_t *halt) {
+    __shared__ float s_hi[BIN_P], s_heat[BIN_P];
+    __shared__ float s_he1[HE ? BIN_P : 1], s_he2[HE ? BIN_P : 1];
+    __shared__ uint32_t s_hits[BIN_P];
+    if (halted(halt)) return;
+    const DevState &S = C->S;
+    const DepRec<HE> *rec = (const DepRec<HE> *)B.rec;
+    for (int64_t b = blockIdx.x; b < nbins; b += gridDim.x) {
+        for (int p = threadIdx.x; p < BIN_P; p += RED_THREADS) {
+            s_hi[p] = 0.0f;
+            s_heat[p] = 0.0f;
+            s_hits[p] = 0;
+            if (HE) {
+                s_he1[p] = 0.0f;
+                s_he2[p] = 0.0f;
+            }
+        }
+        __syncthreads();
+        const uint32_t o0 = B.off[b], room = B.off[b + 1] - o0;
+        const uint32_t n = min(B.fill[b], room);
+        const uint32_t p0 = (uint32_t)(b << BIN_SHIFT);
+        for (uint32_t k0 = threadIdx.x; k0 < n; k0 += RB * RED_THREADS) {
+            DepRec<HE> r[RB];
+#pragma unroll
+            for (int u = 0; u < RB; u++) {
+                const uint32_t k = min(k0 + u * RED_THREADS, n - 1);
+                r[u] = rec[o0 + k];
+            }
+#pragma unroll
+            for (int u = 0; u < RB; u++) {
+                if (k0 + u * RED_THREADS >= n) break;
+                const uint32_t p = r[u].slot - p0;
+                atomicAdd(&s_hi[p], r[u].dHI);
+                if (r[u].dheat != 0.0f) atomicAdd(&s_heat[p], r[u].dheat);
+                if constexpr (HE) {
+                    if (r[u].dHeI != 0.0f) atomicAdd(&s_he1[p], r[u].dHeI);
+                    if (r[u].dHeII != 0.0f) atomicAdd(&s_he2[p], r[u].dHeII);
+                }
+                atomicAdd(&s_hits[p], 1u);
+            }
+        }
+        __syncthreads();
+        for (int p = threadIdx.x; p < BIN_P; p += RED_THREADS) {
+            const int64_t g = (int64_t)p0 + p;
+            const uint32_t nh = s_hits[p];
+            if (g < npar && nh) {
+                S.yGdepHI[g] += s_hi[p];
+                if (s_heat[p] != 0.0f) S.heat[g] += s_heat[p];
+                if (HE) {
+                    S.yGdepHeI[g] += s_he1[p];
+                    S.yGdepHeII[g] += s_he2[p];
+                }
+                S.hits[g] += (int32_t)nh;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// one workgroup: the room of every bin for the next launch, from this
+// launch's claims (claims + 25% + 32, scaled down to the buffer when the sum
+// does not fit; rec_cap / nbins each when `uniform`), then the claim counters
+// cleared
+constexpr int PLAN_THREADS = 1024;
+
+__global__ __launch_bounds__(PLAN_THREADS) void k_bin_plan(uint32_t *__restrict__ fill, int64_t nbins,
+                                                           uint64_t rec_cap, int uniform,
+                                                           uint32_t *__restrict__ off, const int32_t *halt) {
+    __shared__ unsigned long long wsum[PLAN_THREADS / 64];
+    if (!uniform && halted(halt)) return;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int64_t per = (nbins + PLAN_THREADS - 1) / PLAN_THREADS;
+    const int64_t b0 = min(nbins, (int64_t)t * per), b1 = min(nbins, b0 + per);
+    auto want = [&](int64_t b) -> unsigned long long {
+        if (uniform) return rec_cap / (unsigned long long)nbins;
+        const unsigned long long c = fill[b];
+        return c + (c >> 2) + 32;
+    };
+    unsigned long long sum = 0;
+    for (int64_t b = b0; b < b1; b++) sum += want(b);
+    // inclusive scan over the wave, then over the waves
+    unsigned long long x = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned long long y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    unsigned long long before = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < PLAN_THREADS / 64; k++) {
+        const unsigned long long v = wsum[k];
+        if (k < w) before += v;
+        total += v;
+    }
+    const double scale = total > rec_cap ? (double)rec_cap / (double)total : 1.0;
+    // floor(prefix * scale) is monotone and ends inside the buffer
+    unsigned long long run = before + x - sum;
+    for (int64_t b = b0; b < b1; b++) {
+        off[b] = (uint32_t)((double)run * scale);
+        run += want(b);
+    }
+    if (t == PLAN_THREADS - 1) off[nbins] = (uint32_t)fmin((double)rec_cap, (double)total * scale);
+    for (int64_t b = b0; b < b1; b++) fill[b] = 0;
+}
+
 }  // namespace
 
-int psr_launch_deposit(psr_ctx *c, int64_t ray_first, int64_t nray) {
+// bins sized for the context's particles and hit slots; reallocated (after
+// the queued ticks finished) when either grows
+static int ensure_bins(psr_ctx *c, bool he) {
+    const int64_t nbins = (c->npar + BIN_P - 1) / BIN_P;
+    const size_t rsz = he ? sizeof(DepRec<true>) : sizeof(DepRec<false>);
+    // room for every hit of a tick at the slot capacity, or 16 per particle
+    uint64_t rec_cap = std::min<uint64_t>((uint64_t)c->hit_slots * (uint64_t)c->H.cap,
+                                          16 * (uint64_t)c->npar + (1u << 20));
+    rec_cap = std::max<uint64_t>(rec_cap, 1u << 20);
+    rec_cap = std::min<uint64_t>(rec_cap, 0xF0000000u);
+    if (nbins <= c->bin_n && rec_cap * rsz <= c->bin_rec_bytes && (int)he <= c->bin_he) return PSR_OK;
+    PSR_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->d_bin_rec) PSR_HIP(c, hipFree(c->d_bin_rec));
+    if (c->d_bin_off) PSR_HIP(c, hipFree(c->d_bin_off));
+    c->d_bin_rec = nullptr;
+    c->d_bin_off = nullptr;
+    c->bin_n = 0;
+    PSR_HIP(c, hipMalloc(&c->d_bin_rec, rec_cap * rsz));
+    // off[nbins + 1], fill[nbins]
+    PSR_HIP(c, hipMalloc(&c->d_bin_off, sizeof(uint32_t) * (2 * (size_t)nbins + 1)));
+    c->bin_n = nbins;
+    c->bin_rec_bytes = rec_cap * rsz;
+    c->bin_rec_cap = rec_cap;
+    c->bin_he = he;
+    k_bin_plan<<<1, PLAN_THREADS, 0, c->stream>>>(c->d_bin_off + nbins + 1, nbins, rec_cap, 1, c->d_bin_off,
+                                                  nullptr);
+    PSR_HIP(c, hipGetLastError());
+    return PSR_OK;
+}
+
+int psr_launch_deposit(psr_ctx *c, int64_t ray_first, int64_t nray, bool binned) {
     int rc = psr_sync_dev_ctx(c);
     if (rc) return rc;
+    const bool he = !c->F.h_only;
+    if (binned && (rc = ensure_bins(c, he))) return rc;
     const psr_ray *rays = c->d_rays + ray_first;
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nray, 65535 * 4));
     const int32_t *halt = &c->d_stats->flags;
+    const int64_t nbins = binned ? c->bin_n : 0;
+    uint32_t *off = binned ? c->d_bin_off : nullptr, *fill = binned ? off + nbins + 1 : nullptr;
+    // a null record buffer makes the deposit add straight to the particles
+    const DevBins Bn{binned ? c->d_bin_rec : nullptr, off, fill};
+    const unsigned gbin = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nbins, 65535 * 4));
     PSR_EV(c, 2, 0);
-    if (!c->F.h_only)
+    if (he)
         k_deposit<true><<<grid, FTHREADS, 0, c->stream>>>(rays, nray, c->H, c->d_ctx, c->d_stats, c->d_ray_len,
-                                                          c->d_ray_tm, halt);
+                                                          c->d_ray_tm, Bn, halt);
     else
         k_deposit<false><<<grid, FTHREADS, 0, c->stream>>>(rays, nray, c->H, c->d_ctx, c->d_stats, c->d_ray_len,
-                                                           c->d_ray_tm, halt);
+                                                           c->d_ray_tm, Bn, halt);
     PSR_HIP(c, hipGetLastError());
+    if (binned) {
+        if (he) k_bin_reduce<true><<<gbin, RED_THREADS, 0, c->stream>>>(Bn, nbins, c->npar, c->d_ctx, halt);
+        else k_bin_reduce<false><<<gbin, RED_THREADS, 0, c->stream>>>(Bn, nbins, c->npar, c->d_ctx, halt);
+        PSR_HIP(c, hipGetLastError());
+        k_bin_plan<<<1, PLAN_THREADS, 0, c->stream>>>(fill, nbins, c->bin_rec_cap, 0, off, halt);
+        PSR_HIP(c, hipGetLastError());
+    }
     PSR_EV(c, 2, 1);
     return PSR_OK;
 }

@@ -177,6 +177,14 @@ struct psr_ctx {
     int64_t hit_slots = 0;  // number of rays the slot arrays hold
     double *d_ray_len = nullptr, *d_ray_tm = nullptr;
     int64_t last_nray = 0;
+    // deposit records binned by particle range; d_bin_off holds off[nbins + 1],
+    // fill[nbins] and claimed[nbins]
+    void *d_bin_rec = nullptr;
+    uint32_t *d_bin_off = nullptr;
+    int64_t bin_n = 0;
+    size_t bin_rec_bytes = 0;
+    uint64_t bin_rec_cap = 0;
+    int bin_he = 0;
     // stats and phase timers
     DevStats *d_stats = nullptr;
     bool timing = false;
@@ -206,6 +214,15 @@ struct psr_ctx {
     std::deque<PendingTick> pending;
     int32_t *h_ring = nullptr;
     uint32_t seq = 0;
+    // deposit accumulation: requested (PSR_ACCUM_*), chosen (ATOMIC or BINNED
+    // once decided, AUTO while the trial ticks run), and the trials: ticks
+    // alternating the two choices, each with an event pair around its deposit
+    int32_t accum_req = PSR_ACCUM_AUTO;
+    int32_t accum_use = PSR_ACCUM_AUTO;
+    static constexpr int TUNE_N = 4;
+    hipEvent_t tune_ev[2 * TUNE_N] = {};
+    uint32_t tune_slot[TUNE_N] = {};
+    int tune_n = 0;
 
     DevState state() const;
 };
@@ -215,7 +232,9 @@ struct psr_ctx {
 // kernels read the halt flag and the tick is validated later
 int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray, bool sync);
 int psr_launch_sort(psr_ctx *c, int64_t nray);
-int psr_launch_deposit(psr_ctx *c, int64_t ray_first, int64_t nray);
+int psr_launch_deposit(psr_ctx *c, int64_t ray_first, int64_t nray, bool binned);
+// restart the accumulation trials (tree or physics changed)
+void psr_tune_reset(psr_ctx *c);
 int psr_launch_update(psr_ctx *c, int64_t tick, double tick_time, int32_t chem_mode);
 // validate (and if needed replay) every pending tick; blocks
 int psr_resolve(psr_ctx *c);

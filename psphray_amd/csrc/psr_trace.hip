@@ -603,6 +603,35 @@ extern "C" int psr_trace_lists(psr_ctx *c, int64_t ray_first, int64_t nray, int6
     return PSR_OK;
 }
 
+void psr_tune_reset(psr_ctx *c) {
+    c->tune_n = 0;
+    c->accum_use = PSR_ACCUM_AUTO;
+}
+
+// once every trial tick has been published: keep the accumulation whose
+// fastest trial deposit was faster; trials of a tick that overflowed (its
+// kernels stopped early) start the trials again.  Blocks only when `block`.
+static int tune_decide(psr_ctx *c, bool block) {
+    if (c->accum_use != PSR_ACCUM_AUTO || c->tune_n < psr_ctx::TUNE_N) return PSR_OK;
+    const hipEvent_t last = c->tune_ev[2 * psr_ctx::TUNE_N - 1];
+    if (block) PSR_HIP(c, hipEventSynchronize(last));
+    else if (hipEventQuery(last) != hipSuccess) return PSR_OK;
+    float best[2] = {1e30f, 1e30f};
+    for (int k = 0; k < psr_ctx::TUNE_N; k++) {
+        const int32_t v = ((volatile int32_t *)c->h_ring)[c->tune_slot[k]];
+        if (v == 0 && !block) return PSR_OK;
+        if (v == 0 || (v & PSR_DF_HALT)) {
+            psr_tune_reset(c);
+            return PSR_OK;
+        }
+        float ms = 0;
+        PSR_HIP(c, hipEventElapsedTime(&ms, c->tune_ev[2 * k], c->tune_ev[2 * k + 1]));
+        best[k & 1] = std::min(best[k & 1], ms);
+    }
+    c->accum_use = best[1] < best[0] ? PSR_ACCUM_BINNED : PSR_ACCUM_ATOMIC;
+    return PSR_OK;
+}
+
 // the kernels of one tick; sync = wait for the trace's capacity check
 static int tick_body(psr_ctx *c, const PendingTick &t, bool sync) {
     int rc;
@@ -611,7 +640,25 @@ static int tick_body(psr_ctx *c, const PendingTick &t, bool sync) {
         if ((rc = psr_launch_trace(c, t.ray_first, t.nray, sync))) return rc;
         if (t.phases & PSR_PHASE_DEPOSIT) {
             if ((rc = psr_launch_sort(c, t.nray))) return rc;
-            if ((rc = psr_launch_deposit(c, t.ray_first, t.nray))) return rc;
+            // accumulation: the requested or decided one, else the next
+            // trial (odd trials binned); replays use binning meanwhile
+            int trial = -1;
+            bool binned = true;
+            if (c->accum_req != PSR_ACCUM_AUTO) binned = c->accum_req == PSR_ACCUM_BINNED;
+            else if (c->accum_use != PSR_ACCUM_AUTO) binned = c->accum_use == PSR_ACCUM_BINNED;
+            else if (!sync && c->tune_n < psr_ctx::TUNE_N) {
+                trial = c->tune_n;
+                binned = trial & 1;
+                if (!c->tune_ev[0])
+                    for (int k = 0; k < 2 * psr_ctx::TUNE_N; k++) PSR_HIP(c, hipEventCreate(&c->tune_ev[k]));
+                PSR_HIP(c, hipEventRecord(c->tune_ev[2 * trial], c->stream));
+            }
+            if ((rc = psr_launch_deposit(c, t.ray_first, t.nray, binned))) return rc;
+            if (trial >= 0) {
+                PSR_HIP(c, hipEventRecord(c->tune_ev[2 * trial + 1], c->stream));
+                c->tune_slot[trial] = t.slot;
+                c->tune_n++;
+            }
         } else {
             k_count_hits<<<(unsigned)((t.nray + 255) / 256), 256, 0, c->stream>>>(c->H.count, t.nray,
                                                                                   c->d_stats, halt);
@@ -637,8 +684,11 @@ __global__ void k_publish(const int32_t *__restrict__ flags, int32_t *slot) {
 }
 
 int psr_resolve(psr_ctx *c) {
-    if (c->pending.empty()) return PSR_OK;
+    // no tick pending: every trial tick was published, its events are done
+    if (c->pending.empty()) return tune_decide(c, false);
     PSR_HIP(c, hipStreamSynchronize(c->stream));
+    int rc0 = tune_decide(c, true);
+    if (rc0) return rc0;
     std::deque<PendingTick> todo;
     todo.swap(c->pending);
     // ticks before the first overflow were applied; from it on, every kernel
@@ -685,6 +735,7 @@ extern "C" int psr_tick(psr_ctx *c, int64_t ray_first, int64_t nray, int64_t tic
         memset(c->h_ring, 0, sizeof(int32_t) * PSR_RING);
     }
     if ((rc = poll_pending(c))) return rc;
+    if ((rc = tune_decide(c, false))) return rc;
     if (c->pending.size() >= PSR_RING - 1 && (rc = psr_resolve(c))) return rc;
     PendingTick t{ray_first, nray, tick, tick_time, phases, chem_mode, c->seq++ % PSR_RING};
     ((volatile int32_t *)c->h_ring)[t.slot] = 0;
@@ -715,6 +766,23 @@ extern "C" int psr_set_timing(psr_ctx *c, int enable) {
         for (int k = 0; k < 10; k++) PSR_HIP(c, hipEventCreate(&c->ev[k]));
     c->timing = enable != 0;
     for (int k = 0; k < 5; k++) c->t_ms[k] = 0;
+    return PSR_OK;
+}
+
+extern "C" int psr_set_tuning(psr_ctx *c, int32_t param, int32_t value) {
+    if (!c) return PSR_ERR_ARG;
+    if (param != PSR_TUNE_DEPOSIT_ACCUM) PSR_FAIL(c, PSR_ERR_ARG, "unknown tuning parameter");
+    if (value != PSR_ACCUM_AUTO && value != PSR_ACCUM_ATOMIC && value != PSR_ACCUM_BINNED)
+        PSR_FAIL(c, PSR_ERR_ARG, "unknown deposit accumulation");
+    c->accum_req = value;
+    psr_tune_reset(c);
+    return PSR_OK;
+}
+
+extern "C" int psr_get_tuning(psr_ctx *c, int32_t param, int32_t *value) {
+    if (!c || !value) return PSR_ERR_ARG;
+    if (param != PSR_TUNE_DEPOSIT_ACCUM) PSR_FAIL(c, PSR_ERR_ARG, "unknown tuning parameter");
+    *value = c->accum_req != PSR_ACCUM_AUTO ? c->accum_req : c->accum_use;
     return PSR_OK;
 }
 

@@ -59,7 +59,27 @@ def oracle_lists(wl, rays):
     return np.array(offs, np.int64), (np.concatenate(out) if out else np.zeros(0, po.HIT_DTYPE))
 
 
+def assert_f32_sums(a, r, hits, name):
+    """f32 accumulators summed in another order (atomics, or LDS sums of
+    per-hit records).  Both sides add positive terms (at most 4 per hit:
+    the reference's separate increments), so each is within 4 * hits * 2^-24
+    of the exact sum (recursive summation bound); 1e-5 rel covers the rest"""
+    a, r = a.astype(np.float64), r.astype(np.float64)
+    tol = (1e-5 + 8.0 * hits * 2.0 ** -24) * np.abs(r) + 1e-6 * max(np.abs(r).max(), 1e-300)
+    bad = np.abs(a - r) > tol
+    assert not bad.any(), (name, int(bad.sum()), a[bad][:4], r[bad][:4], hits[bad][:4])
+
+
 CASES = [(1, 16, 64), (1, 32, 32), (2, 24, 256), (3, 20, 256)]
+ACCUM = [psr.PSR_ACCUM_ATOMIC, psr.PSR_ACCUM_BINNED]
+
+
+@pytest.fixture
+def accum(ctx, request):
+    """the deposit's accumulation, fixed for the test, automatic afterwards"""
+    ctx.set_tuning(psr.PSR_TUNE_DEPOSIT_ACCUM, request.param)
+    yield request.param
+    ctx.set_tuning(psr.PSR_TUNE_DEPOSIT_ACCUM, psr.PSR_ACCUM_AUTO)
 
 
 @pytest.mark.parametrize("cfg,n,nray", CASES)
@@ -163,9 +183,10 @@ def run_one_tick(ctx, wl, orc, phases, tick=1):
     return out, ref, rays, (ln, tm, st), off
 
 
+@pytest.mark.parametrize("accum", ACCUM, indirect=True)
 @pytest.mark.parametrize("cfg,n,nray", [(1, 32, 32), (2, 24, 256), (3, 20, 128)])
-def test_deposit_parity(ctx, cfg, n, nray):
-    print_case("deposit", cfg, n, nray)
+def test_deposit_parity(ctx, cfg, n, nray, accum):
+    print_case("deposit", cfg, n, nray, accum)
     wl = make_workload(cfg, RATES, n=n, nray=nray)
     orc = oracle_for(wl)
     load(ctx, wl)
@@ -174,9 +195,7 @@ def test_deposit_parity(ctx, cfg, n, nray):
                                                      psr.PSR_PHASE_TRACE | psr.PSR_PHASE_DEPOSIT)
     assert np.array_equal(out.hits, ref.hits)
     for f in ("yGdepHI", "yGdepHeI", "yGdepHeII", "heat"):
-        a, r = getattr(out, f).astype(np.float64), getattr(ref, f).astype(np.float64)
-        # f32 accumulators, summed in another order (float atomics)
-        assert np.allclose(a, r, rtol=1e-5, atol=1e-6 * max(np.abs(r).max(), 1e-300)), f
+        assert_f32_sums(getattr(out, f), getattr(ref, f), ref.hits, f)
     dln, dtm = ctx.ray_results(len(rays))
     assert np.array_equal(dln, ln)
     assert np.allclose(dtm, tm, rtol=1e-9, atol=1e-12 * rays["Nph"].max())
@@ -185,11 +204,12 @@ def test_deposit_parity(ctx, cfg, n, nray):
     assert s["saturated_deposit_count"] == st.saturated_deposit_count
 
 
+@pytest.mark.parametrize("accum", ACCUM, indirect=True)
 @pytest.mark.parametrize("he", [False, True])
-def test_deposit_no_photon_and_cutoff(ctx, he):
+def test_deposit_no_photon_and_cutoff(ctx, he, accum):
     """rays of negative type start with TM = 0 under no_photon (src/loop.c:411);
     the first live hit then ends the ray; other rays run into the cut-off"""
-    print_case("deposit-no-photon", he)
+    print_case("deposit-no-photon", he, accum)
     wl = make_workload(3 if he else 1, RATES, n=16 if he else 24, nray=64)
     f = wl.physics.flags
     flags = psr.make_flags(isothermal=bool(f.isothermal), on_the_spot=bool(f.on_the_spot),
@@ -212,14 +232,78 @@ def test_deposit_no_photon_and_cutoff(ctx, he):
     ln, tm, st = po.deposit(orc, ref, rays, off, hits)
     assert np.array_equal(out.hits, ref.hits)
     for fld in ("yGdepHI", "yGdepHeI", "yGdepHeII", "heat"):
-        a, r = getattr(out, fld).astype(np.float64), getattr(ref, fld).astype(np.float64)
-        assert np.allclose(a, r, rtol=1e-5, atol=1e-6 * max(np.abs(r).max(), 1e-300)), fld
+        assert_f32_sums(getattr(out, fld), getattr(ref, fld), ref.hits, fld)
     dln, dtm = ctx.ray_results(len(rays))
     assert np.array_equal(dln, ln)
     assert np.allclose(dtm, tm, rtol=1e-9, atol=1e-12 * rays["Nph"].max())
     s = ctx.stats()
     assert s["deposit_processed"] < s["total_deposit_count"]   # some rays were cut
     assert s["saturated_deposit_count"] == st.saturated_deposit_count
+
+
+@pytest.mark.parametrize("he", [False, True])
+def test_binned_deposit_full_bins(he):
+    """rays from the corner all cross the lower octant, which is the first two
+    particle bins (2048 particles each, tree order); their even first share of
+    the record buffer (16 per particle + 2^20 records over the 16 bins)
+    overflows, and those records go straight to the particles.  The second
+    tick runs with rooms planned from the first tick's claims.  Both ticks'
+    sums equal the oracle's."""
+    print_case("binned-full", he)
+    wl = make_workload(3 if he else 1, RATES, n=32, nray=8192)
+    orc = oracle_for(wl)
+    rays = wl.rays(1)
+    rays["s"] = np.float32(0.0)
+    # a context of its own: the bins are sized at its first binned deposit
+    with psr.Context(0) as c2:
+        load(c2, wl)
+        c2.set_tuning(psr.PSR_TUNE_DEPOSIT_ACCUM, psr.PSR_ACCUM_BINNED)
+        c2.rays_upload(rays)
+        for t in (1, 2):
+            c2.tick(t, wl.tick_time, phases=psr.PSR_PHASE_TRACE | psr.PSR_PHASE_DEPOSIT)
+        out = wl.ps.copy()
+        c2.state_download(out)
+    ref = wl.ps.copy()
+    t = po.Tree(ref.pos, ref.sml, wl.boxsize)
+    off, hits = po.trace_rays(t, ref, rays)
+    nb = (wl.ps.npar + 2047) // 2048
+    room = (16 * wl.ps.npar + 2 ** 20) // nb
+    octant = (wl.ps.pos < wl.boxsize / 2).all(1)
+    assert octant.sum() == 2 * 2048
+    slot_hits = np.bincount(hits["ipar"], minlength=wl.ps.npar)
+    assert slot_hits[octant].sum() > 2 * room      # the case this test is for
+    po.deposit(orc, ref, rays, off, hits)
+    assert np.array_equal(out.hits, 2 * ref.hits)
+    for fld in ("yGdepHI", "yGdepHeI", "yGdepHeII", "heat"):
+        assert_f32_sums(getattr(out, fld), 2 * getattr(ref, fld).astype(np.float64), 2 * ref.hits, fld)
+
+
+def test_deposit_accumulation_is_tuned(ctx):
+    """automatic accumulation: the first four deposit ticks alternate atomic
+    and binned sums (each timed); a choice is made once they are done, and
+    every tick's sums are right whichever ran"""
+    print_case("autotune")
+    wl = make_workload(2, RATES, n=24, nray=256)
+    orc = oracle_for(wl)
+    load(ctx, wl)
+    assert ctx.tuning() == psr.PSR_ACCUM_AUTO
+    ctx.reset_stats()
+    rays = wl.rays(1)
+    ctx.rays_upload(rays)
+    nt = 6
+    for t in range(1, nt + 1):
+        ctx.tick(t, wl.tick_time, phases=psr.PSR_PHASE_TRACE | psr.PSR_PHASE_DEPOSIT)
+    ctx.sync()
+    assert ctx.tuning() in (psr.PSR_ACCUM_ATOMIC, psr.PSR_ACCUM_BINNED)
+    out = wl.ps.copy()
+    ctx.state_download(out)
+    ref = wl.ps.copy()
+    t = po.Tree(ref.pos, ref.sml, wl.boxsize)
+    off, hits = po.trace_rays(t, ref, rays)
+    po.deposit(orc, ref, rays, off, hits)
+    assert np.array_equal(out.hits, nt * ref.hits)
+    for fld in ("yGdepHI", "heat"):
+        assert_f32_sums(getattr(out, fld), nt * getattr(ref, fld).astype(np.float64), nt * ref.hits, fld)
 
 
 def test_overflowed_tick_is_replayed():
