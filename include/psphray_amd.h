@@ -240,25 +240,53 @@ int psr_get_tuning(psr_ctx *ctx, int32_t param, int32_t *value);
  * in place; rtol <= 0 selects the reference tolerances (src/solve.c:151) */
 int psr_solve_steps(psr_ctx *ctx, psr_step *steps, int64_t n, double rtol);
 
-/* ---- learned chemistry (MLP surrogate of step_evolve) ---- */
-#define PSR_MLP_IN 16
+/* ---- learned chemistry (MLP surrogate of step_evolve) ----
+ * No counterpart in the reference: a network 22 -> 64 -> 64 -> 64 -> 4
+ * (SiLU) replaces step_evolve (src/solve.c:170-224) for one particle-step
+ * in PSR_CHEM_SURROGATE ticks, trained on the device (BF16 MFMA forward and
+ * backward, FP32 accumulation and Adam) on tuples the exact solver writes.
+ * Features (psr_mlp_features): log10 nH, x_HI, x_HeI, x_HeII, log10 T; the
+ * absorbed photons: log10(yGdepHI/HeI/HeII + 1e-12), log10(heat + 1e-30),
+ * yeMET, eleven zero slots; log10 time.  Targets: x_HI, x_HeI, x_HeII and
+ * log10 T after the step; the network predicts their change from features
+ * 1..4.  A prediction is used when the features lie in the training domain
+ * and the charge / particle-conservation residual (how far the fractions
+ * leave [0, 1] with x_HeI + x_HeII <= 1) is at most the residual threshold;
+ * otherwise the exact solver runs for that particle. */
+#define PSR_MLP_IN 22
 #define PSR_MLP_HID 64
 #define PSR_MLP_OUT 4
-/* feature vector of a step (host-side definition shared with the device) */
-int psr_mlp_features(const psr_step *s, const psr_units *u, float *x /*16*/);
-/* train on n (features, targets) pairs on the device with the project's own
- * BF16-MFMA forward/backward kernels; returns the final loss */
+int psr_mlp_features(const psr_step *s, const psr_units *u, float *x /*22*/);
+/* exact chemistry of the following ticks writes one (features, targets) row
+ * per solved particle into a device buffer of cap rows (0: off, and frees) */
+int psr_mlp_capture(psr_ctx *ctx, int64_t cap);
+/* rows captured so far (*n), and the first max_rows of them when x / y */
+int psr_mlp_captured(psr_ctx *ctx, int64_t *n, float *x, float *y,
+                     int64_t max_rows);
+/* train on n (features, targets) rows, or with x = y = NULL on the captured
+ * rows (n <= 0: all).  The first call fits the input normalisation and the
+ * domain to the data and initialises the parameters from seed; later calls
+ * continue from the current parameters.  Adam, cosine-decayed lr; *loss_out
+ * = mean summed squared error of the last epoch. */
 int psr_mlp_train(psr_ctx *ctx, const float *x, const float *y, int64_t n,
                   int32_t epochs, int32_t batch, float lr, uint64_t seed,
                   float *loss_out);
+/* predictions (targets) for n feature rows, before the residual check */
 int psr_mlp_forward(psr_ctx *ctx, const float *x, int64_t n, float *y);
-/* parameter access: W1[16][64] b1[64] W2[64][64] b2 W3[64][64] b3
- * W4[64][4] b4, fp32 master copy; sizes in floats */
+/* parameter access: W1[22][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64]
+ * W4[64][4] b4[4] ([in][out] row-major), fp32 master copy */
 int64_t psr_mlp_param_count(void);
 int psr_mlp_get_params(psr_ctx *ctx, float *p);
 int psr_mlp_set_params(psr_ctx *ctx, const float *p);
-/* gradient of the mean-squared loss at the current parameters on one batch
- * (for the gradient check against the FP64 oracle) */
+/* input normalisation (x - mean) / sigma and the domain [lo, hi] per
+ * feature (lo / hi may be NULL: unbounded) */
+int psr_mlp_set_norm(psr_ctx *ctx, const float *mean, const float *sigma,
+                     const float *lo, const float *hi);
+int psr_mlp_get_norm(psr_ctx *ctx, float *mean, float *sigma, float *lo,
+                     float *hi);
+/* gradient of the loss (mean over the n rows of the summed squared error of
+ * the predicted changes) at the current parameters, for the gradient check
+ * against the FP64 oracle */
 int psr_mlp_grad(psr_ctx *ctx, const float *x, const float *y, int64_t n,
                  float *grad, float *loss);
 /* residual threshold for the fallback to the exact solver (default 1e-4) */

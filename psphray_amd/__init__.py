@@ -284,6 +284,77 @@ class Context:
     def sync(self):
         self._check(self._lib.psr_sync(self.h), "psr_sync")
 
+    # ---- learned chemistry (surrogate of step_evolve) ----
+    def mlp_capture(self, cap):
+        """exact chemistry of later ticks writes (features, targets) rows,
+        up to cap of them, into a device buffer (0: off)"""
+        self._check(self._lib.psr_mlp_capture(self.h, int(cap)), "psr_mlp_capture")
+
+    def mlp_captured(self, max_rows=0):
+        """(n, x[:max_rows], y[:max_rows]) of the captured rows"""
+        n = C.c_int64(0)
+        x = np.zeros((max(max_rows, 0), _abi.PSR_MLP_IN), np.float32)
+        y = np.zeros((max(max_rows, 0), _abi.PSR_MLP_OUT), np.float32)
+        self._check(self._lib.psr_mlp_captured(self.h, C.byref(n), ptr(x) if max_rows else None,
+                                               ptr(y) if max_rows else None, int(max_rows)),
+                    "psr_mlp_captured")
+        k = min(n.value, max(max_rows, 0))
+        return n.value, x[:k], y[:k]
+
+    def mlp_train(self, x=None, y=None, n=0, epochs=1, batch=4096, lr=1e-3, seed=1):
+        """train on (x, y) or, with x = y = None, on the captured rows;
+        returns the last epoch's mean loss"""
+        loss = C.c_float(0)
+        if x is not None:
+            x = np.ascontiguousarray(x, np.float32)
+            y = np.ascontiguousarray(y, np.float32)
+            n = len(x)
+        self._check(self._lib.psr_mlp_train(self.h, ptr(x) if x is not None else None,
+                                            ptr(y) if y is not None else None, int(n), int(epochs),
+                                            int(batch), float(lr), int(seed), C.byref(loss)),
+                    "psr_mlp_train")
+        return loss.value
+
+    def mlp_forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((len(x), _abi.PSR_MLP_OUT), np.float32)
+        self._check(self._lib.psr_mlp_forward(self.h, ptr(x), len(x), ptr(y)), "psr_mlp_forward")
+        return y
+
+    def mlp_params(self):
+        p = np.zeros(self._lib.psr_mlp_param_count(), np.float32)
+        self._check(self._lib.psr_mlp_get_params(self.h, ptr(p)), "psr_mlp_get_params")
+        return p
+
+    def mlp_set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert len(p) == self._lib.psr_mlp_param_count()
+        self._check(self._lib.psr_mlp_set_params(self.h, ptr(p)), "psr_mlp_set_params")
+
+    def mlp_set_norm(self, mean, sigma, lo=None, hi=None):
+        a = [np.ascontiguousarray(v, np.float32) if v is not None else None for v in (mean, sigma, lo, hi)]
+        self._check(self._lib.psr_mlp_set_norm(self.h, *[ptr(v) if v is not None else None for v in a]),
+                    "psr_mlp_set_norm")
+
+    def mlp_norm(self):
+        """(mean, sigma, lo, hi) of the inputs"""
+        v = [np.zeros(_abi.PSR_MLP_IN, np.float32) for _ in range(4)]
+        self._check(self._lib.psr_mlp_get_norm(self.h, *[ptr(a) for a in v]), "psr_mlp_get_norm")
+        return tuple(v)
+
+    def mlp_grad(self, x, y):
+        """(gradient, loss) at the current parameters over all rows"""
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        g = np.zeros(self._lib.psr_mlp_param_count(), np.float32)
+        loss = C.c_float(0)
+        self._check(self._lib.psr_mlp_grad(self.h, ptr(x), ptr(y), len(x), ptr(g), C.byref(loss)),
+                    "psr_mlp_grad")
+        return g, loss.value
+
+    def mlp_set_residual(self, tol):
+        self._check(self._lib.psr_mlp_set_residual(self.h, float(tol)), "psr_mlp_set_residual")
+
     def solve_steps(self, steps, rtol=-1.0):
         steps = np.ascontiguousarray(steps, STEP_DTYPE).copy()
         self._check(self._lib.psr_solve_steps(self.h, ptr(steps), len(steps), rtol),

@@ -19,6 +19,8 @@ PSR_PHASE_UPDATE = 4
 PSR_PHASE_ALL = 7
 PSR_CHEM_EXACT = 0
 PSR_CHEM_SURROGATE = 1
+PSR_MLP_IN = 22
+PSR_MLP_OUT = 4
 PSR_TUNE_DEPOSIT_ACCUM = 1
 PSR_ACCUM_AUTO = 0
 PSR_ACCUM_ATOMIC = 1
@@ -127,6 +129,10 @@ SIGNATURES = {
     "psr_mlp_set_params": (C.c_int, [_VP, _VP]),
     "psr_mlp_grad": (C.c_int, [_VP, _VP, _VP, _I64, _VP, _VP]),
     "psr_mlp_set_residual": (C.c_int, [_VP, _D]),
+    "psr_mlp_capture": (C.c_int, [_VP, _I64]),
+    "psr_mlp_captured": (C.c_int, [_VP, _VP, _VP, _VP, _I64]),
+    "psr_mlp_set_norm": (C.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "psr_mlp_get_norm": (C.c_int, [_VP, _VP, _VP, _VP, _VP]),
 }
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpsphray_amd.so")

@@ -16,6 +16,7 @@
 #include <algorithm>
 
 #include "psr_internal.h"
+#include "psr_mlp.h"
 
 namespace {
 
@@ -490,6 +491,56 @@ __device__ __forceinline__ void count_steps(DevStats *st, int steps, int fail, i
     }
 }
 
+// Capture of exact steps for training the surrogate.  Before the update,
+// k_cap_pre writes the features of every active particle into a claimed row
+// (one slot claim per wavefront) and remembers the row; after it,
+// k_cap_post adds the targets from what store_step left: the fractions and
+// the temperature after the step.
+__global__ __launch_bounds__(256) void k_cap_pre(DevState S, int64_t npar, int64_t tick, double tick_time,
+                                                 DevPhys P, MlpDev M, uint32_t *__restrict__ row_of,
+                                                 const int32_t *halt) {
+    if (halted(halt)) return;
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const bool on = i < npar && S.active[i];
+    if (i < npar) row_of[i] = 0xFFFFFFFFu;
+    const unsigned long long m = __ballot(on);
+    if (!m) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(M.cap_n, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (!on) return;
+    const unsigned long long slot = base + __popcll(m & ((1ull << lane) - 1));
+    if (slot >= M.cap_max) return;
+    double x0[7];
+    StepIn s;
+    load_step(S, (uint32_t)i, tick, tick_time, P, x0, s);
+    float f[MLP_IN];
+    mlp_features(x0[0], x0[1], x0[2], s.T, s.nH, s.yGdepHI, s.yGdepHeI, s.yGdepHeII, s.heat, s.yeMET, s.time, f);
+    float *r = M.cap + slot * MLP_ROW;
+#pragma unroll
+    for (int k = 0; k < MLP_IN; k++) r[k] = f[k];
+    row_of[i] = (uint32_t)slot;
+}
+
+__global__ __launch_bounds__(256) void k_cap_post(DevState S, int64_t npar, DevPhys P, MlpDev M,
+                                                  const uint32_t *__restrict__ row_of, int iso, const int32_t *halt) {
+    if (halted(halt)) return;
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= npar) return;
+    const uint32_t slot = row_of[i];
+    if (slot == 0xFFFFFFFFu) return;
+    float *r = M.cap + (size_t)slot * MLP_ROW;
+    const double lH = S.lH[i], lHeI = S.lHeI[i], lHeII = S.lHeII[i];
+    // isothermal: T is the step's T (feature 4); otherwise from ie and ye
+    const double T1 = iso ? exp10((double)r[4]) : ieye2T(P, S.ie[i], lambda_ye(P, lH, lHeI, lHeII));
+    r[MLP_IN + 0] = (float)lH;
+    r[MLP_IN + 1] = (float)lHeI;
+    r[MLP_IN + 2] = (float)lHeII;
+    r[MLP_IN + 3] = iso ? r[4] : mlp_lg(T1, 1e-30);
+}
+
 #define PHASE1_ATTEMPTS 4
 
 // Chemistry of the active set, phase 1: one thread per particle in tree
@@ -624,6 +675,118 @@ __global__ __launch_bounds__(256) void k_solve(psr_step *__restrict__ steps, int
     steps[i] = st;
 }
 
+// Surrogate chemistry (on-the-spot runs): 64 active particles per
+// wavefront per round; each lane builds its particle's step and features,
+// the wavefront runs the network over the 64 rows on MFMA (weights held in
+// registers), and each lane then checks its prediction x1 against the rate
+// equations: the features inside the training domain, and the particle-
+// conservation (ionisation balance) residual of every evolved variable,
+//     d = x1 - x0 - (f(x0) + f(x1)) / 2,
+// the trapezoidal defect of dx/dtau = f (src/solve.c:76-141, photo-
+// ionisations as the constant deposit rates), at most tol for the
+// fractions (one-sided where x1 sits at a bound the exact step clamps to,
+// src/solve.c:208-213) and 10 tol relative for ie.  Accepted steps are
+// stored as step_evolve's would be (src/loop.c:745-766); the others keep
+// active = 1 for the exact solver that runs next.
+template <bool ISO, bool ADI, bool HONLY>
+__global__ __launch_bounds__(256) void k_mlp_update(DevState S, int64_t npar, int64_t tick, double tick_time,
+                                                    DevTables T, DevPhys P, DevStats *__restrict__ st,
+                                                    const float *__restrict__ Pm, const float *__restrict__ norm,
+                                                    double tol) {
+    __shared__ float s_bias[3 * MLP_HID + MLP_OUT];
+    __shared__ float s_feat[4][64][MLP_IN + 1];
+    __shared__ float s_out[4][64][MLP_OUT];
+    __shared__ int part[2][4];
+    if (halted(&st->flags)) return;
+    for (int k = threadIdx.x; k < 3 * MLP_HID + MLP_OUT; k += blockDim.x) {
+        const int l = k / MLP_HID, o = k % MLP_HID;
+        s_bias[k] = Pm[(l == 0 ? MLP_B1 : l == 1 ? MLP_B2 : l == 2 ? MLP_B3 : MLP_B4) + o];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    MlpW w;
+    mlp_load_w(Pm, w, lane);
+    int nacc = 0, nfb = 0;
+    for (int64_t base = ((int64_t)blockIdx.x * 4 + wv) * 64; base < npar; base += (int64_t)gridDim.x * 256) {
+        const int64_t i = base + lane;
+        const bool act = i < npar && S.active[i] == 1;
+        if (!__ballot(act)) continue;
+        double x0[7];
+        StepIn s;
+        float f[MLP_IN];
+        bool inb = true;
+        if (act) {
+            load_step(S, (uint32_t)i, tick, tick_time, P, x0, s);
+            mlp_features(x0[0], x0[1], x0[2], s.T, s.nH, s.yGdepHI, s.yGdepHeI, s.yGdepHeII, s.heat, s.yeMET,
+                         s.time, f);
+        }
+#pragma unroll
+        for (int k = 0; k < MLP_IN; k++) {
+            const float v = act ? f[k] : 0.0f;
+            inb = inb && v >= norm[2 * MLP_IN + k] && v <= norm[3 * MLP_IN + k];
+            s_feat[wv][lane][k] = act ? (v - norm[k]) * norm[MLP_IN + k] : 0.0f;
+        }
+#pragma unroll
+        for (int sub = 0; sub < 4; sub++) {
+            mlp_bf16x4 xin[2];
+            mlp_inputs(&s_feat[wv][16 * sub][0], MLP_IN + 1, lane, xin);
+            const mlp_f32x4 o = mlp_forward16(w, s_bias, xin, lane);
+            if (lane < 16)
+#pragma unroll
+                for (int j = 0; j < MLP_OUT; j++) s_out[wv][16 * sub + lane][j] = o[j];
+        }
+        if (act) {
+            double pr[MLP_OUT];
+            mlp_predict(f, s_out[wv][lane], pr);
+            double x1[7] = {pr[0], HONLY ? x0[1] : pr[1], HONLY ? x0[2] : pr[2], x0[3], x0[4], x0[5], x0[6]};
+            if (!ISO && !ADI) x1[6] = Tye2ie(P, exp10(pr[3]), lambda_ye(P, x1[0], x1[1], x1[2]));
+            Rates Riso;
+            if (ISO) rates_at<true>(T, log10(s.T), s.nH, Riso);
+            double f0[7], f1[7];
+            rhs<ISO, ADI, true, HONLY>(x0, f0, s, T, P, Riso);
+            rhs<ISO, ADI, true, HONLY>(x1, f1, s, T, P, Riso);
+            bool ok = inb;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                if (HONLY && k > 0) break;
+                const double d = x1[k] - x0[k] - 0.5 * (f0[k] + f1[k]);
+                const bool lo = x1[k] <= 1e-6, hi = x1[k] >= 1.0 - 1e-6;
+                ok = ok && isfinite(d) && (lo ? d >= -tol : hi ? d <= tol : fabs(d) <= tol);
+            }
+            if (!ISO && !ADI) {
+                const double d = x1[6] - x0[6] - 0.5 * (f0[6] + f1[6]);
+                ok = ok && isfinite(d) && fabs(d) <= 10.0 * tol * fmax(x0[6], x1[6]);
+            }
+            if (ok) {
+                store_step<ISO, ADI>(S, (uint32_t)i, tick, P, x1, s.T);
+                nacc++;
+            } else {
+                nfb++;
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        nacc += __shfl_xor(nacc, o, 64);
+        nfb += __shfl_xor(nfb, o, 64);
+    }
+    if (lane == 0) {
+        part[0][wv] = nacc;
+        part[1][wv] = nfb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        nacc = part[0][0] + part[0][1] + part[0][2] + part[0][3];
+        nfb = part[1][0] + part[1][1] + part[1][2] + part[1][3];
+        DevStats *sp = stat_slot(st);
+        if (nacc) {
+            atomicAdd(&sp->surrogate_count, (unsigned long long)nacc);
+            atomicAdd(&sp->active_count, (unsigned long long)nacc);
+        }
+        if (nfb) atomicAdd(&sp->surrogate_fallbacks, (unsigned long long)nfb);
+    }
+}
+
 }  // namespace
 
 #define CHEM_DISPATCH(KERNEL, ...)                                                                       \
@@ -646,10 +809,11 @@ __global__ __launch_bounds__(256) void k_solve(psr_step *__restrict__ steps, int
         }                                                                                                \
     } while (0)
 
-int psr_launch_update(psr_ctx *c, int64_t tick, double tick_time, int32_t) {
+int psr_launch_update(psr_ctx *c, int64_t tick, double tick_time, int32_t, bool timed) {
     if (c->npar == 0) return PSR_OK;
     const DevPhys P = psr_dev_phys(c);
     const DevState S = c->state();
+    const MlpDev M = psr_mlp_dev(c);
     const int64_t n = c->npar;
     if (c->qlist_cap < n) {
         PSR_HIP(c, hipStreamSynchronize(c->stream));  // earlier ticks may still use them
@@ -663,13 +827,24 @@ int psr_launch_update(psr_ctx *c, int64_t tick, double tick_time, int32_t) {
         c->qlist_cap = n;
     }
     if (!c->d_q) PSR_HIP(c, hipMalloc(&c->d_q, sizeof(uint32_t) * 2));
-    PSR_EV(c, 3, 0);
+    if (timed) PSR_EV(c, 3, 0);
     unsigned grid = (unsigned)((n + 255) / 256);
+    const int32_t *halt = &c->d_stats->flags;
+    uint32_t *row_of = nullptr;
+    if (M.cap) {
+        if (!(row_of = psr_mlp_rowof(c, n))) return PSR_ERR_HIP;
+        k_cap_pre<<<grid, 256, 0, c->stream>>>(S, n, tick, tick_time, P, M, row_of, halt);
+        PSR_HIP(c, hipGetLastError());
+    }
     CHEM_DISPATCH(k_update, S, n, tick, tick_time, c->T, P, c->d_stats, c->d_saved);
     PSR_HIP(c, hipGetLastError());
     if (c->F.isothermal && c->F.on_the_spot && c->F.h_only) {
         // closed form: every particle finished in phase 1
-        PSR_EV(c, 3, 1);
+        if (M.cap) {
+            k_cap_post<<<grid, 256, 0, c->stream>>>(S, n, P, M, row_of, 1, halt);
+            PSR_HIP(c, hipGetLastError());
+        }
+        if (timed) PSR_EV(c, 3, 1);
         return PSR_OK;
     }
     // leftovers: compact with a device-wide selection (scan based)
@@ -682,8 +857,46 @@ int psr_launch_update(psr_ctx *c, int64_t tick, double tick_time, int32_t) {
     grid = (unsigned)std::max(1, c->cu_count * 8);
     CHEM_DISPATCH(k_update_rest, S, c->d_qlist, c->d_q, tick, tick_time, c->T, P, c->d_stats, c->d_saved);
     PSR_HIP(c, hipGetLastError());
-    PSR_EV(c, 3, 1);
+    if (M.cap) {
+        grid = (unsigned)((n + 255) / 256);
+        k_cap_post<<<grid, 256, 0, c->stream>>>(S, n, P, M, row_of, c->F.isothermal ? 1 : 0, halt);
+        PSR_HIP(c, hipGetLastError());
+    }
+    if (timed) PSR_EV(c, 3, 1);
     return PSR_OK;
+}
+
+// surrogate chemistry for the active set, then the exact solver for the
+// particles whose prediction was rejected (and for every particle when the
+// run is not on-the-spot, whose recombination counters the network does not
+// predict)
+int psr_mlp_update_dev(psr_ctx *c, int64_t tick, double tick_time) {
+    if (!psr_mlp_ready(c)) PSR_FAIL(c, PSR_ERR_STATE, "surrogate chemistry: no trained network (psr_mlp_train)");
+    if (c->npar == 0) return PSR_OK;
+    if (!c->F.on_the_spot) return psr_launch_update(c, tick, tick_time, PSR_CHEM_EXACT);
+    const DevPhys P = psr_dev_phys(c);
+    const DevState S = c->state();
+    const MlpDev M = psr_mlp_dev(c);
+    const int64_t n = c->npar;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, c->cu_count * 4));
+    PSR_EV(c, 3, 0);
+    const int key = (c->F.isothermal ? 4 : 0) | (c->F.adiabatic ? 2 : 0) | (c->F.h_only ? 1 : 0);
+#define MLP_K(I, A, H) \
+    k_mlp_update<I, A, H><<<grid, 256, 0, c->stream>>>(S, n, tick, tick_time, c->T, P, c->d_stats, M.params, \
+                                                       M.norm, c->mlp_residual)
+    switch (key) {
+    case 0: MLP_K(false, false, false); break;
+    case 1: MLP_K(false, false, true); break;
+    case 2: MLP_K(false, true, false); break;
+    case 3: MLP_K(false, true, true); break;
+    case 4: case 6: MLP_K(true, false, false); break;
+    default: MLP_K(true, false, true); break;
+    }
+#undef MLP_K
+    PSR_HIP(c, hipGetLastError());
+    int rc = psr_launch_update(c, tick, tick_time, PSR_CHEM_EXACT, false);
+    PSR_EV(c, 3, 1);
+    return rc;
 }
 
 int psr_solve_steps_dev(psr_ctx *c, psr_step *d_steps, int64_t n, double rtol) {

@@ -235,7 +235,8 @@ int psr_launch_sort(psr_ctx *c, int64_t nray);
 int psr_launch_deposit(psr_ctx *c, int64_t ray_first, int64_t nray, bool binned);
 // restart the accumulation trials (tree or physics changed)
 void psr_tune_reset(psr_ctx *c);
-int psr_launch_update(psr_ctx *c, int64_t tick, double tick_time, int32_t chem_mode);
+// timed: record the update phase's event pair (the surrogate records its own)
+int psr_launch_update(psr_ctx *c, int64_t tick, double tick_time, int32_t chem_mode, bool timed = true);
 // validate (and if needed replay) every pending tick; blocks
 int psr_resolve(psr_ctx *c);
 int psr_solve_steps_dev(psr_ctx *c, psr_step *d_steps, int64_t n, double rtol);
@@ -243,6 +244,19 @@ DevPhys psr_dev_phys(const psr_ctx *c);
 int psr_sync_dev_ctx(psr_ctx *c);
 int psr_ensure_scratch(psr_ctx *c, size_t bytes);
 int psr_mlp_update_dev(psr_ctx *c, int64_t tick, double tick_time);
+// the surrogate's device pointers: parameters, normalisation, and the
+// capture buffer of exact steps (cap null when capture is off)
+struct MlpDev {
+    const float *params;
+    const float *norm;
+    float *cap;
+    unsigned long long *cap_n;
+    unsigned long long cap_max;
+};
+MlpDev psr_mlp_dev(const psr_ctx *c);
+// per-particle capture row of the current update (n entries); null on error
+uint32_t *psr_mlp_rowof(psr_ctx *c, int64_t n);
+bool psr_mlp_ready(const psr_ctx *c);
 void psr_mlp_free(psr_ctx *c);
 
 #define PSR_HIP(c, call)                                                           \
