@@ -91,7 +91,12 @@ def main():
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--n", type=int, default=None, help="override lattice size")
     ap.add_argument("--nray", type=int, default=None)
-    ap.add_argument("--chem", choices=["exact", "surrogate"], default="exact")
+    ap.add_argument("--chem", choices=["exact", "surrogate"], default="exact",
+                    help="surrogate: exact ticks write training tuples, the network is trained, then "
+                         "the same ticks are timed with the exact solver and with the surrogate")
+    ap.add_argument("--train-ticks", type=int, default=8,
+                    help="exact ticks whose steps train the surrogate (--chem surrogate)")
+    ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--accum", choices=["auto", "atomic", "binned"], default="auto",
                     help="deposit accumulation (auto: timed on the first ticks)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -124,8 +129,11 @@ def main():
     log(f"workload {wl.name}: {wl.ps.npar} particles, {wl.nray} rays/tick "
         f"({time.perf_counter() - t0:.1f}s)")
     # warmup, the timed ticks, then as many again with per-phase event timers
-    # (those synchronise every tick, so they are kept out of the timed run)
-    nt = args.warmup + 2 * args.steps
+    # (those synchronise every tick, so they are kept out of the timed run);
+    # the surrogate run adds the training ticks after the warmup
+    surrogate = args.chem == "surrogate"
+    ntrain = args.train_ticks if surrogate else 0
+    nt = args.warmup + ntrain + 2 * args.steps
     ray_ticks = [wl.rays(t) for t in range(1, nt + 1)]
     firsts = np.cumsum([0] + [len(r) for r in ray_ticks])
     pool = np.concatenate(ray_ticks)
@@ -141,26 +149,67 @@ def main():
     log(f"device tree: {nnode} nodes, depth {depth} ({time.perf_counter() - t0:.2f}s incl. upload)")
     ctx.rays_upload(pool)
     chem = psr.PSR_CHEM_EXACT if args.chem == "exact" else psr.PSR_CHEM_SURROGATE
-    for t in range(1, args.warmup + 1):
-        ctx.tick(t, wl.tick_time, ray_first=int(firsts[t - 1]), nray=len(ray_ticks[t - 1]), chem=chem)
+
+    def run(t0_, t1_, mode):
+        for t in range(t0_, t1_ + 1):
+            ctx.tick(t, wl.tick_time, ray_first=int(firsts[t - 1]), nray=len(ray_ticks[t - 1]), chem=mode)
+
+    run(1, args.warmup, psr.PSR_CHEM_EXACT)
+    ctx.sync()
+    sur = None
+    t_first = args.warmup + ntrain + 1
+    t_end = args.warmup + ntrain + args.steps
+    if surrogate:
+        # training tuples from the exact solver, then the network trained on them
+        ctx.mlp_capture(min(ntrain * int(wl.ps.npar), 40_000_000))
+        run(args.warmup + 1, args.warmup + ntrain, psr.PSR_CHEM_EXACT)
+        ctx.sync()
+        ntup, _, _ = ctx.mlp_captured()
+        t0 = time.perf_counter()
+        loss = ctx.mlp_train(n=0, epochs=args.epochs, batch=8192, lr=2e-3, seed=1)
+        t_train = time.perf_counter() - t0
+        ctx.mlp_capture(0)
+        log(f"surrogate: {ntup} tuples, {args.epochs} epochs in {t_train:.2f}s, loss {loss:.3e}")
+        # the plain method over the timed ticks, from the same state
+        snap = wl.ps.copy()
+        ctx.state_download(snap)
+        ctx.sync()
+        t0 = time.perf_counter()
+        run(t_first, t_end, psr.PSR_CHEM_EXACT)
+        ctx.sync()
+        wall_exact = time.perf_counter() - t0
+        ex = wl.ps.copy()
+        ctx.state_download(ex)
+        ctx.state_upload(snap)
+        sur = {"tuples": int(ntup), "epochs": args.epochs, "train_s": t_train, "train_loss": loss,
+               "exact_ms_per_step": wall_exact / args.steps * 1e3}
     ctx.sync()
     ctx.reset_stats()
     ctx.sync()
     accum = {v: k for k, v in accum_ids.items()}[ctx.tuning()]
     log(f"deposit accumulation: {accum}")
-    t_end = args.warmup + args.steps
     t0 = time.perf_counter()
-    for t in range(args.warmup + 1, t_end + 1):
-        ctx.tick(t, wl.tick_time, ray_first=int(firsts[t - 1]), nray=len(ray_ticks[t - 1]), chem=chem)
+    run(t_first, t_end, chem)
     ctx.sync()
     wall = time.perf_counter() - t0
     log(f"{args.steps} ticks in {wall * 1e3:.2f} ms")
     st = ctx.stats()
+    if surrogate:
+        sg = wl.ps.copy()
+        ctx.state_download(sg)
+        dx = np.abs(sg.lambdaH - ex.lambdaH)
+        sur.update({"x_HII_err_max": float(dx.max()), "x_HII_err_p99": float(np.percentile(dx, 99)),
+                    "accepted": int(st["surrogate_count"]), "fallbacks": int(st["surrogate_fallbacks"]),
+                    "tick_speedup_vs_exact": sur["exact_ms_per_step"] / (wall / args.steps * 1e3)})
+        if not wl.physics.flags.h_only:
+            for f in ("lambdaHeI", "lambdaHeII"):
+                sur[f"{f}_err_max"] = float(np.abs(getattr(sg, f) - getattr(ex, f)).max())
+            sur["T_rel_err_p99"] = float(np.percentile(np.abs(sg.ie / ex.ie - 1), 99))
+        log(f"surrogate: {sur}")
     # per-phase pass: HIP events on the context stream around each kernel
     ctx.reset_stats()
     ctx.set_timing(True)
-    for t in range(t_end + 1, nt + 1):
-        ctx.tick(t, wl.tick_time, ray_first=int(firsts[t - 1]), nray=len(ray_ticks[t - 1]), chem=chem)
+    run(t_end + 1, nt, chem)
     ctx.sync()
     tm = ctx.timers()
     st_t = ctx.stats()
@@ -176,7 +225,8 @@ def main():
     dep_launch_s = tm["deposit"] / 1e3 / K
     roof_achieved = st_t["total_deposit_count"] / K * bytes_dep / dep_launch_s / 1e9
     out = {
-        "metric": "ray-particle intersections/s (whole tick, exact FP64 chemistry)",
+        "metric": "ray-particle intersections/s (whole tick, exact FP64 chemistry)" if not surrogate else
+                  "ray-particle intersections/s (whole tick, MLP-surrogate chemistry, exact FP64 fallback)",
         "value": value,
         "unit": "intersections/s",
         "n_gpus": 1,
@@ -204,6 +254,7 @@ def main():
         "deposit_walked_per_tick": st["deposit_processed"] / K,
         "solver_steps_per_update": st["solver_steps"] / max(st["active_count"], 1),
         "deposit_accumulation": accum,
+        "surrogate": sur,
         "roofline": {
             "kernel": "k_deposit",
             "bound": "hbm",

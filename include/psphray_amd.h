@@ -246,12 +246,16 @@ int psr_solve_steps(psr_ctx *ctx, psr_step *steps, int64_t n, double rtol);
  * in PSR_CHEM_SURROGATE ticks, trained on the device (BF16 MFMA forward and
  * backward, FP32 accumulation and Adam) on tuples the exact solver writes.
  * Features (psr_mlp_features): log10 nH, x_HI, x_HeI, x_HeII, log10 T; the
- * absorbed photons: log10(yGdepHI/HeI/HeII + 1e-12), log10(heat + 1e-30),
- * yeMET, eleven zero slots; log10 time.  Targets: x_HI, x_HeI, x_HeII and
+ * absorbed photons and the state they act on: log10(yGdepHI/HeI/HeII +
+ * 1e-12), log10(heat + 1e-30), yeMET, logit x_HI/HeI/HeII (clamped to
+ * [1e-6, 1 - 1e-6]), eight zero slots; log10 time.  Targets: x_HI, x_HeI, x_HeII and
  * log10 T after the step; the network predicts their change from features
- * 1..4.  A prediction is used when the features lie in the training domain
- * and the charge / particle-conservation residual (how far the fractions
- * leave [0, 1] with x_HeI + x_HeII <= 1) is at most the residual threshold;
+ * 1..4 (logit of the fractions, so they stay inside (0, 1)).  A prediction
+ * is used when the features lie in the training domain and the charge /
+ * particle-conservation residual x_HeI + x_HeII - 1 (x_HeIII >= 0) is at
+ * most the residual threshold, and (balance check, on by default) the
+ * trapezoidal ionisation-balance defect of the rate equations over the step
+ * (the photo-ionisations deposited against the change plus recombinations);
  * otherwise the exact solver runs for that particle. */
 #define PSR_MLP_IN 22
 #define PSR_MLP_HID 64
@@ -291,6 +295,11 @@ int psr_mlp_grad(psr_ctx *ctx, const float *x, const float *y, int64_t n,
                  float *grad, float *loss);
 /* residual threshold for the fallback to the exact solver (default 1e-4) */
 int psr_mlp_set_residual(psr_ctx *ctx, double tol);
+/* also require the ionisation-balance defect within the threshold (fractions)
+ * and ten times it relative (ie); default on.  Off, only the conservation
+ * residual and the domain guard the prediction: faster, but a network's
+ * occasional gross error then goes through (measured up to 1.0 in x_HII). */
+int psr_mlp_set_balance_check(psr_ctx *ctx, int32_t on);
 
 #ifdef __cplusplus
 }

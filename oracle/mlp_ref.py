@@ -127,8 +127,9 @@ def features(lH, lHeI, lHeII, T, nH, gHI, gHeI, gHeII, heat, yeMET, time):
     f[:, 7] = lg(np.asarray(gHeII) + 1e-12)
     f[:, 8] = lg(np.asarray(heat) + 1e-30)
     f[:, 9] = yeMET
-    f[:, 10] = logit(lH)
-    f[:, 11] = logit(lHeI)
-    f[:, 12] = logit(lHeII)
+    lf = lambda v: np.log(np.clip(v, 1e-6, 1 - 1e-6) / (1 - np.clip(v, 1e-6, 1 - 1e-6)))
+    f[:, 10] = lf(np.asarray(lH, np.float64))
+    f[:, 11] = lf(np.asarray(lHeI, np.float64))
+    f[:, 12] = lf(np.asarray(lHeII, np.float64))
     f[:, 21] = lg(time)
     return f

@@ -355,6 +355,9 @@ class Context:
     def mlp_set_residual(self, tol):
         self._check(self._lib.psr_mlp_set_residual(self.h, float(tol)), "psr_mlp_set_residual")
 
+    def mlp_set_balance_check(self, on=True):
+        self._check(self._lib.psr_mlp_set_balance_check(self.h, int(bool(on))), "psr_mlp_set_balance_check")
+
     def solve_steps(self, steps, rtol=-1.0):
         steps = np.ascontiguousarray(steps, STEP_DTYPE).copy()
         self._check(self._lib.psr_solve_steps(self.h, ptr(steps), len(steps), rtol),

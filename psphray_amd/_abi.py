@@ -129,6 +129,7 @@ SIGNATURES = {
     "psr_mlp_set_params": (C.c_int, [_VP, _VP]),
     "psr_mlp_grad": (C.c_int, [_VP, _VP, _VP, _I64, _VP, _VP]),
     "psr_mlp_set_residual": (C.c_int, [_VP, _D]),
+    "psr_mlp_set_balance_check": (C.c_int, [_VP, _I32]),
     "psr_mlp_capture": (C.c_int, [_VP, _I64]),
     "psr_mlp_captured": (C.c_int, [_VP, _VP, _VP, _VP, _I64]),
     "psr_mlp_set_norm": (C.c_int, [_VP, _VP, _VP, _VP, _VP]),

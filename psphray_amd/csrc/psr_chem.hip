@@ -678,17 +678,20 @@ __global__ __launch_bounds__(256) void k_solve(psr_step *__restrict__ steps, int
 // Surrogate chemistry (on-the-spot runs): 64 active particles per
 // wavefront per round; each lane builds its particle's step and features,
 // the wavefront runs the network over the 64 rows on MFMA (weights held in
-// registers), and each lane then checks its prediction x1 against the rate
-// equations: the features inside the training domain, and the particle-
-// conservation (ionisation balance) residual of every evolved variable,
-//     d = x1 - x0 - (f(x0) + f(x1)) / 2,
-// the trapezoidal defect of dx/dtau = f (src/solve.c:76-141, photo-
-// ionisations as the constant deposit rates), at most tol for the
-// fractions (one-sided where x1 sits at a bound the exact step clamps to,
-// src/solve.c:208-213) and 10 tol relative for ie.  Accepted steps are
-// stored as step_evolve's would be (src/loop.c:745-766); the others keep
-// active = 1 for the exact solver that runs next.
-template <bool ISO, bool ADI, bool HONLY>
+// registers), and each lane then checks its prediction x1:
+//   - the features lie inside the training domain;
+//   - the charge / particle-conservation residual, x_HeI + x_HeII - 1
+//     (x_HeIII >= 0; x_HII = 1 - x_HI and the electron count follow from the
+//     fractions), is at most tol;
+//   - with BALANCE, also the ionisation-balance residual of every evolved
+//     variable, d = x1 - x0 - (f(x0) + f(x1)) / 2, the trapezoidal defect of
+//     dx/dtau = f (src/solve.c:76-141, photo-ionisations as the constant
+//     deposit rates): at most tol for the fractions (one-sided where x1 sits
+//     at a bound the exact step clamps to, src/solve.c:208-213) and 10 tol
+//     relative for ie.
+// Accepted steps are stored as step_evolve's would be (src/loop.c:745-766);
+// the others keep active = 1 for the exact solver that runs next.
+template <bool ISO, bool ADI, bool HONLY, bool BALANCE>
 __global__ __launch_bounds__(256) void k_mlp_update(DevState S, int64_t npar, int64_t tick, double tick_time,
                                                     DevTables T, DevPhys P, DevStats *__restrict__ st,
                                                     const float *__restrict__ Pm, const float *__restrict__ norm,
@@ -740,24 +743,27 @@ __global__ __launch_bounds__(256) void k_mlp_update(DevState S, int64_t npar, in
             mlp_predict(f, s_out[wv][lane], pr);
             double x1[7] = {pr[0], HONLY ? x0[1] : pr[1], HONLY ? x0[2] : pr[2], x0[3], x0[4], x0[5], x0[6]};
             if (!ISO && !ADI) x1[6] = Tye2ie(P, exp10(pr[3]), lambda_ye(P, x1[0], x1[1], x1[2]));
-            Rates Riso;
-            if (ISO) rates_at<true>(T, log10(s.T), s.nH, Riso);
-            double f0[7], f1[7];
-            rhs<ISO, ADI, true, HONLY>(x0, f0, s, T, P, Riso);
-            rhs<ISO, ADI, true, HONLY>(x1, f1, s, T, P, Riso);
-            bool ok = inb;
+            bool ok = inb && isfinite(x1[0]) && isfinite(x1[6]);
+            if (!HONLY) ok = ok && x1[1] + x1[2] - 1.0 <= tol;
+            if (BALANCE) {
+                Rates Riso;
+                if (ISO) rates_at<true>(T, log10(s.T), s.nH, Riso);
+                double f0[7], f1[7];
+                rhs<ISO, ADI, true, HONLY>(x0, f0, s, T, P, Riso);
+                rhs<ISO, ADI, true, HONLY>(x1, f1, s, T, P, Riso);
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
-                if (HONLY && k > 0) break;
-                const double d = x1[k] - x0[k] - 0.5 * (f0[k] + f1[k]);
-                const bool lo = x1[k] <= 1e-6, hi = x1[k] >= 1.0 - 1e-6;
-                ok = ok && isfinite(d) && (lo ? d >= -tol : hi ? d <= tol : fabs(d) <= tol);
+                for (int k = 0; k < 3; k++) {
+                    if (HONLY && k > 0) break;
+                    const double d = x1[k] - x0[k] - 0.5 * (f0[k] + f1[k]);
+                    const bool lo = x1[k] <= 1e-6, hi = x1[k] >= 1.0 - 1e-6;
+                    ok = ok && isfinite(d) && (lo ? d >= -tol : hi ? d <= tol : fabs(d) <= tol);
+                }
+                if (!ISO && !ADI) {
+                    const double d = x1[6] - x0[6] - 0.5 * (f0[6] + f1[6]);
+                    ok = ok && isfinite(d) && fabs(d) <= 10.0 * tol * fmax(x0[6], x1[6]);
+                }
             }
-            if (!ISO && !ADI) {
-                const double d = x1[6] - x0[6] - 0.5 * (f0[6] + f1[6]);
-                ok = ok && isfinite(d) && fabs(d) <= 10.0 * tol * fmax(x0[6], x1[6]);
-            }
-            if (ok) {
+    if (ok) {
                 store_step<ISO, ADI>(S, (uint32_t)i, tick, P, x1, s.T);
                 nacc++;
             } else {
@@ -881,9 +887,17 @@ int psr_mlp_update_dev(psr_ctx *c, int64_t tick, double tick_time) {
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, c->cu_count * 4));
     PSR_EV(c, 3, 0);
     const int key = (c->F.isothermal ? 4 : 0) | (c->F.adiabatic ? 2 : 0) | (c->F.h_only ? 1 : 0);
-#define MLP_K(I, A, H) \
-    k_mlp_update<I, A, H><<<grid, 256, 0, c->stream>>>(S, n, tick, tick_time, c->T, P, c->d_stats, M.params, \
-                                                       M.norm, c->mlp_residual)
+#define MLP_K(I, A, H)                                                                                      \
+    do {                                                                                                    \
+        if (c->mlp_balance)                                                                                 \
+            k_mlp_update<I, A, H, true><<<grid, 256, 0, c->stream>>>(S, n, tick, tick_time, c->T, P,       \
+                                                                     c->d_stats, M.params, M.norm,         \
+                                                                     c->mlp_residual);                     \
+        else                                                                                                \
+            k_mlp_update<I, A, H, false><<<grid, 256, 0, c->stream>>>(S, n, tick, tick_time, c->T, P,      \
+                                                                      c->d_stats, M.params, M.norm,        \
+                                                                      c->mlp_residual);                    \
+    } while (0)
     switch (key) {
     case 0: MLP_K(false, false, false); break;
     case 1: MLP_K(false, false, true); break;

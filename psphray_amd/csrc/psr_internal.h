@@ -209,6 +209,7 @@ struct psr_ctx {
     // surrogate
     MlpState *mlp = nullptr;
     double mlp_residual = 1e-4;
+    bool mlp_balance = true;  // also check the ionisation-balance residual
     // ticks launched but not yet validated, and the pinned ring of flag words
     // the device publishes at the end of each tick (0 = not yet written)
     std::deque<PendingTick> pending;

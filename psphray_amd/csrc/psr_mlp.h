@@ -35,7 +35,18 @@ constexpr int MLP_ROW = MLP_IN + MLP_OUT;
 // training (the surrogate's domain), MLP_IN each
 constexpr int MLP_NORM = 4 * MLP_IN;
 
-__host__ __device__ inline float mlp_lg(double v, double floor) { return (float)log10(v > floor ? v : floor); }
+// log10 of a positive double in single precision: the exponent split off
+// first, since internal-unit values (nH) overflow float
+__host__ __device__ inline float mlp_lg(double v, double floor) {
+    int e;
+    const float m = (float)frexp(v > floor ? v : floor, &e);
+    return log10f(m) + (float)e * 0.30102999566398120f;
+}
+// logit of a fraction in single precision, clamped to [1e-6, 1 - 1e-6]
+__host__ __device__ inline float mlp_logitf(double x) {
+    const float v = (float)(x < 1e-6 ? 1e-6 : (x > 1.0 - 1e-6 ? 1.0 - 1e-6 : x));
+    return logf(v / (1.0f - v));
+}
 
 // log(x / (1 - x)) of a fraction clamped to [1e-9, 1 - 1e-9]
 __host__ __device__ inline double mlp_logit(double x) {
@@ -49,7 +60,8 @@ __host__ __device__ inline double mlp_sigmoid(double z) { return 1.0 / (1.0 + ex
 //   0 log10 nH, 1 x_HI, 2 x_HeI, 3 x_HeII, 4 log10 T,
 //   5..20 absorbed photons and the state they act on: 5..7 photo-
 //         ionisations per atom this step (yGdepHI, yGdepHeI, yGdepHeII),
-//         8 heat, 9 yeMET, 10..12 logit x_HI, x_HeI, x_HeII; 13..20 are zero
+//         8 heat, 9 yeMET, 10..12 logit x_HI, x_HeI, x_HeII (clamped to
+//         [1e-6, 1 - 1e-6]); 13..20 are zero
 //         (the state keeps no per-frequency counts and step_evolve reads
 //         the deposits only through 5..8),
 //   21 log10 of the step time.
@@ -69,9 +81,9 @@ __host__ __device__ inline void mlp_features(double lH, double lHeI, double lHeI
     f[7] = mlp_lg(gHeII + 1e-12, 1e-30);
     f[8] = mlp_lg(heat + 1e-30, 1e-30);
     f[9] = (float)yeMET;
-    f[10] = (float)mlp_logit(lH);
-    f[11] = (float)mlp_logit(lHeI);
-    f[12] = (float)mlp_logit(lHeII);
+    f[10] = mlp_logitf(lH);
+    f[11] = mlp_logitf(lHeI);
+    f[12] = mlp_logitf(lHeII);
     for (int k = 13; k < 21; k++) f[k] = 0.0f;
     f[21] = mlp_lg(time, 1e-30);
 }

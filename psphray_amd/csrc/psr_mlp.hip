@@ -779,3 +779,9 @@ extern "C" int psr_mlp_set_residual(psr_ctx *c, double tol) {
     c->mlp_residual = tol;
     return PSR_OK;
 }
+
+extern "C" int psr_mlp_set_balance_check(psr_ctx *c, int32_t on) {
+    if (!c) return PSR_ERR_ARG;
+    c->mlp_balance = on != 0;
+    return PSR_OK;
+}
