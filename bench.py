@@ -86,8 +86,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=6,
-                    help="untimed ticks (the first four also time the two deposit accumulations)")
+    ap.add_argument("--warmup", type=int, default=10,
+                    help="untimed ticks (they also run the autotuner's trials: four for the deposit "
+                         "accumulation, then four for graph vs kernel-by-kernel ticks)")
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--n", type=int, default=None, help="override lattice size")
     ap.add_argument("--nray", type=int, default=None)
@@ -187,13 +188,16 @@ def main():
     ctx.reset_stats()
     ctx.sync()
     accum = {v: k for k, v in accum_ids.items()}[ctx.tuning()]
-    log(f"deposit accumulation: {accum}")
+    graph = {-1: "undecided", 0: "kernel by kernel", 1: "graph"}[ctx.tuning(psr.PSR_TUNE_TICK_GRAPH)]
+    g0 = ctx.tuning(psr.PSR_TUNE_GRAPH_LAUNCHES)
+    log(f"deposit accumulation: {accum}; ticks: {graph}")
     t0 = time.perf_counter()
     run(t_first, t_end, chem)
     ctx.sync()
     wall = time.perf_counter() - t0
     log(f"{args.steps} ticks in {wall * 1e3:.2f} ms")
     st = ctx.stats()
+    ctx_graph_launches = ctx.tuning(psr.PSR_TUNE_GRAPH_LAUNCHES) - g0
     if surrogate:
         sg = wl.ps.copy()
         ctx.state_download(sg)
@@ -254,6 +258,8 @@ def main():
         "deposit_walked_per_tick": st["deposit_processed"] / K,
         "solver_steps_per_update": st["solver_steps"] / max(st["active_count"], 1),
         "deposit_accumulation": accum,
+        "tick_launch": graph,
+        "graph_launches_timed": ctx_graph_launches,
         "surrogate": sur,
         "roofline": {
             "kernel": "k_deposit",

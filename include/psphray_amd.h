@@ -233,6 +233,16 @@ int psr_get_timers(psr_ctx *ctx, double *ms);
 #define PSR_ACCUM_AUTO 0
 #define PSR_ACCUM_ATOMIC 1
 #define PSR_ACCUM_BINNED 2
+/* PSR_TUNE_TICK_GRAPH: 1 runs each asynchronous tick as one hipGraph
+ * launch, recorded once per tick shape (ray count, capacities, phases) and
+ * replayed with the tick's values from a device tick record; ticks with
+ * phase timers, accumulation trials or hit lists beyond 8192 hits run
+ * kernel by kernel.  0: always kernel by kernel.  -1 (default): four timed
+ * trial ticks, alternating, decide; psr_get_tuning then returns the choice
+ * (-1 while undecided).
+ * PSR_TUNE_GRAPH_LAUNCHES (read only): ticks run as graph launches so far. */
+#define PSR_TUNE_TICK_GRAPH 2
+#define PSR_TUNE_GRAPH_LAUNCHES 3
 int psr_set_tuning(psr_ctx *ctx, int32_t param, int32_t value);
 int psr_get_tuning(psr_ctx *ctx, int32_t param, int32_t *value);
 

@@ -13,12 +13,14 @@ import numpy as np
 from . import _abi
 from ._abi import (PSR_ACCUM_ATOMIC, PSR_ACCUM_AUTO, PSR_ACCUM_BINNED, PSR_CHEM_EXACT,
                    PSR_CHEM_SURROGATE, PSR_PHASE_ALL, PSR_PHASE_DEPOSIT, PSR_PHASE_TRACE,
-                   PSR_PHASE_UPDATE, PSR_TUNE_DEPOSIT_ACCUM, RAY_DTYPE, STEP_DTYPE, ptr)
+                   PSR_PHASE_UPDATE, PSR_TUNE_DEPOSIT_ACCUM, PSR_TUNE_GRAPH_LAUNCHES, PSR_TUNE_TICK_GRAPH,
+                   RAY_DTYPE, STEP_DTYPE, ptr)
 
 __all__ = ["Physics", "Context", "ParticleSystem", "PsrError", "RAY_DTYPE", "STEP_DTYPE",
            "PSR_PHASE_ALL", "PSR_PHASE_TRACE", "PSR_PHASE_DEPOSIT", "PSR_PHASE_UPDATE",
            "PSR_CHEM_EXACT", "PSR_CHEM_SURROGATE", "PSR_TUNE_DEPOSIT_ACCUM", "PSR_ACCUM_AUTO",
-           "PSR_ACCUM_ATOMIC", "PSR_ACCUM_BINNED", "default_units", "make_flags"]
+           "PSR_ACCUM_ATOMIC", "PSR_ACCUM_BINNED", "PSR_TUNE_TICK_GRAPH", "PSR_TUNE_GRAPH_LAUNCHES",
+           "default_units", "make_flags"]
 
 # internal unit defaults, src/config.c:43-51
 C_1_CMH = 3.0835455558318480e+21

@@ -25,6 +25,8 @@ PSR_TUNE_DEPOSIT_ACCUM = 1
 PSR_ACCUM_AUTO = 0
 PSR_ACCUM_ATOMIC = 1
 PSR_ACCUM_BINNED = 2
+PSR_TUNE_TICK_GRAPH = 2
+PSR_TUNE_GRAPH_LAUNCHES = 3
 
 _D = C.c_double
 _I32 = C.c_int32

@@ -175,6 +175,10 @@ extern "C" void psr_destroy(psr_ctx *c) {
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
     for (int k = 0; k < 2 * psr_ctx::TUNE_N; k++)
         if (c->tune_ev[k]) (void)hipEventDestroy(c->tune_ev[k]);
+    for (int k = 0; k < 2 * psr_ctx::TUNE_N; k++)
+        if (c->gtune_ev[k]) (void)hipEventDestroy(c->gtune_ev[k]);
+    psr_graph_reset(c);
+    dfree(c->d_tick);
     if (c->d_scratch) (void)hipFree(c->d_scratch);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -248,6 +252,7 @@ extern "C" int psr_set_physics(psr_ctx *c, const psr_units *u, const psr_flags *
     c->have_physics = true;
     c->ctx_dirty = true;
     psr_tune_reset(c);
+    psr_graph_reset(c);
     return PSR_OK;
 }
 
@@ -655,6 +660,7 @@ extern "C" int psr_tree_build(psr_ctx *c, const float *pos, const float *sml, co
     c->have_tree = true;
     c->ctx_dirty = true;
     psr_tune_reset(c);
+    psr_graph_reset(c);
     return PSR_OK;
 }
 
@@ -762,6 +768,7 @@ extern "C" int psr_state_upload(psr_ctx *c, const psr_state *s) {
     PSR_HIP(c, hipStreamSynchronize(c->stream));
     c->have_state = true;
     c->ctx_dirty = true;
+    psr_graph_reset(c);
     return PSR_OK;
 }
 

@@ -11,7 +11,6 @@
 // the Jacobian is a forward finite difference, as LSODA's.  The x_HI
 // variable and x_HII = 1 - x_HI are both kept in FP64.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 
@@ -496,10 +495,12 @@ __device__ __forceinline__ void count_steps(DevStats *st, int steps, int fail, i
 // (one slot claim per wavefront) and remembers the row; after it,
 // k_cap_post adds the targets from what store_step left: the fractions and
 // the temperature after the step.
-__global__ __launch_bounds__(256) void k_cap_pre(DevState S, int64_t npar, int64_t tick, double tick_time,
+__global__ __launch_bounds__(256) void k_cap_pre(DevState S, int64_t npar, const DevTick *__restrict__ tk,
                                                  DevPhys P, MlpDev M, uint32_t *__restrict__ row_of,
                                                  const int32_t *halt) {
     if (halted(halt)) return;
+    const int64_t tick = tk->tick;
+    const double tick_time = tk->tick_time;
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const bool on = i < npar && S.active[i];
     if (i < npar) row_of[i] = 0xFFFFFFFFu;
@@ -549,11 +550,14 @@ __global__ __launch_bounds__(256) void k_cap_post(DevState S, int64_t npar, DevP
 // their integrator state and are marked active = 2 for phase 2.  The split
 // bounds the wavefront divergence: step counts range from 1 to over 100.
 template <bool ISO, bool ADI, bool OTS, bool HONLY>
-__global__ __launch_bounds__(256) void k_update(DevState S, int64_t npar, int64_t tick, double tick_time,
+__global__ __launch_bounds__(256) void k_update(DevState S, int64_t npar, const DevTick *__restrict__ tk,
                                                 DevTables T, DevPhys P, DevStats *__restrict__ st,
-                                                double *__restrict__ saved) {
+                                                double *__restrict__ saved, uint32_t *__restrict__ list,
+                                                uint32_t *__restrict__ nlist) {
     using G = Integ<ISO, ADI, OTS, HONLY>;
     if (halted(&st->flags)) return;  // this tick's trace overflowed: replayed later
+    const int64_t tick = tk->tick;
+    const double tick_time = tk->tick_time;
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     int steps = 0, fail = 0, done = 0;
     if (closed_form<ISO, OTS, HONLY>()) {
@@ -588,6 +592,20 @@ __global__ __launch_bounds__(256) void k_update(DevState S, int64_t npar, int64_
             S.active[i] = 2;
         }
     }
+    // the unfinished particles into the phase-2 list, one claim per wavefront
+    // (the list's order does not matter: each particle is independent)
+    if (!closed_form<ISO, OTS, HONLY>()) {
+        const bool left = i < npar && S.active[i] == 2;
+        const unsigned long long m = __ballot(left);
+        if (m) {
+            const int lane = threadIdx.x & 63;
+            const int leader = __ffsll((long long)m) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(nlist, (uint32_t)__popcll(m));
+            base = __shfl(base, leader, 64);
+            if (left) list[base + __popcll(m & ((1ull << lane) - 1))] = (uint32_t)i;
+        }
+    }
     count_steps(st, steps, fail, done);
 }
 
@@ -595,11 +613,13 @@ __global__ __launch_bounds__(256) void k_update(DevState S, int64_t npar, int64_
 // compacted list whose length the selection wrote to device memory)
 template <bool ISO, bool ADI, bool OTS, bool HONLY>
 __global__ __launch_bounds__(256) void k_update_rest(DevState S, const uint32_t *__restrict__ list,
-                                                     const uint32_t *__restrict__ nlist, int64_t tick,
-                                                     double tick_time, DevTables T, DevPhys P,
+                                                     const uint32_t *__restrict__ nlist,
+                                                     const DevTick *__restrict__ tk, DevTables T, DevPhys P,
                                                      DevStats *__restrict__ st, const double *__restrict__ saved) {
     using G = Integ<ISO, ADI, OTS, HONLY>;
     if (halted(&st->flags)) return;
+    const int64_t tick = tk->tick;
+    const double tick_time = tk->tick_time;
     const uint32_t n = *nlist;
     int steps = 0, fail = 0, done = 0;
     // uniform trip count so every lane reaches the wavefront reduction
@@ -626,11 +646,6 @@ __global__ __launch_bounds__(256) void k_update_rest(DevState S, const uint32_t 
     }
     count_steps(st, steps, fail, done);
 }
-
-struct IsLeftOver {
-    const uint8_t *active;
-    __host__ __device__ bool operator()(uint32_t i) const { return active[i] == 2; }
-};
 
 template <bool ISO, bool ADI, bool OTS, bool HONLY>
 __global__ __launch_bounds__(256) void k_solve(psr_step *__restrict__ steps, int64_t n, DevTables T, DevPhys P,
@@ -692,7 +707,7 @@ __global__ __launch_bounds__(256) void k_solve(psr_step *__restrict__ steps, int
 // Accepted steps are stored as step_evolve's would be (src/loop.c:745-766);
 // the others keep active = 1 for the exact solver that runs next.
 template <bool ISO, bool ADI, bool HONLY, bool BALANCE>
-__global__ __launch_bounds__(256) void k_mlp_update(DevState S, int64_t npar, int64_t tick, double tick_time,
+__global__ __launch_bounds__(256) void k_mlp_update(DevState S, int64_t npar, const DevTick *__restrict__ tk,
                                                     DevTables T, DevPhys P, DevStats *__restrict__ st,
                                                     const float *__restrict__ Pm, const float *__restrict__ norm,
                                                     double tol) {
@@ -701,6 +716,8 @@ __global__ __launch_bounds__(256) void k_mlp_update(DevState S, int64_t npar, in
     __shared__ float s_out[4][64][MLP_OUT];
     __shared__ int part[2][4];
     if (halted(&st->flags)) return;
+    const int64_t tick = tk->tick;
+    const double tick_time = tk->tick_time;
     for (int k = threadIdx.x; k < 3 * MLP_HID + MLP_OUT; k += blockDim.x) {
         const int l = k / MLP_HID, o = k % MLP_HID;
         s_bias[k] = Pm[(l == 0 ? MLP_B1 : l == 1 ? MLP_B2 : l == 2 ? MLP_B3 : MLP_B4) + o];
@@ -795,6 +812,10 @@ __global__ __launch_bounds__(256) void k_mlp_update(DevState S, int64_t npar, in
 
 }  // namespace
 
+__global__ void k_zero_u32(uint32_t *p) {
+    if (threadIdx.x == 0) *p = 0;
+}
+
 #define CHEM_DISPATCH(KERNEL, ...)                                                                       \
     do {                                                                                                 \
         const int key_ = (c->F.isothermal ? 8 : 0) | (c->F.adiabatic ? 4 : 0) |                          \
@@ -815,11 +836,7 @@ __global__ __launch_bounds__(256) void k_mlp_update(DevState S, int64_t npar, in
         }                                                                                                \
     } while (0)
 
-int psr_launch_update(psr_ctx *c, int64_t tick, double tick_time, int32_t, bool timed) {
-    if (c->npar == 0) return PSR_OK;
-    const DevPhys P = psr_dev_phys(c);
-    const DevState S = c->state();
-    const MlpDev M = psr_mlp_dev(c);
+int psr_prepare_update(psr_ctx *c) {
     const int64_t n = c->npar;
     if (c->qlist_cap < n) {
         PSR_HIP(c, hipStreamSynchronize(c->stream));  // earlier ticks may still use them
@@ -833,36 +850,41 @@ int psr_launch_update(psr_ctx *c, int64_t tick, double tick_time, int32_t, bool 
         c->qlist_cap = n;
     }
     if (!c->d_q) PSR_HIP(c, hipMalloc(&c->d_q, sizeof(uint32_t) * 2));
+    if (psr_mlp_dev(c).cap && !psr_mlp_rowof(c, n)) return PSR_ERR_HIP;
+    return PSR_OK;
+}
+
+int psr_launch_update(psr_ctx *c, bool timed) {
+    if (c->npar == 0) return PSR_OK;
+    const DevPhys P = psr_dev_phys(c);
+    const DevState S = c->state();
+    const MlpDev M = psr_mlp_dev(c);
+    const int64_t n = c->npar;
+    const DevTick *tk = c->d_tick;
+    int rc = psr_prepare_update(c);
+    if (rc) return rc;
     if (timed) PSR_EV(c, 3, 0);
     unsigned grid = (unsigned)((n + 255) / 256);
     const int32_t *halt = &c->d_stats->flags;
     uint32_t *row_of = nullptr;
     if (M.cap) {
-        if (!(row_of = psr_mlp_rowof(c, n))) return PSR_ERR_HIP;
-        k_cap_pre<<<grid, 256, 0, c->stream>>>(S, n, tick, tick_time, P, M, row_of, halt);
+        row_of = psr_mlp_rowof(c, n);
+        k_cap_pre<<<grid, 256, 0, c->stream>>>(S, n, tk, P, M, row_of, halt);
         PSR_HIP(c, hipGetLastError());
     }
-    CHEM_DISPATCH(k_update, S, n, tick, tick_time, c->T, P, c->d_stats, c->d_saved);
-    PSR_HIP(c, hipGetLastError());
-    if (c->F.isothermal && c->F.on_the_spot && c->F.h_only) {
-        // closed form: every particle finished in phase 1
-        if (M.cap) {
-            k_cap_post<<<grid, 256, 0, c->stream>>>(S, n, P, M, row_of, 1, halt);
-            PSR_HIP(c, hipGetLastError());
-        }
-        if (timed) PSR_EV(c, 3, 1);
-        return PSR_OK;
+    const bool closed = c->F.isothermal && c->F.on_the_spot && c->F.h_only;
+    if (!closed) {
+        k_zero_u32<<<1, 64, 0, c->stream>>>(c->d_q);
+        PSR_HIP(c, hipGetLastError());
     }
-    // leftovers: compact with a device-wide selection (scan based)
-    hipcub::CountingInputIterator<uint32_t> it(0);
-    IsLeftOver pred{c->d_active};
-    size_t tb = 0;
-    PSR_HIP(c, hipcub::DeviceSelect::If(nullptr, tb, it, c->d_qlist, c->d_q, (int)n, pred, c->stream));
-    if (psr_ensure_scratch(c, tb)) return PSR_ERR_HIP;
-    PSR_HIP(c, hipcub::DeviceSelect::If(c->d_scratch, tb, it, c->d_qlist, c->d_q, (int)n, pred, c->stream));
-    grid = (unsigned)std::max(1, c->cu_count * 8);
-    CHEM_DISPATCH(k_update_rest, S, c->d_qlist, c->d_q, tick, tick_time, c->T, P, c->d_stats, c->d_saved);
+    CHEM_DISPATCH(k_update, S, n, tk, c->T, P, c->d_stats, c->d_saved, c->d_qlist, c->d_q);
     PSR_HIP(c, hipGetLastError());
+    if (!closed) {
+        // phase 2: the particles k_update listed as unfinished
+        grid = (unsigned)std::max(1, c->cu_count * 8);
+        CHEM_DISPATCH(k_update_rest, S, c->d_qlist, c->d_q, tk, c->T, P, c->d_stats, c->d_saved);
+        PSR_HIP(c, hipGetLastError());
+    }
     if (M.cap) {
         grid = (unsigned)((n + 255) / 256);
         k_cap_post<<<grid, 256, 0, c->stream>>>(S, n, P, M, row_of, c->F.isothermal ? 1 : 0, halt);
@@ -876,10 +898,10 @@ int psr_launch_update(psr_ctx *c, int64_t tick, double tick_time, int32_t, bool 
 // particles whose prediction was rejected (and for every particle when the
 // run is not on-the-spot, whose recombination counters the network does not
 // predict)
-int psr_mlp_update_dev(psr_ctx *c, int64_t tick, double tick_time) {
+int psr_mlp_update_dev(psr_ctx *c) {
     if (!psr_mlp_ready(c)) PSR_FAIL(c, PSR_ERR_STATE, "surrogate chemistry: no trained network (psr_mlp_train)");
     if (c->npar == 0) return PSR_OK;
-    if (!c->F.on_the_spot) return psr_launch_update(c, tick, tick_time, PSR_CHEM_EXACT);
+    if (!c->F.on_the_spot) return psr_launch_update(c);
     const DevPhys P = psr_dev_phys(c);
     const DevState S = c->state();
     const MlpDev M = psr_mlp_dev(c);
@@ -890,11 +912,11 @@ int psr_mlp_update_dev(psr_ctx *c, int64_t tick, double tick_time) {
 #define MLP_K(I, A, H)                                                                                      \
     do {                                                                                                    \
         if (c->mlp_balance)                                                                                 \
-            k_mlp_update<I, A, H, true><<<grid, 256, 0, c->stream>>>(S, n, tick, tick_time, c->T, P,       \
+            k_mlp_update<I, A, H, true><<<grid, 256, 0, c->stream>>>(S, n, c->d_tick, c->T, P,             \
                                                                      c->d_stats, M.params, M.norm,         \
                                                                      c->mlp_residual);                     \
         else                                                                                                \
-            k_mlp_update<I, A, H, false><<<grid, 256, 0, c->stream>>>(S, n, tick, tick_time, c->T, P,      \
+            k_mlp_update<I, A, H, false><<<grid, 256, 0, c->stream>>>(S, n, c->d_tick, c->T, P,            \
                                                                       c->d_stats, M.params, M.norm,        \
                                                                       c->mlp_residual);                    \
     } while (0)
@@ -908,7 +930,7 @@ int psr_mlp_update_dev(psr_ctx *c, int64_t tick, double tick_time) {
     }
 #undef MLP_K
     PSR_HIP(c, hipGetLastError());
-    int rc = psr_launch_update(c, tick, tick_time, PSR_CHEM_EXACT, false);
+    int rc = psr_launch_update(c, false);
     PSR_EV(c, 3, 1);
     return rc;
 }

@@ -443,12 +443,13 @@ __device__ __forceinline__ void rec_commit(RecCarry<HE> &c, const DevBins &B, co
 }
 
 template <bool HE>
-__global__ __launch_bounds__(FTHREADS) void k_deposit(const psr_ray *__restrict__ rays, int64_t nray, DevHits H,
+__global__ __launch_bounds__(FTHREADS) void k_deposit(const DevTick *__restrict__ tk, int64_t nray, DevHits H,
                                                       const DevCtx *__restrict__ C, DevStats *__restrict__ st0,
                                                       double *__restrict__ ray_len, double *__restrict__ ray_tm,
                                                       DevBins Bn, const int32_t *halt) {
     __shared__ DepLds<HE> L;
     if (halted(halt)) return;
+    const psr_ray *__restrict__ rays = tk->rays;
     const DevState &S = C->S;
     const DevTables &T = C->T;
     const DevPhys &P = C->P;
@@ -858,12 +859,19 @@ static int ensure_bins(psr_ctx *c, bool he) {
     return PSR_OK;
 }
 
-int psr_launch_deposit(psr_ctx *c, int64_t ray_first, int64_t nray, bool binned) {
+int psr_prepare_deposit(psr_ctx *c, bool binned) {
     int rc = psr_sync_dev_ctx(c);
     if (rc) return rc;
+    return binned ? ensure_bins(c, !c->F.h_only) : PSR_OK;
+}
+
+// the rays of the tick come from the tick record (ray_first is kept for the
+// call's shape checks)
+int psr_launch_deposit(psr_ctx *c, int64_t, int64_t nray, bool binned) {
+    int rc = psr_prepare_deposit(c, binned);
+    if (rc) return rc;
     const bool he = !c->F.h_only;
-    if (binned && (rc = ensure_bins(c, he))) return rc;
-    const psr_ray *rays = c->d_rays + ray_first;
+    const DevTick *rays = c->d_tick;
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nray, 65535 * 4));
     const int32_t *halt = &c->d_stats->flags;
     const int64_t nbins = binned ? c->bin_n : 0;

@@ -16,6 +16,7 @@
 #include <stdint.h>
 
 #include <deque>
+#include <vector>
 #include <string>
 
 #include "psphray_amd.h"
@@ -127,6 +128,16 @@ __device__ __forceinline__ bool halted(const int32_t *halt) { return halt && (*(
 
 // a tick launched without waiting for its trace's capacity check; validated
 // later from its published flag word (host ring slot)
+// per-tick values the tick's kernels read from device memory (written by
+// k_set_tick at the start of every tick), so that one recorded hipGraph
+// serves every tick of the same shape
+struct DevTick {
+    const psr_ray *rays;  // the tick's rays: pool + ray_first
+    int64_t tick;
+    double tick_time;
+    int32_t *slot;  // pinned ring word k_publish writes
+};
+
 struct PendingTick {
     int64_t ray_first, nray, tick;
     double tick_time;
@@ -218,11 +229,22 @@ struct psr_ctx {
     // deposit accumulation: requested (PSR_ACCUM_*), chosen (ATOMIC or BINNED
     // once decided, AUTO while the trial ticks run), and the trials: ticks
     // alternating the two choices, each with an event pair around its deposit
+    static constexpr int TUNE_N = 4;  // timed trials per tuned choice
+    DevTick *d_tick = nullptr;
+    // the tick recorded as a hipGraph: its executable and the shape it was
+    // recorded for (sizes and device pointers its kernels were given)
+    int32_t graph_req = -1;  // -1: chosen by timed trials, 0 / 1: off / on
+    int32_t graph_use = -1;  // the choice once made
+    hipEvent_t gtune_ev[2 * TUNE_N] = {};
+    int32_t gtune_slot[TUNE_N] = {};
+    int gtune_n = 0;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<int64_t> graph_key;
+    int64_t graph_launches = 0;
     int32_t accum_req = PSR_ACCUM_AUTO;
     int32_t accum_use = PSR_ACCUM_AUTO;
-    static constexpr int TUNE_N = 4;
     hipEvent_t tune_ev[2 * TUNE_N] = {};
-    uint32_t tune_slot[TUNE_N] = {};
+    int32_t tune_slot[TUNE_N] = {};  // ring slot of the trial's tick; -1: a synchronous replay
     int tune_n = 0;
 
     DevState state() const;
@@ -231,20 +253,28 @@ struct psr_ctx {
 // ---- launchers implemented in the .hip files ----
 // sync: the trace waits for its capacity check (and retries); otherwise the
 // kernels read the halt flag and the tick is validated later
-int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray, bool sync);
+int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray, bool sync, const DevTick *tk);
 int psr_launch_sort(psr_ctx *c, int64_t nray);
+// hit slots for nray rays of cap hits each (reallocates after a sync)
+int psr_ensure_hits(psr_ctx *c, int64_t nray, int32_t cap);
 int psr_launch_deposit(psr_ctx *c, int64_t ray_first, int64_t nray, bool binned);
 // restart the accumulation trials (tree or physics changed)
 void psr_tune_reset(psr_ctx *c);
-// timed: record the update phase's event pair (the surrogate records its own)
-int psr_launch_update(psr_ctx *c, int64_t tick, double tick_time, int32_t chem_mode, bool timed = true);
+// timed: record the update phase's event pair (the surrogate records its own);
+// the tick and its length come from c->d_tick
+int psr_launch_update(psr_ctx *c, bool timed = true);
+// allocations of the deposit / update phases ahead of a graph recording
+int psr_prepare_deposit(psr_ctx *c, bool binned);
+int psr_prepare_update(psr_ctx *c);
+// drop the recorded tick graph (physics, tree, state or options changed)
+void psr_graph_reset(psr_ctx *c);
 // validate (and if needed replay) every pending tick; blocks
 int psr_resolve(psr_ctx *c);
 int psr_solve_steps_dev(psr_ctx *c, psr_step *d_steps, int64_t n, double rtol);
 DevPhys psr_dev_phys(const psr_ctx *c);
 int psr_sync_dev_ctx(psr_ctx *c);
 int psr_ensure_scratch(psr_ctx *c, size_t bytes);
-int psr_mlp_update_dev(psr_ctx *c, int64_t tick, double tick_time);
+int psr_mlp_update_dev(psr_ctx *c);
 // the surrogate's device pointers: parameters, normalisation, and the
 // capture buffer of exact steps (cap null when capture is off)
 struct MlpDev {

@@ -62,10 +62,12 @@ __device__ __forceinline__ bool box_hit(const float s[3], const float dir[3], fl
 }
 
 __global__ __launch_bounds__(TR_THREADS) void k_trace(
-    const psr_ray *__restrict__ rays, int64_t nray, const Node *__restrict__ nodes,
-    const int32_t *__restrict__ froot, const float4 *__restrict__ ps,
+    const psr_ray *__restrict__ rays_arg, const DevTick *__restrict__ tk, int64_t nray,
+    const Node *__restrict__ nodes, const int32_t *__restrict__ froot, const float4 *__restrict__ ps,
     const uint32_t *__restrict__ orig, DevHits H, uint8_t *__restrict__ active,
     DevStats *__restrict__ st, const int32_t *halt) {
+    // a tick's rays come through its tick record, parity tracing passes them
+    const psr_ray *__restrict__ rays = tk ? tk->rays : rays_arg;
     __shared__ int32_t stack[TR_SCAP];
     __shared__ int32_t leafq[TR_LCAP];
     __shared__ int32_t s_sp, s_lq, s_nh, s_cs, s_cl, s_ovf;
@@ -396,7 +398,7 @@ __global__ void k_export_lists(int64_t nray, DevHits H, const uint32_t *__restri
 }
 
 // ------------------------------------------------------------ launchers
-static int ensure_hits(psr_ctx *c, int64_t nray, int32_t cap) {
+int psr_ensure_hits(psr_ctx *c, int64_t nray, int32_t cap) {
     if (nray <= c->hit_slots && cap <= c->H.cap && c->H.d) return PSR_OK;
     const int64_t ns = std::max(nray, c->hit_slots);
     const int32_t cp = std::max(cap, c->H.cap);
@@ -439,16 +441,16 @@ static int clear_flags(psr_ctx *c) {
     return PSR_OK;
 }
 
-int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray, bool sync) {
+int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray, bool sync, const DevTick *tk) {
     int32_t cap0 = c->H.cap > 0 ? c->H.cap : 2048;
-    int rc = ensure_hits(c, nray, cap0);
+    int rc = psr_ensure_hits(c, nray, cap0);
     if (rc) return rc;
     const unsigned grid = (unsigned)std::min<int64_t>(nray, 65535 * 4);
     if (!sync) {
         // the flag stays as the previous ticks left it: set only if one of
         // them overflowed, and then this tick is replayed as well
         PSR_EV(c, 0, 0);
-        k_trace<<<grid, TR_THREADS, 0, c->stream>>>(c->d_rays + ray_first, nray, c->d_nodes,
+        k_trace<<<grid, TR_THREADS, 0, c->stream>>>(c->d_rays + ray_first, tk, nray, c->d_nodes,
                                                     c->d_field_root, c->d_ps, c->d_orig, c->H,
                                                     c->d_active, c->d_stats, &c->d_stats->flags);
         PSR_HIP(c, hipGetLastError());
@@ -459,7 +461,7 @@ int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray, bool sync) {
     for (int attempt = 0; attempt < 8; attempt++) {
         if ((rc = clear_flags(c))) return rc;
         PSR_EV(c, 0, 0);
-        k_trace<<<grid, TR_THREADS, 0, c->stream>>>(c->d_rays + ray_first, nray, c->d_nodes,
+        k_trace<<<grid, TR_THREADS, 0, c->stream>>>(c->d_rays + ray_first, tk, nray, c->d_nodes,
                                                     c->d_field_root, c->d_ps, c->d_orig, c->H,
                                                     c->d_active, c->d_stats, nullptr);
         PSR_HIP(c, hipGetLastError());
@@ -481,7 +483,7 @@ int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray, bool sync) {
         c->H.cap = 0;  // force reallocation at the new size
         const int64_t ns = c->hit_slots;
         c->hit_slots = 0;
-        if ((rc = ensure_hits(c, std::max(ns, nray), ncap))) return rc;
+        if ((rc = psr_ensure_hits(c, std::max(ns, nray), ncap))) return rc;
     }
     PSR_FAIL(c, PSR_ERR_CAPACITY, "hit capacity did not converge");
 }
@@ -560,7 +562,7 @@ extern "C" int psr_trace_lists(psr_ctx *c, int64_t ray_first, int64_t nray, int6
     if ((rc = psr_resolve(c))) return rc;
     offsets[0] = 0;
     if (nray == 0) return PSR_OK;
-    if ((rc = psr_launch_trace(c, ray_first, nray, true))) return rc;
+    if ((rc = psr_launch_trace(c, ray_first, nray, true, nullptr))) return rc;
     // parity mode marks particles active like the tick does; clear them
     PSR_HIP(c, hipMemsetAsync(c->d_active, 0, c->npar > 0 ? c->npar : 1, c->stream));
     if ((rc = psr_launch_sort(c, nray))) return rc;
@@ -606,49 +608,107 @@ extern "C" int psr_trace_lists(psr_ctx *c, int64_t ray_first, int64_t nray, int6
 void psr_tune_reset(psr_ctx *c) {
     c->tune_n = 0;
     c->accum_use = PSR_ACCUM_AUTO;
+    c->gtune_n = 0;
+    c->graph_use = -1;
 }
 
-// once every trial tick has been published: keep the accumulation whose
-// fastest trial deposit was faster; trials of a tick that overflowed (its
-// kernels stopped early) start the trials again.  Blocks only when `block`.
+// once every trial has run: keep the accumulation whose fastest trial
+// deposit was faster.  A trial on an asynchronous tick that overflowed (its
+// kernels stopped early, its replay ran later) does not count; trials on
+// replays do.  Without a valid trial of each kind the trials start again.
+// Blocks only when `block`.
 static int tune_decide(psr_ctx *c, bool block) {
     if (c->accum_use != PSR_ACCUM_AUTO || c->tune_n < psr_ctx::TUNE_N) return PSR_OK;
     const hipEvent_t last = c->tune_ev[2 * psr_ctx::TUNE_N - 1];
     if (block) PSR_HIP(c, hipEventSynchronize(last));
     else if (hipEventQuery(last) != hipSuccess) return PSR_OK;
-    float best[2] = {1e30f, 1e30f};
+    float best[2] = {INFINITY, INFINITY};
     for (int k = 0; k < psr_ctx::TUNE_N; k++) {
-        const int32_t v = ((volatile int32_t *)c->h_ring)[c->tune_slot[k]];
-        if (v == 0 && !block) return PSR_OK;
-        if (v == 0 || (v & PSR_DF_HALT)) {
-            psr_tune_reset(c);
-            return PSR_OK;
+        if (c->tune_slot[k] >= 0) {
+            const int32_t v = ((volatile int32_t *)c->h_ring)[c->tune_slot[k]];
+            if (v == 0 && !block) return PSR_OK;
+            if (v == 0 || (v & PSR_DF_HALT)) continue;
         }
         float ms = 0;
         PSR_HIP(c, hipEventElapsedTime(&ms, c->tune_ev[2 * k], c->tune_ev[2 * k + 1]));
         best[k & 1] = std::min(best[k & 1], ms);
     }
+    if (!(best[0] < INFINITY && best[1] < INFINITY)) {
+        psr_tune_reset(c);
+        return PSR_OK;
+    }
     c->accum_use = best[1] < best[0] ? PSR_ACCUM_BINNED : PSR_ACCUM_ATOMIC;
     return PSR_OK;
 }
 
-// the kernels of one tick; sync = wait for the trace's capacity check
-static int tick_body(psr_ctx *c, const PendingTick &t, bool sync) {
+// the tick record of the kernels that follow (stream order keeps it from
+// changing under the previous tick's kernels)
+__global__ void k_set_tick(DevTick *tk, const psr_ray *rays, int64_t tick, double tick_time, int32_t *slot) {
+    if (threadIdx.x == 0) *tk = DevTick{rays, tick, tick_time, slot};
+}
+
+__global__ void k_fill_u8(uint8_t *p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+// the accumulation of this tick's deposit: the requested or decided one,
+// else the next trial (odd trials binned); replays use binning meanwhile
+static bool tick_binned(const psr_ctx *c, int *trial) {
+    *trial = -1;
+    if (c->accum_req != PSR_ACCUM_AUTO) return c->accum_req == PSR_ACCUM_BINNED;
+    if (c->accum_use != PSR_ACCUM_AUTO) return c->accum_use == PSR_ACCUM_BINNED;
+    if (c->tune_n < psr_ctx::TUNE_N) {
+        *trial = c->tune_n;
+        return *trial & 1;
+    }
+    return true;
+}
+
+// the tick-graph trials: whole ticks alternately kernel by kernel (even
+// trials) and as graph launches, timed from after the recording to the end
+// of the tick; the faster way is kept.  Trials on ticks that overflowed do
+// not count.
+static int gtune_decide(psr_ctx *c, bool block) {
+    if (c->graph_use >= 0 || c->gtune_n < psr_ctx::TUNE_N) return PSR_OK;
+    const hipEvent_t last = c->gtune_ev[2 * psr_ctx::TUNE_N - 1];
+    if (block) PSR_HIP(c, hipEventSynchronize(last));
+    else if (hipEventQuery(last) != hipSuccess) return PSR_OK;
+    float best[2] = {INFINITY, INFINITY};
+    for (int k = 0; k < psr_ctx::TUNE_N; k++) {
+        const int32_t v = ((volatile int32_t *)c->h_ring)[c->gtune_slot[k]];
+        if (v == 0 && !block) return PSR_OK;
+        if (v == 0 || (v & PSR_DF_HALT)) continue;
+        float ms = 0;
+        PSR_HIP(c, hipEventElapsedTime(&ms, c->gtune_ev[2 * k], c->gtune_ev[2 * k + 1]));
+        best[k & 1] = std::min(best[k & 1], ms);
+    }
+    if (!(best[0] < INFINITY && best[1] < INFINITY)) {
+        c->gtune_n = 0;
+        return PSR_OK;
+    }
+    c->graph_use = best[1] < best[0] ? 1 : 0;
+    return PSR_OK;
+}
+
+// the kernels of one tick; sync = wait for the trace's capacity check.
+// set: write the tick record first (a recorded graph leaves that to its
+// launch, since the record is what changes from tick to tick)
+static int tick_body(psr_ctx *c, const PendingTick &t, bool sync, bool set = true) {
     int rc;
     const int32_t *halt = sync ? nullptr : &c->d_stats->flags;
+    if (set) {
+        k_set_tick<<<1, 64, 0, c->stream>>>(c->d_tick, c->d_rays + t.ray_first, t.tick, t.tick_time,
+                                            c->h_ring + t.slot);
+        PSR_HIP(c, hipGetLastError());
+    }
     if (t.nray > 0) {
-        if ((rc = psr_launch_trace(c, t.ray_first, t.nray, sync))) return rc;
+        if ((rc = psr_launch_trace(c, t.ray_first, t.nray, sync, c->d_tick))) return rc;
         if (t.phases & PSR_PHASE_DEPOSIT) {
             if ((rc = psr_launch_sort(c, t.nray))) return rc;
-            // accumulation: the requested or decided one, else the next
-            // trial (odd trials binned); replays use binning meanwhile
-            int trial = -1;
-            bool binned = true;
-            if (c->accum_req != PSR_ACCUM_AUTO) binned = c->accum_req == PSR_ACCUM_BINNED;
-            else if (c->accum_use != PSR_ACCUM_AUTO) binned = c->accum_use == PSR_ACCUM_BINNED;
-            else if (!sync && c->tune_n < psr_ctx::TUNE_N) {
-                trial = c->tune_n;
-                binned = trial & 1;
+            int trial;
+            const bool binned = tick_binned(c, &trial);
+            if (trial >= 0) {
                 if (!c->tune_ev[0])
                     for (int k = 0; k < 2 * psr_ctx::TUNE_N; k++) PSR_HIP(c, hipEventCreate(&c->tune_ev[k]));
                 PSR_HIP(c, hipEventRecord(c->tune_ev[2 * trial], c->stream));
@@ -656,7 +716,7 @@ static int tick_body(psr_ctx *c, const PendingTick &t, bool sync) {
             if ((rc = psr_launch_deposit(c, t.ray_first, t.nray, binned))) return rc;
             if (trial >= 0) {
                 PSR_HIP(c, hipEventRecord(c->tune_ev[2 * trial + 1], c->stream));
-                c->tune_slot[trial] = t.slot;
+                c->tune_slot[trial] = sync ? -1 : (int32_t)t.slot;
                 c->tune_n++;
             }
         } else {
@@ -666,28 +726,131 @@ static int tick_body(psr_ctx *c, const PendingTick &t, bool sync) {
         }
     }
     if (t.phases & PSR_PHASE_UPDATE) {
-        if (t.chem_mode == PSR_CHEM_SURROGATE) rc = psr_mlp_update_dev(c, t.tick, t.tick_time);
-        else rc = psr_launch_update(c, t.tick, t.tick_time, t.chem_mode);
+        if (t.chem_mode == PSR_CHEM_SURROGATE) rc = psr_mlp_update_dev(c);
+        else rc = psr_launch_update(c);
         if (rc) return rc;
-    } else {
+    } else if (c->npar > 0) {
         // merge without update: the active set is dropped (traceOnly); an
         // overflowed tick's replay marks its particles again
-        PSR_HIP(c, hipMemsetAsync(c->d_active, 0, c->npar > 0 ? c->npar : 1, c->stream));
+        k_fill_u8<<<(unsigned)std::min<int64_t>((c->npar + 255) / 256, 4096), 256, 0, c->stream>>>(c->d_active,
+                                                                                                 c->npar);
+        PSR_HIP(c, hipGetLastError());
     }
     return PSR_OK;
 }
 
 // end of a tick: its flag word (with bit 8 set, so 0 means "not yet") into
 // the pinned ring slot the host polls
-__global__ void k_publish(const int32_t *__restrict__ flags, int32_t *slot) {
-    if (threadIdx.x == 0) *(volatile int32_t *)slot = 0x100 | *(volatile const int32_t *)flags;
+__global__ void k_publish(const int32_t *__restrict__ flags, const DevTick *__restrict__ tk) {
+    if (threadIdx.x == 0) *(volatile int32_t *)tk->slot = 0x100 | *(volatile const int32_t *)flags;
+}
+
+void psr_graph_reset(psr_ctx *c) {
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    c->graph_exec = nullptr;
+    c->graph_key.clear();
+}
+
+// A tick can run as a recorded graph when nothing in it waits on the host:
+// no phase timers, no accumulation trial, hit lists within the LDS sort
+// tiers (longer lists are sorted through a host-built list).  The shape key
+// holds every size and device pointer the tick's launches are given.
+static bool graph_usable(const psr_ctx *c, const PendingTick &t) {
+    if (c->timing || t.nray <= 0) return false;
+    int trial;
+    (void)tick_binned(c, &trial);
+    if (trial >= 0) return false;
+    return c->H.cap > 0 && c->H.cap <= 8192;
+}
+
+static std::vector<int64_t> graph_key(const psr_ctx *c, const PendingTick &t, bool binned) {
+    const MlpDev M = psr_mlp_dev(c);
+    return {t.nray,
+            t.phases,
+            t.chem_mode,
+            (int64_t)binned,
+            c->H.cap,
+            c->hit_slots,
+            (int64_t)(intptr_t)c->H.d,
+            (int64_t)(intptr_t)c->d_bin_rec,
+            c->bin_n,
+            (int64_t)(intptr_t)c->d_qlist,
+            (int64_t)(intptr_t)c->d_saved,
+            (int64_t)(intptr_t)c->d_q,
+            c->npar,
+            (int64_t)(intptr_t)c->d_ctx,
+            (int64_t)(intptr_t)M.params,
+            (int64_t)(intptr_t)M.cap,
+            (int64_t)c->mlp_balance,
+            (int64_t)(c->mlp_residual * 1e12)};
+}
+
+// one tick as a graph launch: allocations first (nothing may allocate or
+// wait while recording), the recording when the shape changed, then the
+// tick record and the launch
+static int tick_graph(psr_ctx *c, const PendingTick &t, bool *done, int *trial) {
+    *done = false;
+    *trial = -1;
+    int rc;
+    if (c->graph_req == 0 || (c->graph_req < 0 && c->graph_use == 0)) return PSR_OK;
+    const int32_t cap0 = c->H.cap > 0 ? c->H.cap : 2048;
+    if ((rc = psr_ensure_hits(c, t.nray, cap0))) return rc;
+    if (!graph_usable(c, t)) return PSR_OK;
+    if (c->graph_req < 0 && c->graph_use < 0) {
+        // a trial tick: even trials run kernel by kernel
+        if (c->gtune_n >= psr_ctx::TUNE_N) return PSR_OK;
+        *trial = c->gtune_n;
+        if (!c->gtune_ev[0])
+            for (int k = 0; k < 2 * psr_ctx::TUNE_N; k++) PSR_HIP(c, hipEventCreate(&c->gtune_ev[k]));
+        if (!(*trial & 1)) {
+            PSR_HIP(c, hipEventRecord(c->gtune_ev[2 * *trial], c->stream));
+            return PSR_OK;
+        }
+    }
+    int atrial;
+    const bool binned = tick_binned(c, &atrial);
+    if ((rc = psr_sync_dev_ctx(c))) return rc;
+    if ((t.phases & PSR_PHASE_DEPOSIT) && (rc = psr_prepare_deposit(c, binned))) return rc;
+    if ((t.phases & PSR_PHASE_UPDATE) && (rc = psr_prepare_update(c))) return rc;
+    const std::vector<int64_t> key = graph_key(c, t, binned);
+    if (!c->graph_exec || key != c->graph_key) {
+        psr_graph_reset(c);
+        PSR_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = tick_body(c, t, false, false);
+        if (!rc) k_publish<<<1, 64, 0, c->stream>>>(&c->d_stats->flags, c->d_tick);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (!rc && e != hipSuccess) {
+            c->err = std::string("recording the tick graph: ") + hipGetErrorString(e);
+            rc = PSR_ERR_HIP;
+        }
+        if (!rc && hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            c->err = "instantiating the tick graph failed";
+            c->graph_exec = nullptr;
+            rc = PSR_ERR_HIP;
+        }
+        if (g) (void)hipGraphDestroy(g);
+        if (rc) return rc;
+        c->graph_key = key;
+    }
+    if (*trial >= 0) PSR_HIP(c, hipEventRecord(c->gtune_ev[2 * *trial], c->stream));
+    k_set_tick<<<1, 64, 0, c->stream>>>(c->d_tick, c->d_rays + t.ray_first, t.tick, t.tick_time, c->h_ring + t.slot);
+    PSR_HIP(c, hipGetLastError());
+    PSR_HIP(c, hipGraphLaunch(c->graph_exec, c->stream));
+    c->graph_launches++;
+    *done = true;
+    return PSR_OK;
 }
 
 int psr_resolve(psr_ctx *c) {
     // no tick pending: every trial tick was published, its events are done
-    if (c->pending.empty()) return tune_decide(c, false);
+    if (c->pending.empty()) {
+        const int rc = tune_decide(c, false);
+        return rc ? rc : gtune_decide(c, false);
+    }
     PSR_HIP(c, hipStreamSynchronize(c->stream));
     int rc0 = tune_decide(c, true);
+    if (!rc0) rc0 = gtune_decide(c, true);
     if (rc0) return rc0;
     std::deque<PendingTick> todo;
     todo.swap(c->pending);
@@ -705,7 +868,7 @@ int psr_resolve(psr_ctx *c) {
         if (rc) return rc;
     }
     PSR_HIP(c, hipStreamSynchronize(c->stream));
-    return PSR_OK;
+    return tune_decide(c, true);
 }
 
 // drop validated ticks from the front of the queue without blocking; an
@@ -734,16 +897,28 @@ extern "C" int psr_tick(psr_ctx *c, int64_t ray_first, int64_t nray, int64_t tic
         PSR_HIP(c, hipHostMalloc((void **)&c->h_ring, sizeof(int32_t) * PSR_RING, hipHostMallocCoherent));
         memset(c->h_ring, 0, sizeof(int32_t) * PSR_RING);
     }
+    if (!c->d_tick) PSR_HIP(c, hipMalloc(&c->d_tick, sizeof(DevTick)));
     if ((rc = poll_pending(c))) return rc;
     if ((rc = tune_decide(c, false))) return rc;
+    if ((rc = gtune_decide(c, false))) return rc;
     if (c->pending.size() >= PSR_RING - 1 && (rc = psr_resolve(c))) return rc;
     PendingTick t{ray_first, nray, tick, tick_time, phases, chem_mode, c->seq++ % PSR_RING};
     ((volatile int32_t *)c->h_ring)[t.slot] = 0;
     c->ev_mask = 0;
     PSR_EV(c, 4, 0);
-    if ((rc = tick_body(c, t, false))) return rc;
-    k_publish<<<1, 64, 0, c->stream>>>(&c->d_stats->flags, c->h_ring + t.slot);
-    PSR_HIP(c, hipGetLastError());
+    bool graphed = false;
+    int gtrial = -1;
+    if ((rc = tick_graph(c, t, &graphed, &gtrial))) return rc;
+    if (!graphed) {
+        if ((rc = tick_body(c, t, false))) return rc;
+        k_publish<<<1, 64, 0, c->stream>>>(&c->d_stats->flags, c->d_tick);
+        PSR_HIP(c, hipGetLastError());
+    }
+    if (gtrial >= 0) {
+        PSR_HIP(c, hipEventRecord(c->gtune_ev[2 * gtrial + 1], c->stream));
+        c->gtune_slot[gtrial] = (int32_t)t.slot;
+        c->gtune_n++;
+    }
     c->pending.push_back(t);
     PSR_EV(c, 4, 1);
     if (c->timing) {
@@ -771,18 +946,28 @@ extern "C" int psr_set_timing(psr_ctx *c, int enable) {
 
 extern "C" int psr_set_tuning(psr_ctx *c, int32_t param, int32_t value) {
     if (!c) return PSR_ERR_ARG;
-    if (param != PSR_TUNE_DEPOSIT_ACCUM) PSR_FAIL(c, PSR_ERR_ARG, "unknown tuning parameter");
-    if (value != PSR_ACCUM_AUTO && value != PSR_ACCUM_ATOMIC && value != PSR_ACCUM_BINNED)
-        PSR_FAIL(c, PSR_ERR_ARG, "unknown deposit accumulation");
-    c->accum_req = value;
-    psr_tune_reset(c);
+    if (param == PSR_TUNE_DEPOSIT_ACCUM) {
+        if (value != PSR_ACCUM_AUTO && value != PSR_ACCUM_ATOMIC && value != PSR_ACCUM_BINNED)
+            PSR_FAIL(c, PSR_ERR_ARG, "unknown deposit accumulation");
+        c->accum_req = value;
+        psr_tune_reset(c);
+    } else if (param == PSR_TUNE_TICK_GRAPH) {
+        if (value < -1 || value > 1) PSR_FAIL(c, PSR_ERR_ARG, "tick graph: -1 (timed trials), 0 or 1");
+        c->graph_req = value;
+        c->gtune_n = 0;
+        c->graph_use = -1;
+    } else {
+        PSR_FAIL(c, PSR_ERR_ARG, "unknown tuning parameter");
+    }
     return PSR_OK;
 }
 
 extern "C" int psr_get_tuning(psr_ctx *c, int32_t param, int32_t *value) {
     if (!c || !value) return PSR_ERR_ARG;
-    if (param != PSR_TUNE_DEPOSIT_ACCUM) PSR_FAIL(c, PSR_ERR_ARG, "unknown tuning parameter");
-    *value = c->accum_req != PSR_ACCUM_AUTO ? c->accum_req : c->accum_use;
+    if (param == PSR_TUNE_DEPOSIT_ACCUM) *value = c->accum_req != PSR_ACCUM_AUTO ? c->accum_req : c->accum_use;
+    else if (param == PSR_TUNE_TICK_GRAPH) *value = c->graph_req >= 0 ? c->graph_req : c->graph_use;
+    else if (param == PSR_TUNE_GRAPH_LAUNCHES) *value = (int32_t)std::min<int64_t>(c->graph_launches, INT32_MAX);
+    else PSR_FAIL(c, PSR_ERR_ARG, "unknown tuning parameter");
     return PSR_OK;
 }
 

@@ -306,12 +306,15 @@ def test_deposit_accumulation_is_tuned(ctx):
         assert_f32_sums(getattr(out, fld), nt * getattr(ref, fld).astype(np.float64), nt * ref.hits, fld)
 
 
-def test_overflowed_tick_is_replayed():
+@pytest.mark.parametrize("accum", [psr.PSR_ACCUM_AUTO, psr.PSR_ACCUM_ATOMIC])
+def test_overflowed_tick_is_replayed(accum):
     """ticks are launched without waiting for the trace's capacity check; a
     tick whose lists outgrow the capacity (2048 hits per ray at first) stops
     every later kernel and is replayed, with the ticks after it, at the next
-    call -- the result equals three sequential oracle ticks"""
-    print_case("overflow-replay")
+    call -- the result equals three sequential oracle ticks.  With a fixed
+    accumulation the ticks run as recorded graphs, including the overflowed
+    one."""
+    print_case("overflow-replay", accum)
     wl = make_workload(1, RATES, n=104, nray=16)
     box = wl.boxsize
     r1, r2, r3 = wl.rays(1), wl.rays(2), wl.rays(3)
@@ -324,6 +327,7 @@ def test_overflowed_tick_is_replayed():
     pool = np.concatenate([r1, r2, r3])
     orc = oracle_for(wl)
     with psr.Context(0) as c2:
+        c2.set_tuning(psr.PSR_TUNE_DEPOSIT_ACCUM, accum)
         load(c2, wl)
         c2.rays_upload(pool)
         for t, (a, rr) in enumerate(((0, r1), (len(r1), r2), (len(r1) + len(r2), r3)), start=1):
@@ -344,6 +348,45 @@ def test_overflowed_tick_is_replayed():
     assert s["total_deposit_count"] == total
     assert np.array_equal(out.lasthit, ref.lasthit)
     assert np.abs(out.lambdaH - ref.lambdaH).max() < 1e-6
+
+
+@pytest.mark.parametrize("cfg,accum", [(2, psr.PSR_ACCUM_ATOMIC), (2, psr.PSR_ACCUM_BINNED),
+                                       (3, psr.PSR_ACCUM_BINNED)])
+def test_tick_graph_matches_kernel_by_kernel(cfg, accum):
+    """ten ticks recorded once as a hipGraph and replayed with each tick's
+    record against the same ten ticks launched kernel by kernel, from the
+    same state.  Two runs differ only through the f32 summation order of the
+    deposits, which moves fractions in the last bits and so, over ten ticks,
+    which hit ends a few rays at the 1e-10 cut-off: full hit counts are
+    equal, per-particle hits and last hits nearly everywhere, fractions and
+    energies to rounding."""
+    print_case("tick-graph", cfg, accum)
+    wl = make_workload(cfg, RATES, n=24 if cfg == 2 else 16, nray=256)
+    ticks = [wl.rays(t) for t in range(1, 11)]
+    pool = np.concatenate(ticks)
+    firsts = np.concatenate([[0], np.cumsum([len(r) for r in ticks])])
+    out = {}
+    for graph in (0, 1):
+        with psr.Context(0) as c2:
+            c2.set_tuning(psr.PSR_TUNE_DEPOSIT_ACCUM, accum)
+            c2.set_tuning(psr.PSR_TUNE_TICK_GRAPH, graph)
+            load(c2, wl)
+            c2.rays_upload(pool)
+            for t in range(1, 11):
+                c2.tick(t, wl.tick_time, ray_first=int(firsts[t - 1]), nray=len(ticks[t - 1]))
+            ps = wl.ps.copy()
+            c2.state_download(ps)
+            out[graph] = (ps, c2.stats(), c2.tuning(psr.PSR_TUNE_GRAPH_LAUNCHES))
+    (a, sa, ga), (b, sb, gb) = out[0], out[1]
+    assert ga == 0 and gb >= 8                     # the first tick(s) may size the hit slots
+    assert sa["total_deposit_count"] == sb["total_deposit_count"]
+    assert np.mean(a.hits != b.hits) < 1e-3 and np.mean(a.lasthit != b.lasthit) < 1e-3
+    for f in ("lambdaH", "lambdaHeI", "lambdaHeII"):
+        assert np.abs(getattr(a, f) - getattr(b, f)).max() < 1e-5, f
+    # ie: an adaptive step decision can flip on a last-bit difference (see
+    # test_full_tick_parity)
+    dr = np.abs(a.ie / b.ie - 1)
+    assert np.median(dr) < 1e-6 and dr.max() < 1e-2
 
 
 @pytest.mark.parametrize("cfg,n,nray", [(1, 32, 32), (2, 24, 256), (3, 16, 128)])
