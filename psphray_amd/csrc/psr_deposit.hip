@@ -442,8 +442,12 @@ __device__ __forceinline__ void rec_commit(RecCarry<HE> &c, const DevBins &B, co
     }
 }
 
-template <bool HE>
-__global__ __launch_bounds__(FTHREADS) void k_deposit(const DevTick *__restrict__ tk, int64_t nray, DevHits H,
+// BIN: records into the particle-range bins (else f32 atomics into the
+// particles); the atomic variant carries no reservation state.  Hydrogen
+// only: at most 128 VGPRs (a few spills, in the apply wave), so that 5
+// workgroups share a CU instead of 4 -- 10% faster on 128^3.
+template <bool HE, bool BIN>
+__global__ __launch_bounds__(FTHREADS, HE ? 2 : 4) void k_deposit(const DevTick *__restrict__ tk, int64_t nray, DevHits H,
                                                       const DevCtx *__restrict__ C, DevStats *__restrict__ st0,
                                                       double *__restrict__ ray_len, double *__restrict__ ray_tm,
                                                       DevBins Bn, const int32_t *halt) {
@@ -462,11 +466,13 @@ __global__ __launch_bounds__(FTHREADS) void k_deposit(const DevTick *__restrict_
     double lost = 0.0;
     constexpr int HPL = FSEG / 64;  // hits per lane and segment
     RecCarry<HE> carry;             // apply wave: records awaiting their room
+    if constexpr (BIN) {
 #pragma unroll
-    for (int h = 0; h < HPL; h++) {
-        carry.has[h] = false;
-        carry.leader[h] = lane;
-        carry.old[h] = 0;
+        for (int h = 0; h < HPL; h++) {
+            carry.has[h] = false;
+            carry.leader[h] = lane;
+            carry.old[h] = 0;
+        }
     }
     for (int64_t r = blockIdx.x; r < nray; r += gridDim.x) {
         const int n = H.count[r];
@@ -629,7 +635,7 @@ __global__ __launch_bounds__(FTHREADS) void k_deposit(const DevTick *__restrict_
                 const int stop = L.stop_seg;
                 if (stop >= 0 ? i > stop + 2 : i > nseg + 1) break;
                 // last iteration's records: their room has arrived by now
-                if (Bn.rec) rec_commit<HE>(carry, Bn, S);
+                if constexpr (BIN) rec_commit<HE>(carry, Bn, S);
                 const int s = i - 2;
                 if (s >= 0 && s < nseg && (stop < 0 || s <= stop)) {
                     const int ring = s % 3;
@@ -664,7 +670,7 @@ __global__ __launch_bounds__(FTHREADS) void k_deposit(const DevTick *__restrict_
                             for (int q = g0; q < jj; q++) pre = walk_step<HE>(pre, L.rec[ring][q]);
                             rr = apply_values<HE>(pre, v, L.aux[ring][jj], T, P, kc, acc);
                         }
-                        if (Bn.rec) rec_reserve<HE>(carry, h, has, rr, Bn, lane);
+                        if constexpr (BIN) rec_reserve<HE>(carry, h, has, rr, Bn, lane);
                         else if (has) rec_atomic<HE>(S, rr);
                     }
                 }
@@ -682,7 +688,8 @@ __global__ __launch_bounds__(FTHREADS) void k_deposit(const DevTick *__restrict_
         }
         __syncthreads();
     }
-    if (wv == 2 && Bn.rec) rec_commit<HE>(carry, Bn, S);  // the last segment's records
+    if constexpr (BIN)
+        if (wv == 2) rec_commit<HE>(carry, Bn, S);  // the last segment's records
     // counters: thread 0 holds the ray totals, wave 2 the deposit sums
     DevStats *st = stat_slot(st0);
     if (threadIdx.x == 0 && cnt) {
@@ -876,16 +883,20 @@ int psr_launch_deposit(psr_ctx *c, int64_t, int64_t nray, bool binned) {
     const int32_t *halt = &c->d_stats->flags;
     const int64_t nbins = binned ? c->bin_n : 0;
     uint32_t *off = binned ? c->d_bin_off : nullptr, *fill = binned ? off + nbins + 1 : nullptr;
-    // a null record buffer makes the deposit add straight to the particles
     const DevBins Bn{binned ? c->d_bin_rec : nullptr, off, fill};
     const unsigned gbin = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nbins, 65535 * 4));
     PSR_EV(c, 2, 0);
-    if (he)
-        k_deposit<true><<<grid, FTHREADS, 0, c->stream>>>(rays, nray, c->H, c->d_ctx, c->d_stats, c->d_ray_len,
-                                                          c->d_ray_tm, Bn, halt);
-    else
-        k_deposit<false><<<grid, FTHREADS, 0, c->stream>>>(rays, nray, c->H, c->d_ctx, c->d_stats, c->d_ray_len,
-                                                           c->d_ray_tm, Bn, halt);
+#define DEP_K(HE_, BIN_)                                                                                \
+    k_deposit<HE_, BIN_><<<grid, FTHREADS, 0, c->stream>>>(rays, nray, c->H, c->d_ctx, c->d_stats, c->d_ray_len, \
+                                                           c->d_ray_tm, Bn, halt)
+    if (he) {
+        if (binned) DEP_K(true, true);
+        else DEP_K(true, false);
+    } else {
+        if (binned) DEP_K(false, true);
+        else DEP_K(false, false);
+    }
+#undef DEP_K
     PSR_HIP(c, hipGetLastError());
     if (binned) {
         if (he) k_bin_reduce<true><<<gbin, RED_THREADS, 0, c->stream>>>(Bn, nbins, c->npar, c->d_ctx, halt);
