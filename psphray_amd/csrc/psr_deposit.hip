@@ -443,11 +443,13 @@ __device__ __forceinline__ void rec_commit(RecCarry<HE> &c, const DevBins &B, co
 }
 
 // BIN: records into the particle-range bins (else f32 atomics into the
-// particles); the atomic variant carries no reservation state.  Hydrogen
-// only: at most 128 VGPRs (a few spills, in the apply wave), so that 5
-// workgroups share a CU instead of 4 -- 10% faster on 128^3.
+// particles); the atomic variant carries no reservation state.  Register
+// caps trade a few spills (in the apply wave, off the walk's critical path)
+// for resident workgroups: hydrogen only 128 VGPRs, 5 workgroups per CU
+// instead of 4 (10% faster on 128^3); H + He 168 VGPRs, 4 instead of 2 (the
+// LDS ring's limit; 1.6x faster on 256^3).
 template <bool HE, bool BIN>
-__global__ __launch_bounds__(FTHREADS, HE ? 2 : 4) void k_deposit(const DevTick *__restrict__ tk, int64_t nray, DevHits H,
+__global__ __launch_bounds__(FTHREADS, HE ? 3 : 4) void k_deposit(const DevTick *__restrict__ tk, int64_t nray, DevHits H,
                                                       const DevCtx *__restrict__ C, DevStats *__restrict__ st0,
                                                       double *__restrict__ ray_len, double *__restrict__ ray_tm,
                                                       DevBins Bn, const int32_t *halt) {
