@@ -1028,6 +1028,133 @@ int or_chem_rhs(double t, double *x, double *dxdt, void *data) {
     return 0;
 }
 
+/* ---- the right-hand side with its Jacobian by forward-mode dual numbers:
+ * values with the operations and order of or_chem_rhs, derivatives with
+ * respect to the 7 variables; the rates' derivative in log10 T is the
+ * interpolation segment's slope (the same algorithm as the device). */
+typedef struct { double v, d[7]; } dual7;
+static dual7 dc(double v) { dual7 r; r.v = v; for (int k = 0; k < 7; k++) r.d[k] = 0.0; return r; }
+static dual7 dadd(dual7 a, dual7 b) { dual7 r; r.v = a.v + b.v; for (int k = 0; k < 7; k++) r.d[k] = a.d[k] + b.d[k]; return r; }
+static dual7 dsub(dual7 a, dual7 b) { dual7 r; r.v = a.v - b.v; for (int k = 0; k < 7; k++) r.d[k] = a.d[k] - b.d[k]; return r; }
+static dual7 dmul(dual7 a, dual7 b) {
+    dual7 r;
+    r.v = a.v * b.v;
+    for (int k = 0; k < 7; k++) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+    return r;
+}
+static dual7 dneg(dual7 a) { dual7 r; r.v = -a.v; for (int k = 0; k < 7; k++) r.d[k] = -a.d[k]; return r; }
+static dual7 cadd(double a, dual7 b) { dual7 r = b; r.v = a + b.v; return r; }    /* a + b */
+static dual7 addc(dual7 a, double b) { dual7 r = a; r.v = a.v + b; return r; }    /* a + b */
+static dual7 csub(double a, dual7 b) { dual7 r; r.v = a - b.v; for (int k = 0; k < 7; k++) r.d[k] = -b.d[k]; return r; }
+static dual7 cmul(double a, dual7 b) { dual7 r; r.v = a * b.v; for (int k = 0; k < 7; k++) r.d[k] = a * b.d[k]; return r; }
+static dual7 dfdim(dual7 a, dual7 b) { return a.v > b.v ? dsub(a, b) : dc(fdim_(a.v, b.v)); }
+
+/* tab_get and its slope in value */
+static double tab_get_slope(const or_tab *t, int id, double value, double *slope) {
+    ssize_t index = (ssize_t)((value - t->min) * t->step_inv);
+    *slope = 0.0;
+    if (index < 0) return t->data[id][0];
+    if (index >= t->nrows - 1) return t->data[id][t->nrows - 1];
+    double left = t->data[0][index];
+    double right = t->data[0][index + 1];
+    double leftwt = (right - value);
+    double rightwt = (value - left);
+    *slope = (t->data[id][index + 1] - t->data[id][index]) * t->step_inv;
+    return (leftwt * t->data[id][index] + rightwt * t->data[id][index + 1]) * t->step_inv;
+}
+
+/* rate col at log10 T times nh, with its derivatives through dual L */
+static dual7 drate(const or_ctx *c, int col, dual7 L, double nh, int iso) {
+    double sl;
+    dual7 r;
+    r.v = tab_get_slope(&c->ar, c->arcol[col], L.v, &sl) * nh;
+    for (int k = 0; k < 7; k++) r.d[k] = iso ? 0.0 : sl * nh * L.d[k];
+    return r;
+}
+
+/* f = or_chem_rhs(x) and jac[i][j] = d f_i / d x_j (7 x 7, row-major) */
+int or_chem_jac(const or_ctx *c, const or_step *step, const double *x, double *f, double *jac) {
+    const or_flags *fl = &c->f;
+    const or_units *u = &c->u;
+    dual7 X[7];
+    for (int q = 0; q < 7; q++) { X[q] = dc(x[q]); X[q].d[q] = 1.0; }
+    const dual7 xHI = X[0];
+    const dual7 xHII = csub(1.0, X[0]);
+    const dual7 xHeI = X[1];
+    const dual7 xHeII = X[2];
+    const dual7 xHeIII = dsub(csub(1.0, X[2]), X[1]);
+    const dual7 ye = addc(dadd(csub(1.0, X[0]), cmul(u->C_HEMF / u->C_HMF * 0.25,
+                                                     dadd(X[2], cmul(2, dsub(csub(1.0, X[2]), X[1]))))),
+                          step->yeMET);
+    const double nH = step->nH;
+    const int iso = fl->isothermal != 0;
+    dual7 L = dc(0.0);
+    if (iso) {
+        L.v = log10(step->T);
+    } else {
+        const double den = u->C_HEMF * 0.25 + u->C_HMF + u->C_HMF * ye.v;
+        L.v = log10(or_ieye2T(c, x[6], ye.v));
+        const double dye = -u->C_HMF * 0.43429448190325182765 / den;
+        for (int k = 0; k < 7; k++) L.d[k] = dye * ye.d[k];
+        L.d[6] += 0.43429448190325182765 / x[6];
+    }
+    const dual7 gamma_HI = drate(c, AR_HI_CI, L, nH, iso);
+    const dual7 gamma_HeI = drate(c, AR_HEI_CI, L, nH, iso);
+    const dual7 gamma_HeII = drate(c, AR_HEII_CI, L, nH, iso);
+    dual7 g[7];
+    /* -a * ye * b as ((-a) * ye) * b */
+#define NAB(a, b) dmul(dmul(dneg(a), ye), (b))
+#define AB(a, b) dmul(dmul((a), ye), (b))
+    if (!fl->on_the_spot) {
+        const dual7 aHA = drate(c, AR_HII_RC_A, L, nH, iso), aHB = drate(c, AR_HII_RC_B, L, nH, iso);
+        g[0] = cadd(-step->yGdepHI / step->time, dadd(NAB(gamma_HI, xHI), AB(aHA, xHII)));
+        g[3] = AB(dfdim(aHA, aHB), xHII);
+        if (fl->h_only) {
+            g[1] = g[2] = g[4] = g[5] = dc(0.0);
+        } else {
+            const dual7 a2A = drate(c, AR_HEII_RC_A, L, nH, iso), a2B = drate(c, AR_HEII_RC_B, L, nH, iso);
+            const dual7 a3A = drate(c, AR_HEIII_RC_A, L, nH, iso), a3B = drate(c, AR_HEIII_RC_B, L, nH, iso);
+            g[1] = cadd(-step->yGdepHeI / step->time, dadd(NAB(gamma_HeI, xHeI), AB(a2A, xHeII)));
+            g[2] = cadd(-step->yGdepHeII / step->time,
+                        dadd(dsub(NAB(gamma_HeII, xHeII), AB(a2A, xHeII)), AB(a3A, xHeIII)));
+            g[4] = AB(dfdim(a2A, a2B), xHeII);
+            g[5] = AB(dfdim(a3A, a3B), xHeIII);
+        }
+    } else {
+        const dual7 aHB = drate(c, AR_HII_RC_B, L, nH, iso);
+        g[0] = cadd(-step->yGdepHI / step->time, dadd(NAB(gamma_HI, xHI), AB(aHB, xHII)));
+        if (fl->h_only) {
+            g[1] = g[2] = dc(0.0);
+        } else {
+            const dual7 a2B = drate(c, AR_HEII_RC_B, L, nH, iso), a3B = drate(c, AR_HEIII_RC_B, L, nH, iso);
+            g[1] = cadd(-step->yGdepHeI / step->time, dadd(NAB(gamma_HeI, xHI), AB(a2B, xHeII)));
+            g[2] = cadd(-step->yGdepHeII / step->time,
+                        dadd(dsub(NAB(gamma_HeII, xHI), AB(a2B, xHeII)), AB(a3B, xHeIII)));
+        }
+        g[3] = g[4] = g[5] = dc(0.0);
+    }
+    if (fl->adiabatic | fl->isothermal) {
+        g[6] = dc(0.0);
+    } else {
+        const dual7 zeta = drate(c, AR_HI_CIC, L, nH, iso), psi = drate(c, AR_HI_CEC, L, nH, iso);
+        const dual7 eta = drate(c, fl->on_the_spot ? AR_HII_RCC_B : AR_HII_RCC_A, L, nH, iso);
+        const dual7 beta = drate(c, AR_E_BREMC, L, nH, iso);
+        const dual7 chi = drate(c, AR_E_COMPC, L, 1.0, iso);
+        const dual7 Lc = cmul(u->C_H_PER_MASS,
+                              dadd(dadd(dadd(AB(dadd(zeta, psi), xHI), AB(eta, xHII)), AB(beta, xHII)),
+                                   dmul(chi, ye)));
+        g[6] = csub(step->heat / step->time, Lc);
+    }
+#undef NAB
+#undef AB
+    for (int i = 0; i < 7; i++) {
+        f[i] = g[i].v * step->time;
+        if (jac)
+            for (int j = 0; j < 7; j++) jac[i * 7 + j] = g[i].d[j] * step->time;
+    }
+    return 0;
+}
+
 /* variables the flags leave evolving (the others have zero derivative) */
 static int chem_active(const or_flags *f, int act[7]) {
     int n = 0;
@@ -1121,7 +1248,7 @@ double or_riccati_tick(double x0, double g, double m, double G, double A) {
 /* Two-stage L-stable Rosenbrock (Verwer, Spee, Blom & Hundsdorfer 1999):
  *   (I - g h J) k1 = F(y),  (I - g h J) k2 = F(y + h k1) - 2 k1,
  *   y' = y + 1.5 h k1 + 0.5 h k2,  err = 0.5 h (k1 + k2).
- * Finite-difference Jacobian over the evolving variables, as LSODA does.
+ * Exact Jacobian over the evolving variables (or_chem_jac, dual numbers).
  * Tolerances of src/solve.c:151-152 unless overridden. */
 int32_t or_step_evolve(const or_ctx *c, or_step *s, double rtol, const double *atol7) {
     static const double atol_def[7] = {1e-7, 1e-7, 1e-7, 1e-7, 1e-7, 1e-7, 1.};
@@ -1134,7 +1261,7 @@ int32_t or_step_evolve(const or_ctx *c, or_step *s, double rtol, const double *a
                    s->yGrecHeIII, s->ie};
     double t = 0.0, h = 1.0;
     int nsteps = 0, rejected = 0;
-    double f0[7], f1[7], fp[7], xt[7], k1[7], k2[7], J[49], M[49];
+    double f0[7], f1[7], xt[7], k1[7], k2[7], J[49], M[49];
     int piv[7];
     if (c->f.isothermal && c->f.h_only && c->f.on_the_spot) {
         const double logT = log10(s->T);
@@ -1146,16 +1273,10 @@ int32_t or_step_evolve(const or_ctx *c, or_step *s, double rtol, const double *a
     }
     while (t < 1.0 && nsteps < ROS2_MAXSTEPS) {
         if (h > 1.0 - t) h = 1.0 - t;
-        or_chem_rhs(t, x, f0, &cd);
-        for (int jj = 0; jj < n; jj++) {
-            const int j = act[jj];
-            memcpy(xt, x, sizeof(xt));
-            const double dl = 1.4901161193847656e-08 * fmax(fabs(x[j]), 1e-8);
-            xt[j] = x[j] + dl;
-            const double dlr = xt[j] - x[j];
-            or_chem_rhs(t, xt, fp, &cd);
-            for (int ii = 0; ii < n; ii++) J[ii * n + jj] = (fp[act[ii]] - f0[act[ii]]) / dlr;
-        }
+        double J7[49];
+        or_chem_jac(c, s, x, f0, J7);
+        for (int ii = 0; ii < n; ii++)
+            for (int jj = 0; jj < n; jj++) J[ii * n + jj] = J7[act[ii] * 7 + act[jj]];
         for (;;) {
             nsteps++;
             const double gh = ROS2_GAMMA * h;

@@ -190,6 +190,9 @@ typedef struct {
     const or_step *step;
 } or_chem_data;
 int or_chem_rhs(double t, double *x, double *dxdt, void *data);
+/* f = or_chem_rhs(x) and jac = df/dx (7 x 7 row-major; may be NULL) by dual
+ * numbers, the integrator's Jacobian */
+int or_chem_jac(const or_ctx *c, const or_step *step, const double *x, double *f, double *jac);
 
 /* exact one-step solution of the isothermal hydrogen-only on-the-spot rate
  * equation dx/dtau = -g + (1 + m - x)(A - (A + G) x) over tau in [0, 1] */

@@ -86,6 +86,7 @@ _SIG = {
     "or_update": (_I32, [_VP, C.POINTER(OrPsys), _VP, _I64, _I64, _D, C.POINTER(OrStats), _I32]),
     "or_tick": (_I32, [_VP, _VP, C.POINTER(OrPsys), _VP, _I64, _I64, _D, C.POINTER(OrStats), _I32]),
     "or_chem_rhs": (C.c_int, [_D, _VP, _VP, _VP]),
+    "or_chem_jac": (C.c_int, [_VP, _VP, _VP, _VP, _VP]),
     "or_step_evolve": (_I32, [_VP, _VP, _D, _VP]),
     "or_riccati_tick": (_D, [_D, _D, _D, _D, _D]),
     "or_step_evolve_many": (_I32, [_VP, _VP, _I64, _D, _I32]),
@@ -212,6 +213,24 @@ class Oracle:
         s = np.ascontiguousarray(steps, STEP_DTYPE).copy()
         bad = self.L.or_step_evolve_many(self.h, p(s), len(s), rtol, nthreads)
         return s, bad
+
+    def chem_rhs(self, step, x):
+        """or_chem_rhs of one step record at state x[7]"""
+        st = np.ascontiguousarray(step, STEP_DTYPE).reshape(1).copy()
+        data = (C.c_void_p * 2)(self.h, st.ctypes.data)
+        xx = np.array(x, np.float64)
+        f = np.zeros(7)
+        self.L.or_chem_rhs(0.0, p(xx), p(f), C.cast(data, C.c_void_p))
+        return f
+
+    def chem_jac(self, step, x):
+        """(f, J) of or_chem_jac"""
+        st = np.ascontiguousarray(step, STEP_DTYPE).reshape(1).copy()
+        xx = np.array(x, np.float64)
+        f = np.zeros(7)
+        J = np.zeros((7, 7))
+        self.L.or_chem_jac(self.h, p(st), p(xx), p(f), p(J))
+        return f, J
 
     def lsoda_evolve(self, steps):
         """the reference's LSODA (oracle/_ref) driven like step_evolve_lsoda

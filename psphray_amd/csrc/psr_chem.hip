@@ -8,8 +8,10 @@
 // NV = 4 for H+He with temperature.  The integrator is a two-stage L-stable
 // Rosenbrock method (Verwer et al. 1999) with an embedded first-order error
 // estimate and the reference tolerances (rtol 1e-4, atol 1e-7 / 1 for ie);
-// the Jacobian is a forward finite difference, as LSODA's.  The x_HI
-// variable and x_HII = 1 - x_HI are both kept in FP64.
+// the Jacobian is exact (dual numbers through the right-hand side and the
+// rate table's interpolation), one table pass per step where finite
+// differences took N + 1.  The x_HI variable and x_HII = 1 - x_HI are both
+// kept in FP64.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -147,6 +149,255 @@ __device__ void rhs(const double x[7], double f[7], const StepIn &s, const DevTa
     }
 #pragma unroll
     for (int i = 0; i < 7; i++) f[i] *= s.time;
+}
+
+// ---- right-hand side with its Jacobian, by forward-mode dual numbers ----
+// Every quantity carries its value and its derivatives with respect to the
+// N evolving variables.  The rates are the table's piecewise-linear
+// interpolation in log10 T, so d rate / d log10 T is the segment's slope,
+// and log10 T depends on ie and (through ye) on the fractions.  The value
+// parts are computed with the operations and order of rhs(), so f is the
+// same; the Jacobian replaces N finite-difference evaluations (each one a
+// full pass over the rate table) with one.
+template <int N>
+struct Dn {
+    double v;
+    double d[N];
+};
+template <int N>
+__device__ __forceinline__ Dn<N> dn_const(double v) {
+    Dn<N> r;
+    r.v = v;
+#pragma unroll
+    for (int k = 0; k < N; k++) r.d[k] = 0.0;
+    return r;
+}
+template <int N>
+__device__ __forceinline__ Dn<N> operator+(const Dn<N> &a, const Dn<N> &b) {
+    Dn<N> r;
+    r.v = a.v + b.v;
+#pragma unroll
+    for (int k = 0; k < N; k++) r.d[k] = a.d[k] + b.d[k];
+    return r;
+}
+template <int N>
+__device__ __forceinline__ Dn<N> operator-(const Dn<N> &a, const Dn<N> &b) {
+    Dn<N> r;
+    r.v = a.v - b.v;
+#pragma unroll
+    for (int k = 0; k < N; k++) r.d[k] = a.d[k] - b.d[k];
+    return r;
+}
+template <int N>
+__device__ __forceinline__ Dn<N> operator*(const Dn<N> &a, const Dn<N> &b) {
+    Dn<N> r;
+    r.v = a.v * b.v;
+#pragma unroll
+    for (int k = 0; k < N; k++) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+    return r;
+}
+template <int N>
+__device__ __forceinline__ Dn<N> operator-(const Dn<N> &a) {
+    Dn<N> r;
+    r.v = -a.v;
+#pragma unroll
+    for (int k = 0; k < N; k++) r.d[k] = -a.d[k];
+    return r;
+}
+template <int N>
+__device__ __forceinline__ Dn<N> operator+(double a, const Dn<N> &b) {
+    Dn<N> r = b;
+    r.v = a + b.v;
+    return r;
+}
+template <int N>
+__device__ __forceinline__ Dn<N> operator+(const Dn<N> &a, double b) {
+    Dn<N> r = a;
+    r.v = a.v + b;
+    return r;
+}
+template <int N>
+__device__ __forceinline__ Dn<N> operator-(double a, const Dn<N> &b) {
+    Dn<N> r;
+    r.v = a - b.v;
+#pragma unroll
+    for (int k = 0; k < N; k++) r.d[k] = -b.d[k];
+    return r;
+}
+template <int N>
+__device__ __forceinline__ Dn<N> operator-(const Dn<N> &a, double b) {
+    Dn<N> r = a;
+    r.v = a.v - b;
+    return r;
+}
+template <int N>
+__device__ __forceinline__ Dn<N> operator*(double a, const Dn<N> &b) {
+    Dn<N> r;
+    r.v = a * b.v;
+#pragma unroll
+    for (int k = 0; k < N; k++) r.d[k] = a * b.d[k];
+    return r;
+}
+template <int N>
+__device__ __forceinline__ Dn<N> operator*(const Dn<N> &a, double b) {
+    Dn<N> r;
+    r.v = a.v * b;
+#pragma unroll
+    for (int k = 0; k < N; k++) r.d[k] = a.d[k] * b;
+    return r;
+}
+// fdim(a, b) = a - b where a > b, else 0 (and so its derivatives)
+template <int N>
+__device__ __forceinline__ Dn<N> dn_fdim(const Dn<N> &a, const Dn<N> &b) {
+    return a.v > b.v ? a - b : dn_const<N>(fdim(a.v, b.v));
+}
+
+// the rates at log10 T (as rates_at) and d rate / d log10 T
+template <bool OTS>
+__device__ __forceinline__ void rates_slopes_at(const DevTables &T, double logT, double nH, Rates &R, Rates &S) {
+    const int n = T.ar_nrows;
+    const long long idx = (long long)((logT - T.ar_min) * T.ar_step_inv);
+    double lw = 0, rw = 0;
+    int i0 = 0;
+    int mode;  // 0 clamp low, 1 interior, 2 clamp high
+    if (idx < 0) {
+        mode = 0;
+    } else if (idx >= n - 1) {
+        mode = 2;
+        i0 = n - 1;
+    } else {
+        mode = 1;
+        i0 = (int)idx;
+        lw = T.ar[i0 + 1] - logT;
+        rw = logT - T.ar[i0];
+    }
+    auto col = [&](int k, double &sl) -> double {
+        const double *c = T.ar + (size_t)(k + 1) * n;
+        if (mode != 1) {
+            sl = 0.0;
+            return c[i0];
+        }
+        sl = (c[i0 + 1] - c[i0]) * T.ar_step_inv;
+        return (lw * c[i0] + rw * c[i0 + 1]) * T.ar_step_inv;
+    };
+    double sl;
+    R.gamma_HI = col(AR_HI_CI, sl) * nH;
+    S.gamma_HI = sl * nH;
+    R.alpha_HII_A = col(AR_HII_RC_A, sl) * nH;
+    S.alpha_HII_A = sl * nH;
+    R.alpha_HII_B = col(AR_HII_RC_B, sl) * nH;
+    S.alpha_HII_B = sl * nH;
+    R.gamma_HeI = col(AR_HEI_CI, sl) * nH;
+    S.gamma_HeI = sl * nH;
+    R.gamma_HeII = col(AR_HEII_CI, sl) * nH;
+    S.gamma_HeII = sl * nH;
+    R.alpha_HeII_A = col(AR_HEII_RC_A, sl) * nH;
+    S.alpha_HeII_A = sl * nH;
+    R.alpha_HeII_B = col(AR_HEII_RC_B, sl) * nH;
+    S.alpha_HeII_B = sl * nH;
+    R.alpha_HeIII_A = col(AR_HEIII_RC_A, sl) * nH;
+    S.alpha_HeIII_A = sl * nH;
+    R.alpha_HeIII_B = col(AR_HEIII_RC_B, sl) * nH;
+    S.alpha_HeIII_B = sl * nH;
+    R.eta_HII = (OTS ? col(AR_HII_RCC_B, sl) : col(AR_HII_RCC_A, sl)) * nH;
+    S.eta_HII = sl * nH;
+    R.psi_HI = col(AR_HI_CEC, sl) * nH;
+    S.psi_HI = sl * nH;
+    R.zeta_HI = col(AR_HI_CIC, sl) * nH;
+    S.zeta_HI = sl * nH;
+    R.beta = col(AR_E_BREMC, sl) * nH;
+    S.beta = sl * nH;
+    R.chi = col(AR_E_COMPC, sl);
+    S.chi = sl;
+}
+
+// f = rhs(x) and J[i][j] = d f_act(i) / d x_act(j) over the evolving variables
+template <bool ISO, bool ADI, bool OTS, bool HONLY>
+__device__ void rhs_jac(const double x[7], double f[7], double *J, const StepIn &s, const DevTables &T,
+                        const DevPhys &P, const Rates &Riso) {
+    using SY = Sys<ISO, ADI, OTS, HONLY>;
+    constexpr int N = SY::NV;
+    using D = Dn<N>;
+    D X[7];
+#pragma unroll
+    for (int q = 0; q < 7; q++) X[q] = dn_const<N>(x[q]);
+#pragma unroll
+    for (int jj = 0; jj < N; jj++) X[SY::act(jj)].d[jj] = 1.0;
+    const D xHI = X[0];
+    const D xHII = 1.0 - X[0];
+    const D xHeI = X[1];
+    const D xHeII = X[2];
+    const D xHeIII = 1.0 - X[2] - X[1];
+    const D ye = (1.0 - X[0]) + P.C_HEMF / P.C_HMF * 0.25 * (X[2] + 2 * (1.0 - X[2] - X[1])) + s.yeMET;
+    Rates R, Sl;
+    D L = dn_const<N>(0.0);  // log10 T
+    if (ISO) {
+        R = Riso;
+    } else {
+        const double den = P.C_HEMF * 0.25 + P.C_HMF + P.C_HMF * ye.v;
+        const double Tk = P.U_MPROTON_OVER_C_BOLTZMANN * x[6] / den * 2.0 / 3.0;
+        L.v = log10(Tk);
+        rates_slopes_at<OTS>(T, L.v, s.nH, R, Sl);
+        // log10 T = log10 ie - log10 den + const
+        const double dye = -P.C_HMF * 0.43429448190325182765 / den;
+#pragma unroll
+        for (int jj = 0; jj < N; jj++) {
+            L.d[jj] = dye * ye.d[jj];
+            if (SY::act(jj) == 6) L.d[jj] += 0.43429448190325182765 / x[6];
+        }
+    }
+    auto rate = [&](double r, double sl) -> D {
+        D o;
+        o.v = r;
+#pragma unroll
+        for (int jj = 0; jj < N; jj++) o.d[jj] = ISO ? 0.0 : sl * L.d[jj];
+        return o;
+    };
+    const D gamma_HI = rate(R.gamma_HI, Sl.gamma_HI);
+    const D gamma_HeI = rate(R.gamma_HeI, Sl.gamma_HeI);
+    const D gamma_HeII = rate(R.gamma_HeII, Sl.gamma_HeII);
+    D g[7];
+    if (!OTS) {
+        const D aHA = rate(R.alpha_HII_A, Sl.alpha_HII_A), aHB = rate(R.alpha_HII_B, Sl.alpha_HII_B);
+        g[0] = -s.yGdepHI / s.time + (-gamma_HI * ye * xHI + aHA * ye * xHII);
+        g[3] = dn_fdim(aHA, aHB) * ye * xHII;
+        if (HONLY) {
+            g[1] = g[2] = g[4] = g[5] = dn_const<N>(0.0);
+        } else {
+            const D a2A = rate(R.alpha_HeII_A, Sl.alpha_HeII_A), a2B = rate(R.alpha_HeII_B, Sl.alpha_HeII_B);
+            const D a3A = rate(R.alpha_HeIII_A, Sl.alpha_HeIII_A), a3B = rate(R.alpha_HeIII_B, Sl.alpha_HeIII_B);
+            g[1] = -s.yGdepHeI / s.time + (-gamma_HeI * ye * xHeI + a2A * ye * xHeII);
+            g[2] = -s.yGdepHeII / s.time + (-gamma_HeII * ye * xHeII - a2A * ye * xHeII + a3A * ye * xHeIII);
+            g[4] = dn_fdim(a2A, a2B) * ye * xHeII;
+            g[5] = dn_fdim(a3A, a3B) * ye * xHeIII;
+        }
+    } else {
+        const D aHB = rate(R.alpha_HII_B, Sl.alpha_HII_B);
+        g[0] = -s.yGdepHI / s.time + (-gamma_HI * ye * xHI + aHB * ye * xHII);
+        if (HONLY) {
+            g[1] = g[2] = dn_const<N>(0.0);
+        } else {
+            const D a2B = rate(R.alpha_HeII_B, Sl.alpha_HeII_B), a3B = rate(R.alpha_HeIII_B, Sl.alpha_HeIII_B);
+            // x_HI in the He lines, as src/solve.c:103-104
+            g[1] = -s.yGdepHeI / s.time + (-gamma_HeI * ye * xHI + a2B * ye * xHeII);
+            g[2] = -s.yGdepHeII / s.time + (-gamma_HeII * ye * xHI - a2B * ye * xHeII + a3B * ye * xHeIII);
+        }
+        g[3] = g[4] = g[5] = dn_const<N>(0.0);
+    }
+    if (ADI || ISO) {
+        g[6] = dn_const<N>(0.0);
+    } else {
+        const D zeta = rate(R.zeta_HI, Sl.zeta_HI), psi = rate(R.psi_HI, Sl.psi_HI);
+        const D eta = rate(R.eta_HII, Sl.eta_HII), beta = rate(R.beta, Sl.beta), chi = rate(R.chi, Sl.chi);
+        const D Lc = P.C_H_PER_MASS * ((zeta + psi) * ye * xHI + (eta * ye * xHII) + beta * ye * xHII + chi * ye);
+        g[6] = s.heat / s.time - Lc;
+    }
+#pragma unroll
+    for (int i = 0; i < 7; i++) f[i] = g[i].v * s.time;
+#pragma unroll
+    for (int ii = 0; ii < N; ii++)
+#pragma unroll
+        for (int jj = 0; jj < N; jj++) J[ii * N + jj] = g[SY::act(ii)].d[jj] * s.time;
 }
 
 #define ROS2_GAMMA 1.7071067811865475
@@ -299,24 +550,12 @@ struct Integ {
     // returns true when the integration has ended (t = 1 or the step limit)
     __device__ bool attempt(const DevTables &T, const DevPhys &P, double rtol) {
         const double atol[7] = {1e-7, 1e-7, 1e-7, 1e-7, 1e-7, 1e-7, 1.};
-        double f1[7], fp[7], xt[7], M[N * N], k1[N], k2[N];
+        double f1[7], xt[7], M[N * N], k1[N], k2[N];
         int piv[N];
         if (need_fj) {
             if (!(t < 1.0 && nsteps < ROS2_MAXSTEPS)) return true;
             if (h > 1.0 - t) h = 1.0 - t;
-            rhs<ISO, ADI, OTS, HONLY>(x, f0, s, T, P, Riso);
-#pragma unroll
-            for (int jj = 0; jj < N; jj++) {
-                const int j = S::act(jj);
-#pragma unroll
-                for (int q = 0; q < 7; q++) xt[q] = x[q];
-                const double dl = 1.4901161193847656e-08 * fmax(fabs(x[j]), 1e-8);
-                xt[j] = x[j] + dl;
-                const double dlr = xt[j] - x[j];
-                rhs<ISO, ADI, OTS, HONLY>(xt, fp, s, T, P, Riso);
-#pragma unroll
-                for (int ii = 0; ii < N; ii++) J[ii * N + jj] = (fp[S::act(ii)] - f0[S::act(ii)]) / dlr;
-            }
+            rhs_jac<ISO, ADI, OTS, HONLY>(x, f0, J, s, T, P, Riso);
             need_fj = false;
         }
         nsteps++;

@@ -174,6 +174,34 @@ def test_chemistry_converges_to_tight_solution(golden, orc_h):
     assert np.abs(a["lambdaH"] - b["lambdaH"]).max() < 3e-4
 
 
+@pytest.mark.parametrize("ots", [True, False])
+def test_chemistry_jacobian_matches_central_differences(golden, ots):
+    """or_chem_jac (the integrator's exact Jacobian, dual numbers through the
+    rate table's interpolation) against central differences of or_chem_rhs
+    on the fixture's H + He temperature steps; its f is or_chem_rhs bit for
+    bit"""
+    orc = po.Oracle(po.units(hmf=0.76), po.flags(isothermal=False, h_only=False, on_the_spot=ots))
+    c = np.load(f"{golden}/chem_lsoda.npz")
+    steps = c["he_in"][:40]
+    worst = 0.0
+    for st in steps:
+        x = np.array([st["lambdaH"], st["lambdaHeI"], st["lambdaHeII"], st["yGrecHII"],
+                      st["yGrecHeII"], st["yGrecHeIII"], st["ie"]], np.float64)
+        f, J = orc.chem_jac(st, x)
+        assert np.array_equal(f, orc.chem_rhs(st, x))
+        for j in (0, 1, 2, 6):
+            h = 1e-7 * max(abs(x[j]), 1e-6)
+            xp, xm = x.copy(), x.copy()
+            xp[j] += h
+            xm[j] -= h
+            fd = (orc.chem_rhs(st, xp) - orc.chem_rhs(st, xm)) / (xp[j] - xm[j])
+            scale = np.abs(J[:, j]).max() + np.abs(fd).max() + 1e-300
+            worst = max(worst, np.abs(J[:, j] - fd).max() / scale)
+    # central differences agree to their truncation and rounding error; a
+    # wrong term (a dropped ye or log T dependence) is off by O(1)
+    assert worst < 1e-4, worst
+
+
 def _step(orc, **kw):
     s = np.zeros(1, po.STEP_DTYPE)
     s["lambdaHeI"] = 1.0
