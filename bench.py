@@ -30,8 +30,8 @@ BYTES_DEPOSIT_H = 36
 BYTES_DEPOSIT_HE = 68
 # HBM bytes per k_deposit launch from rocprofv3 FETCH_SIZE + WRITE_SIZE
 # passes, scaled by the same counters on a copy of known size (FETCH_SIZE x 2,
-# WRITE_SIZE x 1): 79,002 KiB x 2 + 34,458 KiB for workload 2
-TRAFFIC_DEPOSIT = {2: (79002 * 2 + 34458) * 1024}
+# WRITE_SIZE x 1): 85,049 KiB x 2 + 50,911 KiB for workload 2
+TRAFFIC_DEPOSIT = {2: (85049 * 2 + 50911) * 1024}
 WORKLOADS = {
     1: "32^3 jittered lattice, H-only, isothermal 1e4 K, one source",
     2: "128^3 lognormal, H-only, isothermal 1e4 K, 64 sources",
@@ -82,6 +82,31 @@ def cpu_baseline(wl, rays, seconds_target=15.0):
     }
 
 
+MLP_FLOP_PER_ROW = 2 * (22 * 64 + 64 * 64 + 64 * 64 + 64 * 4)   # 19,712
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def mlp_forward_only(n):
+    """the network's forward kernel alone (k_mlp_forward) on n random rows;
+    prints the rate from the wall time of the calls (host copies included)
+    -- the kernel's own time comes from rocprofv3"""
+    import psphray_amd as psr
+    rng = np.random.default_rng(1)
+    x = rng.normal(0, 1, (n, 22)).astype(np.float32)
+    x[:, 1:4] = rng.uniform(0, 1, (n, 3))
+    with psr.Context(0) as c:
+        c.mlp_set_params(rng.normal(0, 0.1, psr._abi.lib().psr_mlp_param_count()).astype(np.float32))
+        c.mlp_set_norm(np.zeros(22, np.float32), np.ones(22, np.float32))
+        best = 1e30
+        for _ in range(3):
+            t0 = time.perf_counter()
+            c.mlp_forward(x)
+            best = min(best, time.perf_counter() - t0)
+    print(json.dumps({"mlp_forward_rows": n, "flop_per_row": MLP_FLOP_PER_ROW,
+                      "best_call_ms_incl_copies": best * 1e3,
+                      "tflops_incl_copies": n * MLP_FLOP_PER_ROW / best / 1e12}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +126,9 @@ def main():
     ap.add_argument("--accum", choices=["auto", "atomic", "binned"], default="auto",
                     help="deposit accumulation (auto: timed on the first ticks)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mlp-forward-rows", type=int, default=0,
+                    help="only time the surrogate network's forward kernel on this many random "
+                         "feature rows (random weights) and exit")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run plain device copies of a known size (scale for the PMC "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -114,6 +142,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes_read": nbytes, "bytes_written": nbytes, "dispatches": 5}))
+        return
+    if args.mlp_forward_rows:
+        mlp_forward_only(args.mlp_forward_rows)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "

@@ -112,14 +112,12 @@ __host__ __device__ inline void mlp_predict(const float *f, const float *o, doub
 typedef short mlp_bf16x4 __attribute__((ext_vector_type(4)));
 typedef float mlp_f32x4 __attribute__((ext_vector_type(4)));
 
-// float -> bfloat16, round to nearest even (finite inputs)
-__host__ __device__ inline short mlp_bf16(float f) {
-    uint32_t u = __builtin_bit_cast(uint32_t, f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (short)(u >> 16);
-}
+// float -> bfloat16, round to nearest even (v_cvt_pk_bf16_f32 on gfx950)
+__device__ __forceinline__ short mlp_bf16(float f) { return __builtin_bit_cast(short, (__bf16)f); }
 
-__device__ __forceinline__ float mlp_silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU z / (1 + e^-z) with the hardware exp and reciprocal (their ulp-level
+// error is far below the BF16 rounding that follows)
+__device__ __forceinline__ float mlp_silu(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 
 // weights as MFMA A operands (tiles of W^T), held in registers:
 // lane l of tile (mt, kb) holds W[i][o], o = 16 mt + l%16, i = 16 kb + 4(l/16) + j

@@ -82,7 +82,7 @@ MlpBuf mlp_buf(short *base, int bp) {
 constexpr size_t MLP_BUF_ROWS = MLP_INP + 6 * MLP_HID + 16;
 
 __device__ __forceinline__ float silu_grad(float z) {
-    const float s = 1.0f / (1.0f + __expf(-z));
+    const float s = __builtin_amdgcn_rcpf(1.0f + __expf(-z));
     return s * (1.0f + z * (1.0f - s));
 }
 
