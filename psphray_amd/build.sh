@@ -8,7 +8,9 @@ OUT="$HERE/_build"
 mkdir -p "$OUT"
 HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
 # -ffp-contract=off: the traversal reproduces separately rounded float
-#   products of the reference geometry; -packed-fp32-ops off: see DESIGN.md
+#   products of the reference geometry; -packed-fp32-ops off: packed FP32
+#   instructions are left out of every kernel (the compiler otherwise forms
+#   them from scalar or float2 code)
 HIPFLAGS=(--offload-arch=gfx950 -O3 -std=c++17 -fPIC -ffp-contract=off -munsafe-fp-atomics
           -Xclang -target-feature -Xclang -packed-fp32-ops
           -I"$ROOT/include" -I"$HERE/csrc" -Wno-unused-result)
