@@ -210,7 +210,14 @@ extern "C" int psr_set_physics(psr_ctx *c, const psr_units *u, const psr_flags *
     const size_t nsi = t->si_n1 > 0 ? (size_t)t->si_n1 + t->si_n2 + 3 * (size_t)t->si_n1 * t->si_n2 : 0;
     PSR_HIP(c, hipMalloc(&c->d_tables, sizeof(double) * (nar + nxs + nk + nsi)));
     double *p = c->d_tables;
-    PSR_HIP(c, hipMemcpy(p, t->ar, sizeof(double) * nar, hipMemcpyHostToDevice));
+    {
+        // rate table row-major on the device: one log10 T row (logT and the 15
+        // rate columns) is 128 bytes, a single cache line per interpolation end
+        std::vector<double> rows(nar);
+        for (int32_t i = 0; i < t->ar_nrows; i++)
+            for (int k = 0; k < 16; k++) rows[(size_t)i * 16 + k] = t->ar[(size_t)k * t->ar_nrows + i];
+        PSR_HIP(c, hipMemcpy(p, rows.data(), sizeof(double) * nar, hipMemcpyHostToDevice));
+    }
     PSR_HIP(c, hipMemcpy(p + nar, t->xs, sizeof(double) * nxs, hipMemcpyHostToDevice));
     PSR_HIP(c, hipMemcpy(p + nar + nxs, t->kernel, sizeof(double) * nk, hipMemcpyHostToDevice));
     DevTables T{};

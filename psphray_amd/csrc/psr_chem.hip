@@ -50,13 +50,13 @@ __device__ __forceinline__ void rates_at(const DevTables &T, double logT, double
     } else {
         mode = 1;
         i0 = (int)idx;
-        lw = T.ar[i0 + 1] - logT;
-        rw = logT - T.ar[i0];
+        lw = T.ar[(size_t)(i0 + 1) * 16] - logT;
+        rw = logT - T.ar[(size_t)i0 * 16];
     }
+    const double *r0 = T.ar + (size_t)i0 * 16;
     auto col = [&](int k) -> double {
-        const double *c = T.ar + (size_t)(k + 1) * n;
-        if (mode != 1) return c[i0];
-        return (lw * c[i0] + rw * c[i0 + 1]) * T.ar_step_inv;
+        if (mode != 1) return r0[k + 1];
+        return (lw * r0[k + 1] + rw * r0[16 + k + 1]) * T.ar_step_inv;
     };
     R.gamma_HI = col(AR_HI_CI) * nH;
     R.alpha_HII_A = col(AR_HII_RC_A) * nH;
@@ -268,17 +268,17 @@ __device__ __forceinline__ void rates_slopes_at(const DevTables &T, double logT,
     } else {
         mode = 1;
         i0 = (int)idx;
-        lw = T.ar[i0 + 1] - logT;
-        rw = logT - T.ar[i0];
+        lw = T.ar[(size_t)(i0 + 1) * 16] - logT;
+        rw = logT - T.ar[(size_t)i0 * 16];
     }
+    const double *r0 = T.ar + (size_t)i0 * 16;
     auto col = [&](int k, double &sl) -> double {
-        const double *c = T.ar + (size_t)(k + 1) * n;
         if (mode != 1) {
             sl = 0.0;
-            return c[i0];
+            return r0[k + 1];
         }
-        sl = (c[i0 + 1] - c[i0]) * T.ar_step_inv;
-        return (lw * c[i0] + rw * c[i0 + 1]) * T.ar_step_inv;
+        sl = (r0[16 + k + 1] - r0[k + 1]) * T.ar_step_inv;
+        return (lw * r0[k + 1] + rw * r0[16 + k + 1]) * T.ar_step_inv;
     };
     double sl;
     R.gamma_HI = col(AR_HI_CI, sl) * nH;
@@ -848,39 +848,54 @@ __global__ __launch_bounds__(256) void k_update(DevState S, int64_t npar, const 
     count_steps(st, steps, fail, done);
 }
 
-// phase 2: the particles left over, run to the end (grid-stride over the
-// compacted list whose length the selection wrote to device memory)
+// phase 2: the listed particles, lane by lane.  Integrations need from one
+// to over a hundred steps, so a wavefront that ran one particle per lane to
+// the end would wait for its slowest lane; here a lane whose particle has
+// finished takes the next one from the list (one claim per wavefront for all
+// its idle lanes), and every iteration is one step attempt on every busy
+// lane.  nlist[0] is the list's length, nlist[1] the next unclaimed entry.
 template <bool ISO, bool ADI, bool OTS, bool HONLY>
 __global__ __launch_bounds__(256) void k_update_rest(DevState S, const uint32_t *__restrict__ list,
-                                                     const uint32_t *__restrict__ nlist,
+                                                     uint32_t *__restrict__ nlist,
                                                      const DevTick *__restrict__ tk, DevTables T, DevPhys P,
                                                      DevStats *__restrict__ st, const double *__restrict__ saved) {
     using G = Integ<ISO, ADI, OTS, HONLY>;
     if (halted(&st->flags)) return;
     const int64_t tick = tk->tick;
     const double tick_time = tk->tick_time;
-    const uint32_t n = *nlist;
+    const uint32_t n = nlist[0];
+    const int lane = threadIdx.x & 63;
     int steps = 0, fail = 0, done = 0;
-    // uniform trip count so every lane reaches the wavefront reduction
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t k0 = blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint32_t kb = blockIdx.x * blockDim.x; kb < n; kb += stride) {
-        const uint32_t k = kb + (k0 - blockIdx.x * blockDim.x);
-        if (k < n) {
-            const uint32_t i = list[k];
-            double x0[7];
-            StepIn s;
-            load_step(S, i, tick, tick_time, P, x0, s);
-            G g;
-            g.init(x0, s, T);
-            g.restore(saved + (size_t)i * G::SAVE_WORDS);
-            while (!g.attempt(T, P, 1e-4)) {
+    G g;
+    uint32_t i = 0;
+    bool have = false, drained = false;
+    for (;;) {
+        const unsigned long long idle = __ballot(!have);
+        if (idle && !drained) {
+            const int leader = __ffsll((long long)idle) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&nlist[1], (uint32_t)__popcll(idle));
+            base = __shfl(base, leader, 64);
+            drained = base + (uint32_t)__popcll(idle) >= n;
+            const uint32_t k = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1));
+            if (!have && k < n) {
+                i = list[k];
+                double x0[7];
+                StepIn s;
+                load_step(S, i, tick, tick_time, P, x0, s);
+                g.init(x0, s, T);
+                g.restore(saved + (size_t)i * G::SAVE_WORDS);
+                have = true;
             }
+        }
+        if (!__ballot(have)) break;
+        if (have && g.attempt(T, P, 1e-4)) {
             g.clamp();
             steps += g.nsteps;
             fail += g.t < 1.0;
             done++;
-            store_step<ISO, ADI>(S, i, tick, P, g.x, s.T);
+            store_step<ISO, ADI>(S, i, tick, P, g.x, g.s.T);
+            have = false;
         }
     }
     count_steps(st, steps, fail, done);
@@ -1051,8 +1066,8 @@ __global__ __launch_bounds__(256) void k_mlp_update(DevState S, int64_t npar, co
 
 }  // namespace
 
-__global__ void k_zero_u32(uint32_t *p) {
-    if (threadIdx.x == 0) *p = 0;
+__global__ void k_zero_u32x2(uint32_t *p) {
+    if (threadIdx.x < 2) p[threadIdx.x] = 0;
 }
 
 #define CHEM_DISPATCH(KERNEL, ...)                                                                       \
@@ -1113,7 +1128,7 @@ int psr_launch_update(psr_ctx *c, bool timed) {
     }
     const bool closed = c->F.isothermal && c->F.on_the_spot && c->F.h_only;
     if (!closed) {
-        k_zero_u32<<<1, 64, 0, c->stream>>>(c->d_q);
+        k_zero_u32x2<<<1, 64, 0, c->stream>>>(c->d_q);
         PSR_HIP(c, hipGetLastError());
     }
     CHEM_DISPATCH(k_update, S, n, tk, c->T, P, c->d_stats, c->d_saved, c->d_qlist, c->d_q);

@@ -35,7 +35,7 @@ static_assert(sizeof(psr_ray) == 64, "psr_ray must be 64 bytes");
 
 // tables in device memory (layouts as psr_tables)
 struct DevTables {
-    const double *ar;
+    const double *ar;  // [ar_nrows][16]: log10 T, then the 15 rate columns
     int32_t ar_nrows;
     double ar_min, ar_step_inv;
     const double *xs;
