@@ -297,110 +297,6 @@ __global__ __launch_bounds__(256) void k_sort(int64_t nray, DevHits H, const uin
     }
 }
 
-// LDS writes of one wavefront visible to its other lanes, with no code
-// moved across (a wavefront's LDS operations complete in order)
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
-
-// The bucket sort of k_sort for short lists, one wavefront per ray: four rays
-// per workgroup, each in its own LDS, with no workgroup barriers (a short
-// list left most of a 256-thread workgroup idle at every barrier).
-template <int MAXN, int MAXB>
-__global__ __launch_bounds__(256) void k_sort_wave(int64_t nray, DevHits H, const uint32_t *__restrict__ orig,
-                                                   int32_t lo, int32_t hi, const int32_t *halt) {
-    __shared__ unsigned long long key_s[4][MAXN];
-    __shared__ uint16_t idx_s[4][MAXN];
-    __shared__ int32_t start_s[4][MAXB];
-    __shared__ int32_t fill_s[4][MAXB];
-    if (halted(halt)) return;
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    unsigned long long *key = key_s[wv];
-    uint16_t *idx = idx_s[wv];
-    int32_t *start = start_s[wv], *fill = fill_s[wv];
-    for (int64_t r = (int64_t)blockIdx.x * 4 + wv; r < nray; r += (int64_t)gridDim.x * 4) {
-        const int n = H.count[r];
-        if (n <= lo || n > hi) continue;
-        const int64_t base = r * (int64_t)H.cap;
-        double mn = INFINITY, mx = -INFINITY;
-        for (int i = lane; i < n; i += 64) {
-            const double d = H.d[base + i];
-            mn = fmin(mn, d);
-            mx = fmax(mx, d);
-        }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            mn = fmin(mn, __shfl_xor(mn, o, 64));
-            mx = fmax(mx, __shfl_xor(mx, o, 64));
-        }
-        int B = 1;
-        while (B * 4 < n && B < MAXB) B <<= 1;
-        const double scale = mx > mn ? (double)B / (mx - mn) : 0.0;
-        for (int b = lane; b < B; b += 64) {
-            start[b] = 0;
-            fill[b] = 0;
-        }
-        wave_sync();
-        for (int i = lane; i < n; i += 64) {
-            const int b = min(B - 1, (int)((H.d[base + i] - mn) * scale));
-            atomicAdd(&start[b], 1);
-        }
-        wave_sync();
-        {
-            // exclusive scan of the counts, each lane a contiguous run
-            const int per = (B + 63) / 64;
-            const int b0 = lane * per, b1 = min(B, b0 + per);
-            int sum = 0;
-            for (int b = b0; b < b1; b++) sum += start[b];
-            int incl = sum;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const int v = __shfl_up(incl, o, 64);
-                if (lane >= o) incl += v;
-            }
-            int run = incl - sum;
-            for (int b = b0; b < b1; b++) {
-                const int cnt = start[b];
-                start[b] = run;
-                run += cnt;
-            }
-        }
-        wave_sync();
-        for (int i = lane; i < n; i += 64) {
-            const double d = H.d[base + i];
-            const int b = min(B - 1, (int)((d - mn) * scale));
-            const int p = start[b] + atomicAdd(&fill[b], 1);
-            key[p] = (unsigned long long)__double_as_longlong(d);
-            idx[p] = (uint16_t)i;
-        }
-        wave_sync();
-        for (int b = lane; b < B; b += 64) {
-            const int s0 = start[b], e0 = s0 + fill[b];
-            for (int j = s0 + 1; j < e0; j++) {
-                const unsigned long long kj = key[j];
-                const uint16_t ij = idx[j];
-                int q = j - 1;
-                while (q >= s0 && hit_less(kj, key[q], ij, idx[q], H, base, orig)) {
-                    key[q + 1] = key[q];
-                    idx[q + 1] = idx[q];
-                    q--;
-                }
-                key[q + 1] = kj;
-                idx[q + 1] = ij;
-            }
-        }
-        wave_sync();
-        for (int i = lane; i < n; i += 64) {
-            const uint32_t q = idx[i];
-            H.perm[base + i] = q;
-            H.sslot[base + i] = H.slot[base + q];
-            H.sb[base + i] = H.b[base + q];
-        }
-        wave_sync();
-    }
-}
-
 // sorted slot / b columns from a permutation (rays of the global fallback)
 __global__ void k_apply_perm(const int32_t *__restrict__ rays_big, int32_t nbig, DevHits H, const int32_t *halt) {
     if (halted(halt)) return;
@@ -596,12 +492,10 @@ int psr_launch_sort(psr_ctx *c, int64_t nray) {
     const unsigned grid = (unsigned)std::min<int64_t>(nray, 65535 * 4);
     const int32_t *halt = &c->d_stats->flags;
     PSR_EV(c, 1, 0);
-    // LDS tiers by list length: 10 B per hit + 8 B per bucket; lists of up to
-    // 512 hits one wavefront each
-    k_sort_wave<512, 128><<<(unsigned)std::min<int64_t>((nray + 3) / 4, 65535 * 4), 256, 0, c->stream>>>(
-        nray, c->H, c->d_orig, -1, 512, halt);
-    PSR_HIP(c, hipGetLastError());
-    k_sort<1024, 256><<<grid, 256, 0, c->stream>>>(nray, c->H, c->d_orig, 512, 1024, halt);
+    // LDS tiers by list length: 10 B per hit + 8 B per bucket (a wavefront-per-
+    // ray tier for short lists saved 15 us on the batched fields and cost 20 us
+    // on 128^3, its extra launch included, so there is none)
+    k_sort<1024, 256><<<grid, 256, 0, c->stream>>>(nray, c->H, c->d_orig, -1, 1024, halt);
     PSR_HIP(c, hipGetLastError());
     if (c->H.cap > 1024) {
         k_sort<4096, 1024><<<grid, 256, 0, c->stream>>>(nray, c->H, c->d_orig, 1024, 4096, halt);
