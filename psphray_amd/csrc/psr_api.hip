@@ -238,9 +238,17 @@ extern "C" int psr_set_physics(psr_ctx *c, const psr_units *u, const psr_flags *
         PSR_HIP(c, hipMemcpy(q, t->si_x, sizeof(double) * t->si_n1, hipMemcpyHostToDevice));
         PSR_HIP(c, hipMemcpy(q + t->si_n1, t->si_y, sizeof(double) * t->si_n2, hipMemcpyHostToDevice));
         double *d = q + t->si_n1 + t->si_n2;
-        PSR_HIP(c, hipMemcpy(d, t->si_phi_hi, sizeof(double) * n12, hipMemcpyHostToDevice));
-        PSR_HIP(c, hipMemcpy(d + n12, t->si_phi_hei, sizeof(double) * n12, hipMemcpyHostToDevice));
-        PSR_HIP(c, hipMemcpy(d + 2 * n12, t->si_eh, sizeof(double) * n12, hipMemcpyHostToDevice));
+        {
+            // the three tables interleaved: one lookup reads all three from the
+            // same cache lines
+            std::vector<double> tab3(3 * n12);
+            for (size_t k = 0; k < n12; k++) {
+                tab3[3 * k] = t->si_phi_hi[k];
+                tab3[3 * k + 1] = t->si_phi_hei[k];
+                tab3[3 * k + 2] = t->si_eh[k];
+            }
+            PSR_HIP(c, hipMemcpy(d, tab3.data(), sizeof(double) * 3 * n12, hipMemcpyHostToDevice));
+        }
         T.si_n1 = t->si_n1;
         T.si_n2 = t->si_n2;
         T.si_min1 = t->si_min1;
@@ -249,9 +257,7 @@ extern "C" int psr_set_physics(psr_ctx *c, const psr_units *u, const psr_flags *
         T.si_step2_inv = t->si_step2_inv;
         T.si_x = q;
         T.si_y = q + t->si_n1;
-        T.si_phi_hi = d;
-        T.si_phi_hei = d + n12;
-        T.si_eh = d + 2 * n12;
+        T.si_tab3 = d;
     }
     c->T = T;
     c->U = *u;

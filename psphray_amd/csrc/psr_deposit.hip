@@ -301,19 +301,22 @@ __device__ __forceinline__ DepRec<HE> apply_values(double tm, const WalkRec<HE> 
         acc.fiHI += deltaHI * NH;
         if (k.secion_terms) {
             const double log10x = log10(x);
-            const double sec = deltaHI * k.dEHI / P.C_HI_ENERGY * tab2_get(T, T.si_phi_hi, k.dEHI, log10x) +
-                               deltaHeI * k.dEHeI / P.C_HI_ENERGY * tab2_get(T, T.si_phi_hi, k.dEHeI, log10x);
+            // phi_HI, phi_HeI, E_h at (dE_HI, log10 x) and at (dE_HeI, log10 x);
+            // hydrogen only, deltaHeI is 0 and so is its term for any table value
+            double vH[3], vHe[3] = {0.0, 0.0, 0.0};
+            tab2_get3(T, k.dEHI, log10x, vH);
+            if (HE) tab2_get3(T, k.dEHeI, log10x, vHe);
+            const double sec = deltaHI * k.dEHI / P.C_HI_ENERGY * vH[0] + deltaHeI * k.dEHeI / P.C_HI_ENERGY * vHe[0];
             yHI += sec;
             acc.siHI += sec * NH;
-            ht += k.hf_HI * deltaHI * tab2_get(T, T.si_eh, k.dEHI, log10x);
+            ht += k.hf_HI * deltaHI * vH[2];
             if (he) {
                 const double sec2 =
-                    deltaHI * k.dEHI / P.C_HEI_ENERGY * tab2_get(T, T.si_phi_hei, k.dEHI, log10x) +
-                    deltaHeI * k.dEHeI / P.C_HEI_ENERGY * tab2_get(T, T.si_phi_hei, k.dEHeI, log10x);
+                    deltaHI * k.dEHI / P.C_HEI_ENERGY * vH[1] + deltaHeI * k.dEHeI / P.C_HEI_ENERGY * vHe[1];
                 yHeI += deltaHeI + sec2;
                 acc.siHeI += sec2 * NHe;
                 acc.fiHeI += deltaHeI * NHe;
-                ht += k.hf_HeI * deltaHeI * tab2_get(T, T.si_eh, k.dEHeI, log10x);
+                ht += k.hf_HeI * deltaHeI * vHe[2];
                 yHeII += deltaHeII;
                 acc.fiHeII += deltaHeII * NHe;
             }

@@ -41,7 +41,10 @@ __device__ __forceinline__ double sph_depth(const DevTables &T, double r_h) {
 }
 
 // bilinear TabFun2 lookup of src/tabfun.c:173-223
-__device__ inline double tab2_get(const DevTables &T, const double *d, double x, double y) {
+__device__ inline void tab2_get3(const DevTables &T, double x, double y, double v[3]) {
+    // the three tables interleaved, [n1][n2][3]; each value with TabFun2's
+    // interpolation (src/tabfun.c:173-223), the indices computed once
+    const double *d = T.si_tab3;
     const int n1 = T.si_n1, n2 = T.si_n2;
     const int xind = (int)((x - T.si_min1) * T.si_step1_inv);
     const int yind = (int)((y - T.si_min2) * T.si_step2_inv);
@@ -49,28 +52,42 @@ __device__ inline double tab2_get(const DevTables &T, const double *d, double x,
     if (xind < 0) linoffset = 0;
     if (xind >= n1 - 1) linoffset = (n1 - 1) * n2;
     if (linoffset != -1) {
-        if (yind < 0) return d[linoffset];
-        if (yind >= n2 - 1) return d[linoffset + n2 - 1];
+        if (yind < 0) {
+            for (int t = 0; t < 3; t++) v[t] = d[3 * (size_t)linoffset + t];
+            return;
+        }
+        if (yind >= n2 - 1) {
+            for (int t = 0; t < 3; t++) v[t] = d[3 * (size_t)(linoffset + n2 - 1) + t];
+            return;
+        }
         const double left = T.si_y[yind], right = T.si_y[yind + 1];
-        return ((right - y) * d[linoffset + yind] + (y - left) * d[linoffset + yind + 1]) * T.si_step2_inv;
+        for (int t = 0; t < 3; t++)
+            v[t] = ((right - y) * d[3 * (size_t)(linoffset + yind) + t] +
+                    (y - left) * d[3 * (size_t)(linoffset + yind + 1) + t]) *
+                   T.si_step2_inv;
+        return;
     }
     int lin = -1;
     if (yind < 0) lin = 0;
     if (yind >= n2 - 1) lin = n2 - 1;
     if (lin != -1) {
         const double left = T.si_x[xind], right = T.si_x[xind + 1];
-        return ((right - x) * d[(size_t)xind * n2 + lin] + (x - left) * d[(size_t)(xind + 1) * n2 + lin]) *
-               T.si_step1_inv;
+        for (int t = 0; t < 3; t++)
+            v[t] = ((right - x) * d[3 * ((size_t)xind * n2 + lin) + t] +
+                    (x - left) * d[3 * ((size_t)(xind + 1) * n2 + lin) + t]) *
+                   T.si_step1_inv;
+        return;
     }
     const double x1 = T.si_x[xind], x2 = T.si_x[xind + 1];
     const double y1 = T.si_y[yind], y2 = T.si_y[yind + 1];
     const double fac = T.si_step1_inv * T.si_step2_inv;
-    const double f11 = d[(size_t)xind * n2 + yind];
-    const double f12 = d[(size_t)xind * n2 + yind + 1];
-    const double f21 = d[(size_t)(xind + 1) * n2 + yind];
-    const double f22 = d[(size_t)(xind + 1) * n2 + yind + 1];
-    return fac * (f11 * (x2 - x) * (y2 - y) + f21 * (x - x1) * (y2 - y) + f12 * (x2 - x) * (y - y1) +
-                  f22 * (x - x1) * (y - y1));
+    const size_t o11 = 3 * ((size_t)xind * n2 + yind), o21 = 3 * ((size_t)(xind + 1) * n2 + yind);
+    for (int t = 0; t < 3; t++) {
+        const double f11 = d[o11 + t], f12 = d[o11 + 3 + t];
+        const double f21 = d[o21 + t], f22 = d[o21 + 3 + t];
+        v[t] = fac * (f11 * (x2 - x) * (y2 - y) + f21 * (x - x1) * (y2 - y) + f12 * (x2 - x) * (y - y1) +
+                      f22 * (x - x1) * (y - y1));
+    }
 }
 
 __device__ __forceinline__ double wave_sum(double v) {

@@ -46,7 +46,8 @@ struct DevTables {
     int32_t kernel_n;
     int32_t si_n1, si_n2;
     double si_min1, si_min2, si_step1_inv, si_step2_inv;
-    const double *si_x, *si_y, *si_phi_hi, *si_phi_hei, *si_eh;
+    const double *si_x, *si_y;
+    const double *si_tab3;  // PhiHI, PhiHeI, Eh interleaved: [si_n1][si_n2][3]
 };
 
 // constants the kernels need, passed by value
