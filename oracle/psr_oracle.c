@@ -1208,8 +1208,36 @@ static void lu_solve(const double *a, int n, const int *piv, double *b) {
     }
 }
 
-#define ROS2_GAMMA 1.7071067811865475 /* 1 + 1/sqrt(2) */
-#define ROS2_MAXSTEPS 100000
+/* ROS3 of Sandu, Verwer, Blom, Spee, Carmichael & Potra, "Benchmarking stiff
+ * ODE solvers for atmospheric chemistry problems II: Rosenbrock solvers",
+ * Atmos. Environ. 31 (1997) 3459-3472: three stages, two evaluations of F,
+ * order 3 with an embedded order-2 solution, L-stable.  Coefficients of the
+ * form G = I/(gamma h) - J, G K_i = F(y + sum a_ij K_j) + sum (c_ij/h) K_j. */
+#define ROS3_GAMMA 0.43586652150845899941601945119356
+#define ROS3_C21 (-1.0156171083877702091975600115545)
+#define ROS3_C31 4.0759956452537699824805835358067
+#define ROS3_C32 9.2076794298330791242156818474003
+#define ROS3_M1 1.0
+#define ROS3_M2 6.1697947043828245592553615689730
+#define ROS3_M3 (-0.42772256543218573326238373806514)
+#define ROS3_E1 0.5
+#define ROS3_E2 (-2.9079558716805469821718236208017)
+#define ROS3_E3 0.22354069897811569627360909276199
+#define ROS3_MAXSTEPS 100000
+
+/* v^(1/3) for the step-size controller, v in [1e-10, 1e10]: Newton's
+ * iteration on the mantissa, only correctly rounded operations, so the
+ * device computes the same bits */
+static double ctl_cbrt(double v) {
+    int e;
+    double m = frexp(v, &e);
+    const int r = ((e % 3) + 3) % 3;
+    m = ldexp(m, r);
+    e -= r;
+    double y = 0.7 + 0.25 * m;
+    for (int k = 0; k < 5; k++) y = y - (y * y * y - m) / (3.0 * y * y);
+    return ldexp(y, e / 3);
+}
 
 /* Isothermal, hydrogen-only, on-the-spot: the rates are fixed over the step
  * and src/solve.c:97-99 reduces to the Riccati equation (x = x_HI, tau = t /
@@ -1245,11 +1273,52 @@ double or_riccati_tick(double x0, double g, double m, double G, double A) {
     return x0 + F0 * (s / lam) / D;
 }
 
-/* Two-stage L-stable Rosenbrock (Verwer, Spee, Blom & Hundsdorfer 1999):
- *   (I - g h J) k1 = F(y),  (I - g h J) k2 = F(y + h k1) - 2 k1,
- *   y' = y + 1.5 h k1 + 0.5 h k2,  err = 0.5 h (k1 + k2).
- * Exact Jacobian over the evolving variables (or_chem_jac, dual numbers).
- * Tolerances of src/solve.c:151-152 unless overridden. */
+/* One ROS3 attempt of size h from x with F(x) = f0 and the Jacobian J over
+ * the n evolving variables act[]: the new state xn[7] and the embedded
+ * error err[n]; returns -1 when I/(gamma h) - J is singular. */
+static int ros3_attempt(or_chem_data *cd, const int *act, int n, const double *x, const double *f0,
+                        const double *J, double t, double h, double *xn, double *err) {
+    double M[49], f1[7], xt[7], k1[7], k2[7], k3[7];
+    int piv[7];
+    const double ghinv = 1.0 / (ROS3_GAMMA * h);
+    for (int ii = 0; ii < n; ii++)
+        for (int jj = 0; jj < n; jj++) M[ii * n + jj] = (ii == jj ? ghinv : 0.0) - J[ii * n + jj];
+    if (lu_factor(M, n, piv)) return -1;
+    const double c21 = ROS3_C21 / h, c31 = ROS3_C31 / h, c32 = ROS3_C32 / h;
+    for (int ii = 0; ii < n; ii++) k1[ii] = f0[act[ii]];
+    lu_solve(M, n, piv, k1);
+    memcpy(xt, x, sizeof(xt));
+    for (int ii = 0; ii < n; ii++) xt[act[ii]] = x[act[ii]] + k1[ii];
+    or_chem_rhs(t + h, xt, f1, cd);
+    for (int ii = 0; ii < n; ii++) k2[ii] = f1[act[ii]] + c21 * k1[ii];
+    lu_solve(M, n, piv, k2);
+    for (int ii = 0; ii < n; ii++) k3[ii] = f1[act[ii]] + c31 * k1[ii] + c32 * k2[ii];
+    lu_solve(M, n, piv, k3);
+    memcpy(xn, x, sizeof(double) * 7);
+    for (int ii = 0; ii < n; ii++) {
+        const int i = act[ii];
+        xn[i] = x[i] + ROS3_M1 * k1[ii] + ROS3_M2 * k2[ii] + ROS3_M3 * k3[ii];
+        err[ii] = ROS3_E1 * k1[ii] + ROS3_E2 * k2[ii] + ROS3_E3 * k3[ii];
+    }
+    return 0;
+}
+
+static void jac_active(const or_ctx *c, const or_step *s, const int *act, int n, const double *x,
+                       double *f0, double *J) {
+    double J7[49];
+    or_chem_jac(c, s, x, f0, J7);
+    for (int ii = 0; ii < n; ii++)
+        for (int jj = 0; jj < n; jj++) J[ii * n + jj] = J7[act[ii] * 7 + act[jj]];
+}
+
+/* Three-stage L-stable Rosenbrock method ROS3 (coefficients above):
+ *   G = I/(gamma h) - J,   G K1 = F(y),   G K2 = F(y + K1) + (c21/h) K1,
+ *   G K3 = F(y + K1) + (c31/h) K1 + (c32/h) K2,
+ *   y' = y + m1 K1 + m2 K2 + m3 K3,   err = e1 K1 + e2 K2 + e3 K3,
+ * step factor 0.9 E^(-1/3) within [0.2, 5], E the RMS of err over the
+ * evolving variables scaled by atol + rtol max(|y|, |y'|).  Exact Jacobian
+ * over the evolving variables (or_chem_jac, dual numbers).  Tolerances of
+ * src/solve.c:151-152 unless overridden. */
 int32_t or_step_evolve(const or_ctx *c, or_step *s, double rtol, const double *atol7) {
     static const double atol_def[7] = {1e-7, 1e-7, 1e-7, 1e-7, 1e-7, 1e-7, 1.};
     const double *atol = atol7 ? atol7 : atol_def;
@@ -1261,8 +1330,7 @@ int32_t or_step_evolve(const or_ctx *c, or_step *s, double rtol, const double *a
                    s->yGrecHeIII, s->ie};
     double t = 0.0, h = 1.0;
     int nsteps = 0, rejected = 0;
-    double f0[7], f1[7], xt[7], k1[7], k2[7], J[49], M[49];
-    int piv[7];
+    double f0[7], J[49], err[7];
     if (c->f.isothermal && c->f.h_only && c->f.on_the_spot) {
         const double logT = log10(s->T);
         const double G = or_ar_get(c, AR_HI_CI, logT) * s->nH * s->time;
@@ -1271,36 +1339,19 @@ int32_t or_step_evolve(const or_ctx *c, or_step *s, double rtol, const double *a
         nsteps = 1;
         t = 1.0;
     }
-    while (t < 1.0 && nsteps < ROS2_MAXSTEPS) {
+    while (t < 1.0 && nsteps < ROS3_MAXSTEPS) {
         if (h > 1.0 - t) h = 1.0 - t;
-        double J7[49];
-        or_chem_jac(c, s, x, f0, J7);
-        for (int ii = 0; ii < n; ii++)
-            for (int jj = 0; jj < n; jj++) J[ii * n + jj] = J7[act[ii] * 7 + act[jj]];
+        jac_active(c, s, act, n, x, f0, J);
         for (;;) {
             nsteps++;
-            const double gh = ROS2_GAMMA * h;
-            for (int ii = 0; ii < n; ii++)
-                for (int jj = 0; jj < n; jj++) M[ii * n + jj] = (ii == jj ? 1.0 : 0.0) - gh * J[ii * n + jj];
-            int sing = lu_factor(M, n, piv);
             double E = INFINITY;
             double xn[7];
-            if (!sing) {
-                for (int ii = 0; ii < n; ii++) k1[ii] = f0[act[ii]];
-                lu_solve(M, n, piv, k1);
-                memcpy(xt, x, sizeof(xt));
-                for (int ii = 0; ii < n; ii++) xt[act[ii]] = x[act[ii]] + h * k1[ii];
-                or_chem_rhs(t + h, xt, f1, &cd);
-                for (int ii = 0; ii < n; ii++) k2[ii] = f1[act[ii]] - 2.0 * k1[ii];
-                lu_solve(M, n, piv, k2);
-                memcpy(xn, x, sizeof(xn));
+            if (!ros3_attempt(&cd, act, n, x, f0, J, t, h, xn, err)) {
                 double acc = 0.0;
                 for (int ii = 0; ii < n; ii++) {
                     const int i = act[ii];
-                    xn[i] = x[i] + 1.5 * h * k1[ii] + 0.5 * h * k2[ii];
-                    const double e = 0.5 * h * (k1[ii] + k2[ii]);
                     const double sc = atol[i] + rtol * fmax(fabs(x[i]), fabs(xn[i]));
-                    acc += (e / sc) * (e / sc);
+                    acc += (err[ii] / sc) * (err[ii] / sc);
                 }
                 E = sqrt(acc / n);
                 if (!(E == E)) E = INFINITY;
@@ -1308,7 +1359,7 @@ int32_t or_step_evolve(const or_ctx *c, or_step *s, double rtol, const double *a
             if (E <= 1.0 || h < 1e-14) {
                 t += h;
                 memcpy(x, xn, sizeof(x));
-                double fac = 0.9 / sqrt(fmax(E, 1e-10));
+                double fac = 0.9 / ctl_cbrt(fmin(fmax(E, 1e-10), 1e10));
                 fac = fmin(5.0, fmax(0.2, fac));
                 if (rejected) fac = fmin(fac, 1.0);
                 rejected = 0;
@@ -1317,7 +1368,7 @@ int32_t or_step_evolve(const or_ctx *c, or_step *s, double rtol, const double *a
                 break;
             }
             rejected = 1;
-            double fac = (E == INFINITY) ? 0.2 : fmax(0.2, 0.9 / sqrt(E));
+            double fac = (E == INFINITY) ? 0.2 : fmax(0.2, 0.9 / ctl_cbrt(fmin(E, 1e10)));
             h *= fac;
         }
     }
@@ -1338,6 +1389,28 @@ int32_t or_step_evolve(const or_ctx *c, or_step *s, double rtol, const double *a
     s->step_remain = 0.0;
     s->nsteps = nsteps;
     return t >= 1.0 ? 0 : -1;
+}
+
+/* ROS3 with nsub equal steps and no step control, for the convergence-order
+ * test: the final state x7 (unclamped) and the sum of the embedded error
+ * vectors err[7] (zero in frozen variables); -1 on a singular matrix */
+int32_t or_ros3_fixed(const or_ctx *c, const or_step *s, int32_t nsub, double *x7, double *err7) {
+    or_chem_data cd = {c, s};
+    int act[7];
+    const int n = chem_active(&c->f, act);
+    double x[7] = {s->lambdaH, s->lambdaHeI, s->lambdaHeII, s->yGrecHII, s->yGrecHeII,
+                   s->yGrecHeIII, s->ie};
+    double f0[7], J[49], err[7], xn[7];
+    const double h = 1.0 / nsub;
+    for (int q = 0; q < 7; q++) err7[q] = 0.0;
+    for (int k = 0; k < nsub; k++) {
+        jac_active(c, s, act, n, x, f0, J);
+        if (ros3_attempt(&cd, act, n, x, f0, J, k * h, h, xn, err)) return -1;
+        memcpy(x, xn, sizeof(x));
+        for (int ii = 0; ii < n; ii++) err7[act[ii]] += err[ii];
+    }
+    memcpy(x7, x, sizeof(x));
+    return 0;
 }
 
 int32_t or_step_evolve_many(const or_ctx *c, or_step *s, int64_t n, double rtol,

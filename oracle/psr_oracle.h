@@ -197,12 +197,15 @@ int or_chem_jac(const or_ctx *c, const or_step *step, const double *x, double *f
 /* exact one-step solution of the isothermal hydrogen-only on-the-spot rate
  * equation dx/dtau = -g + (1 + m - x)(A - (A + G) x) over tau in [0, 1] */
 double or_riccati_tick(double x0, double g, double m, double G, double A);
-/* the oracle's implicit FP64 solver (2-stage L-stable Rosenbrock with
+/* the oracle's implicit FP64 solver (3-stage L-stable Rosenbrock ROS3 with
  * embedded error estimate, same algorithm as the device), integrating
  * normalized time 0..1 and applying the clamps of src/solve.c:208-222; the
  * isothermal hydrogen-only on-the-spot case takes or_riccati_tick instead */
 int32_t or_step_evolve(const or_ctx *c, or_step *s, double rtol,
                        const double *atol7);
+/* ROS3 with nsub equal steps and no step control (convergence-order tests):
+ * final state x7 and the summed embedded error vectors err7 */
+int32_t or_ros3_fixed(const or_ctx *c, const or_step *s, int32_t nsub, double *x7, double *err7);
 /* chemistry in bulk (for fixtures): n steps in place */
 int32_t or_step_evolve_many(const or_ctx *c, or_step *s, int64_t n, double rtol,
                             int32_t nthreads);

@@ -90,6 +90,7 @@ _SIG = {
     "or_step_evolve": (_I32, [_VP, _VP, _D, _VP]),
     "or_riccati_tick": (_D, [_D, _D, _D, _D, _D]),
     "or_step_evolve_many": (_I32, [_VP, _VP, _I64, _D, _I32]),
+    "or_ros3_fixed": (_I32, [_VP, _VP, _I32, _VP, _VP]),
     "or_ieye2T": (_D, [_VP, _D, _D]),
     "or_Tye2ie": (_D, [_VP, _D, _D]),
     "or_set_flags": (None, [_VP, C.POINTER(OrFlags)]),
@@ -213,6 +214,15 @@ class Oracle:
         s = np.ascontiguousarray(steps, STEP_DTYPE).copy()
         bad = self.L.or_step_evolve_many(self.h, p(s), len(s), rtol, nthreads)
         return s, bad
+
+    def ros3_fixed(self, step, nsub):
+        """(x[7], summed embedded error[7]) of nsub fixed ROS3 steps"""
+        st = np.ascontiguousarray(step, STEP_DTYPE).reshape(1).copy()
+        x = np.zeros(7)
+        e = np.zeros(7)
+        if self.L.or_ros3_fixed(self.h, p(st), nsub, p(x), p(e)):
+            raise RuntimeError("singular ROS3 matrix")
+        return x, e
 
     def chem_rhs(self, step, x):
         """or_chem_rhs of one step record at state x[7]"""

@@ -5,9 +5,10 @@
 // 7-variable system (src/solve.c:42-141); variables the run flags freeze
 // (zero derivative) are left out of the implicit solve, so the linear algebra
 // is NV x NV with NV = 1 for an isothermal hydrogen-only on-the-spot run and
-// NV = 4 for H+He with temperature.  The integrator is a two-stage L-stable
-// Rosenbrock method (Verwer et al. 1999) with an embedded first-order error
-// estimate and the reference tolerances (rtol 1e-4, atol 1e-7 / 1 for ie);
+// NV = 4 for H+He with temperature.  The integrator is the three-stage
+// L-stable Rosenbrock method ROS3 (Sandu et al. 1997, order 3) with an
+// embedded second-order error estimate and the reference tolerances (rtol
+// 1e-4, atol 1e-7 / 1 for ie);
 // the Jacobian is exact (dual numbers through the right-hand side and the
 // rate table's interpolation), one table pass per step where finite
 // differences took N + 1.  The x_HI variable and x_HII = 1 - x_HI are both
@@ -400,8 +401,36 @@ __device__ void rhs_jac(const double x[7], double f[7], double *J, const StepIn 
         for (int jj = 0; jj < N; jj++) J[ii * N + jj] = g[SY::act(ii)].d[jj] * s.time;
 }
 
-#define ROS2_GAMMA 1.7071067811865475
-#define ROS2_MAXSTEPS 100000
+// ROS3 of Sandu, Verwer, Blom, Spee, Carmichael & Potra, Atmos. Environ. 31
+// (1997) 3459-3472: three stages, two evaluations of F, order 3 with an
+// embedded order-2 solution, L-stable; coefficients of the form
+// G = I/(gamma h) - J, G K_i = F(y + sum a_ij K_j) + sum (c_ij/h) K_j
+#define ROS3_GAMMA 0.43586652150845899941601945119356
+#define ROS3_C21 (-1.0156171083877702091975600115545)
+#define ROS3_C31 4.0759956452537699824805835358067
+#define ROS3_C32 9.2076794298330791242156818474003
+#define ROS3_M1 1.0
+#define ROS3_M2 6.1697947043828245592553615689730
+#define ROS3_M3 (-0.42772256543218573326238373806514)
+#define ROS3_E1 0.5
+#define ROS3_E2 (-2.9079558716805469821718236208017)
+#define ROS3_E3 0.22354069897811569627360909276199
+#define ROS3_MAXSTEPS 100000
+
+// v^(1/3) for the step-size controller, v in [1e-10, 1e10]: Newton's
+// iteration on the mantissa with correctly rounded operations only, so the
+// host oracle computes the same bits
+__device__ __forceinline__ double ctl_cbrt(double v) {
+    int e;
+    double m = frexp(v, &e);
+    const int r = ((e % 3) + 3) % 3;
+    m = ldexp(m, r);
+    e -= r;
+    double y = 0.7 + 0.25 * m;
+#pragma unroll
+    for (int k = 0; k < 5; k++) y = y - (y * y * y - m) / (3.0 * y * y);
+    return ldexp(y, e / 3);
+}
 
 // Isothermal hydrogen-only on-the-spot: with the rates fixed over the step
 // the rate equation (x = x_HI, tau = t / time) is the Riccati equation
@@ -550,43 +579,47 @@ struct Integ {
     // returns true when the integration has ended (t = 1 or the step limit)
     __device__ bool attempt(const DevTables &T, const DevPhys &P, double rtol) {
         const double atol[7] = {1e-7, 1e-7, 1e-7, 1e-7, 1e-7, 1e-7, 1.};
-        double f1[7], xt[7], M[N * N], k1[N], k2[N];
+        double f1[7], xt[7], M[N * N], k1[N], k2[N], k3[N];
         int piv[N];
         if (need_fj) {
-            if (!(t < 1.0 && nsteps < ROS2_MAXSTEPS)) return true;
+            if (!(t < 1.0 && nsteps < ROS3_MAXSTEPS)) return true;
             if (h > 1.0 - t) h = 1.0 - t;
             rhs_jac<ISO, ADI, OTS, HONLY>(x, f0, J, s, T, P, Riso);
             need_fj = false;
         }
         nsteps++;
-        const double gh = ROS2_GAMMA * h;
+        const double ghinv = 1.0 / (ROS3_GAMMA * h);
 #pragma unroll
         for (int ii = 0; ii < N; ii++)
 #pragma unroll
-            for (int jj = 0; jj < N; jj++) M[ii * N + jj] = (ii == jj ? 1.0 : 0.0) - gh * J[ii * N + jj];
+            for (int jj = 0; jj < N; jj++) M[ii * N + jj] = (ii == jj ? ghinv : 0.0) - J[ii * N + jj];
         const bool ok = lu_factor<N>(M, piv);
         double E = INFINITY;
         double xn[7];
         if (ok) {
+            const double c21 = ROS3_C21 / h, c31 = ROS3_C31 / h, c32 = ROS3_C32 / h;
 #pragma unroll
             for (int ii = 0; ii < N; ii++) k1[ii] = f0[S::act(ii)];
             lu_solve<N>(M, piv, k1);
 #pragma unroll
             for (int q = 0; q < 7; q++) xt[q] = x[q];
 #pragma unroll
-            for (int ii = 0; ii < N; ii++) xt[S::act(ii)] = x[S::act(ii)] + h * k1[ii];
+            for (int ii = 0; ii < N; ii++) xt[S::act(ii)] = x[S::act(ii)] + k1[ii];
             rhs<ISO, ADI, OTS, HONLY>(xt, f1, s, T, P, Riso);
 #pragma unroll
-            for (int ii = 0; ii < N; ii++) k2[ii] = f1[S::act(ii)] - 2.0 * k1[ii];
+            for (int ii = 0; ii < N; ii++) k2[ii] = f1[S::act(ii)] + c21 * k1[ii];
             lu_solve<N>(M, piv, k2);
+#pragma unroll
+            for (int ii = 0; ii < N; ii++) k3[ii] = f1[S::act(ii)] + c31 * k1[ii] + c32 * k2[ii];
+            lu_solve<N>(M, piv, k3);
 #pragma unroll
             for (int q = 0; q < 7; q++) xn[q] = x[q];
             double acc = 0.0;
 #pragma unroll
             for (int ii = 0; ii < N; ii++) {
                 const int i = S::act(ii);
-                xn[i] = x[i] + 1.5 * h * k1[ii] + 0.5 * h * k2[ii];
-                const double e = 0.5 * h * (k1[ii] + k2[ii]);
+                xn[i] = x[i] + ROS3_M1 * k1[ii] + ROS3_M2 * k2[ii] + ROS3_M3 * k3[ii];
+                const double e = ROS3_E1 * k1[ii] + ROS3_E2 * k2[ii] + ROS3_E3 * k3[ii];
                 const double sc = atol[i] + rtol * fmax(fabs(x[i]), fabs(xn[i]));
                 acc += (e / sc) * (e / sc);
             }
@@ -597,17 +630,17 @@ struct Integ {
             t += h;
 #pragma unroll
             for (int q = 0; q < 7; q++) x[q] = xn[q];
-            double fac = 0.9 / sqrt(fmax(E, 1e-10));
+            double fac = 0.9 / ctl_cbrt(fmin(fmax(E, 1e-10), 1e10));
             fac = fmin(5.0, fmax(0.2, fac));
             if (rejected) fac = fmin(fac, 1.0);
             rejected = false;
             h *= fac;
             if (1.0 - t < 1e-13) t = 1.0;
             need_fj = true;
-            return !(t < 1.0 && nsteps < ROS2_MAXSTEPS);
+            return !(t < 1.0 && nsteps < ROS3_MAXSTEPS);
         }
         rejected = true;
-        const double fac = (E == INFINITY) ? 0.2 : fmax(0.2, 0.9 / sqrt(E));
+        const double fac = (E == INFINITY) ? 0.2 : fmax(0.2, 0.9 / ctl_cbrt(fmin(E, 1e10)));
         h *= fac;
         return false;
     }

@@ -175,6 +175,30 @@ def test_chemistry_converges_to_tight_solution(golden, orc_h):
 
 
 @pytest.mark.parametrize("ots", [True, False])
+def test_ros3_convergence_orders(golden, ots):
+    """the integrator's method (or_ros3_fixed runs the same attempt as
+    or_step_evolve with equal steps and no step control) converges with order
+    3 and its embedded solution with order 2: halving the step divides the
+    errors by 8 and 4.  Isothermal H + He runs have constant rates, so the
+    right-hand side is smooth; a wrong tableau entry lowers an order."""
+    orc = po.Oracle(po.units(hmf=0.76), po.flags(isothermal=True, h_only=False, on_the_spot=ots))
+    c = np.load(f"{golden}/chem_lsoda.npz")
+    main, emb = [], []
+    for st in c["he_in"][::5]:
+        ref, _ = orc.ros3_fixed(st, 8192)
+        e = []
+        for ns in (32, 64, 128):
+            x, err = orc.ros3_fixed(st, ns)
+            e.append((np.abs(x - ref)[:6].max(), np.abs(x - err - ref)[:6].max()))
+        if 1e-10 < e[0][0] < 1e-4:
+            main.append(e[1][0] / e[2][0])
+            emb.append(e[1][1] / e[2][1])
+    assert len(main) > 20
+    assert 7.0 < np.median(main) < 9.0, np.median(main)
+    assert 3.5 < np.median(emb) < 4.5, np.median(emb)
+
+
+@pytest.mark.parametrize("ots", [True, False])
 def test_chemistry_jacobian_matches_central_differences(golden, ots):
     """or_chem_jac (the integrator's exact Jacobian, dual numbers through the
     rate table's interpolation) against central differences of or_chem_rhs
