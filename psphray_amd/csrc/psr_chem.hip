@@ -814,7 +814,9 @@ __global__ __launch_bounds__(256) void k_cap_post(DevState S, int64_t npar, DevP
     r[MLP_IN + 3] = iso ? r[4] : mlp_lg(T1, 1e-30);
 }
 
-#define PHASE1_ATTEMPTS 4
+#ifndef PHASE1_ATTEMPTS
+#define PHASE1_ATTEMPTS 1
+#endif
 
 // Chemistry of the active set, phase 1: one thread per particle in tree
 // order, at most PHASE1_ATTEMPTS step attempts (then up to the next accepted

@@ -101,15 +101,16 @@ __device__ __forceinline__ double walk_step<false>(double TM, const WalkRec<fals
 }
 template <>
 __device__ __forceinline__ double walk_step<true>(double TM, const WalkRec<true> &v) {
+    // TM - (aHI + (aHeI + aHeII)), each a = min(f * (-TM * em), N): fmin
+    // (one instruction) equals the reference's compare and replace for the
+    // non-negative operands here, up to the sign of a zero, which leaves TM
+    // unchanged.  A hit without helium carries f = N = 0 in the helium
+    // terms, so its sum is aHI + 0 = aHI exactly and no select is needed.
     const double ae = -TM * v.em;
-    double a0 = v.fhi * ae;
-    a0 = (a0 > v.nhi) ? v.nhi : a0;
-    double a1 = v.fhei * ae;
-    a1 = (a1 > v.nhei) ? v.nhei : a1;
-    double a2 = v.fheii * ae;
-    a2 = (a2 > v.nheii) ? v.nheii : a2;
-    const double absorb = (v.nhei >= 0.0) ? a0 + (a1 + a2) : a0;
-    return TM - absorb;
+    const double a0 = fmin(v.fhi * ae, v.nhi);
+    const double a1 = fmin(v.fhei * ae, v.nhei);
+    const double a2 = fmin(v.fheii * ae, v.nheii);
+    return TM - (a0 + (a1 + a2));
 }
 
 template <bool HE>
@@ -117,8 +118,7 @@ __device__ __forceinline__ void noop_rec(WalkRec<HE> &r, HitAux<HE> &a) {
     r.em = 0.0;
     r.nhi = 0.0;
     if constexpr (HE) {
-        r.fhi = r.fhei = r.fheii = r.nheii = 0.0;
-        r.nhei = -1.0;
+        r.fhi = r.fhei = r.fheii = r.nhei = r.nheii = 0.0;
         a.xHeI = a.xHeII = 0.0;
     }
     a.slot = 0;
@@ -206,7 +206,7 @@ __device__ __forceinline__ void prep_hit(const HitGather<HE> &g, double sdepth, 
         r.fhi = live ? tauHI / tausum : 0.0;
         r.fhei = (live && he) ? tauHeI / tausum : 0.0;
         r.fheii = (live && he) ? tauHeII / tausum : 0.0;
-        r.nhei = (live && he) ? NHeI : -1.0;  // -1: no helium terms
+        r.nhei = (live && he) ? NHeI : 0.0;
         r.nheii = (live && he) ? NHeII : 0.0;
         a.xHeI = xHeI;
         a.xHeII = xHeII;
@@ -256,7 +256,7 @@ __device__ __forceinline__ DepRec<HE> apply_values(double tm, const WalkRec<HE> 
     if constexpr (HE) {
         fHI = v.fhi;
         NHeI = v.nhei;
-        he = NHeI >= 0.0;
+        he = NHe != 0.0;  // src/loop.c:488
         if (he) {
             fHeI = v.fhei;
             fHeII = v.fheii;
@@ -378,6 +378,7 @@ struct DepLds {
     WalkRec<HE> rec[3][FSEG];
     HitAux<HE> aux[3][FSEG];
     double ck[3][FNCK];       // TM before hit FG * g of the segment
+    uint8_t glive[3][FNCK];   // group g of the segment has a live hit
     int32_t stop_seg, stop_grp, last;
     double tm_end;
 };
@@ -538,6 +539,11 @@ __global__ __launch_bounds__(FTHREADS, HE ? 3 : 4) void k_deposit(const DevTick 
                     if (i * FSEG + jj >= n) noop_rec<HE>(rr, au);
                     L.rec[ring][jj] = rr;
                     L.aux[ring][jj] = au;
+                    // the walk's test for a cut looks at groups with a live hit
+                    static_assert(64 % FG == 0 && 64 / FG <= 64, "groups within a wavefront");
+                    const unsigned long long lv = __ballot(is_live(rr.em));
+                    if (lane < 64 / FG)
+                        L.glive[ring][h * (64 / FG) + lane] = ((lv >> (lane * FG)) & ((1ull << FG) - 1)) != 0;
                 }
 #pragma unroll
                 for (int h = 0; h < HPL; h++) {
@@ -576,10 +582,9 @@ __global__ __launch_bounds__(FTHREADS, HE ? 3 : 4) void k_deposit(const DevTick 
                         for (int g = 0; g < ngrp; g++) {
                             const int g1 = min(g + 1, FNCK - 1);
                             L.ck[ring][g] = t;
-                            bool seen = false;
+                            const bool seen = L.glive[ring][g];
 #pragma unroll
                             for (int q = 0; q < FG; q++) {
-                                seen = seen | is_live(A[q].em);
                                 t = walk_step<HE>(t, A[q]);
                                 A[q] = L.rec[ring][g1 * FG + q];
                             }
@@ -599,12 +604,9 @@ __global__ __launch_bounds__(FTHREADS, HE ? 3 : 4) void k_deposit(const DevTick 
                             for (int q = 0; q < FG; q++) B[q] = L.rec[ring][g1 * FG + q];
                             __builtin_amdgcn_sched_barrier(0);
                             L.ck[ring][g] = t;
-                            bool seen = false;
+                            bool seen = L.glive[ring][g];
 #pragma unroll
-                            for (int q = 0; q < FG; q++) {
-                                seen = seen | is_live(A[q].em);
-                                t = walk_step<HE>(t, A[q]);
-                            }
+                            for (int q = 0; q < FG; q++) t = walk_step<HE>(t, A[q]);
                             if (t < T0 && seen) {
                                 L.stop_seg = s;
                                 L.stop_grp = g;
@@ -616,12 +618,9 @@ __global__ __launch_bounds__(FTHREADS, HE ? 3 : 4) void k_deposit(const DevTick 
                             for (int q = 0; q < FG; q++) A[q] = L.rec[ring][g2 * FG + q];
                             __builtin_amdgcn_sched_barrier(0);
                             L.ck[ring][g + 1] = t;
-                            seen = false;
+                            seen = L.glive[ring][g + 1];
 #pragma unroll
-                            for (int q = 0; q < FG; q++) {
-                                seen = seen | is_live(B[q].em);
-                                t = walk_step<HE>(t, B[q]);
-                            }
+                            for (int q = 0; q < FG; q++) t = walk_step<HE>(t, B[q]);
                             if (t < T0 && seen) {
                                 L.stop_seg = s;
                                 L.stop_grp = g + 1;

@@ -383,9 +383,17 @@ def test_tick_graph_matches_kernel_by_kernel(cfg, accum):
     # (measured: 0.1% of particles; a stale tick record would change most)
     assert np.mean(a.hits != b.hits) < 1e-2 and np.mean(a.lasthit != b.lasthit) < 1e-2
     for f in ("lambdaH", "lambdaHeI", "lambdaHeII"):
-        assert np.abs(getattr(a, f) - getattr(b, f)).max() < 1e-5, f
-    # ie: an adaptive step decision can flip on a last-bit difference (see
-    # test_full_tick_parity)
+        va, vb = getattr(a, f), getattr(b, f)
+        d = np.abs(va - vb)
+        if wl.physics.flags.isothermal:
+            # closed-form update: last-bit differences stay last-bit
+            assert d.max() < 1e-5, f
+        else:
+            # an adaptive step decision can flip on a last-bit difference (see
+            # test_full_tick_parity), which moves that particle's result within
+            # the integrator's tolerance (rtol 1e-4); bounded by the 1e-3 bar
+            assert np.median(d) < 1e-9, f
+            assert (d <= 1e-3 * np.maximum(np.abs(va), np.abs(vb)) + 1e-6).all(), f
     dr = np.abs(a.ie / b.ie - 1)
     assert np.median(dr) < 1e-6 and dr.max() < 1e-2
 
