@@ -23,8 +23,12 @@ for src in "$HERE"/csrc/*.hip; do
   for dep in "$src" "$HERE"/csrc/*.h "$ROOT/include/psphray_amd.h"; do
     [[ "$dep" -nt "$obj" ]] && stale=1
   done
+  # the deposit: no NaN reaches its arithmetic, so fmin/fmax take their
+  # operands as they are (without NaN-quieting instructions on the walk's chain)
+  extra=()
+  [[ "$(basename "$src")" == psr_deposit.hip ]] && extra=(-fno-honor-nans)
   if [[ "$FORCE" == 1 || ! -f "$obj" || "$stale" == 1 ]]; then
-    "$HIPCC" "${HIPFLAGS[@]}" -c "$src" -o "$obj" &
+    "$HIPCC" "${HIPFLAGS[@]}" "${extra[@]}" -c "$src" -o "$obj" &
     pids+=($!)
   fi
 done
