@@ -187,29 +187,38 @@ __device__ __forceinline__ bool hit_less(unsigned long long ka, unsigned long lo
     return pa < pb;
 }
 
-__device__ __forceinline__ double block_reduce(double v, bool is_max, double *red) {
+// the range [min, max] of one value per thread over the 256-thread block
+__device__ __forceinline__ void block_minmax(double &mn, double &mx, double *red) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-        const double w = __shfl_xor(v, o, 64);
-        v = is_max ? fmax(v, w) : fmin(v, w);
+        mn = fmin(mn, __shfl_xor(mn, o, 64));
+        mx = fmax(mx, __shfl_xor(mx, o, 64));
     }
     const int wid = threadIdx.x >> 6;
     __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[wid] = v;
+    if ((threadIdx.x & 63) == 0) {
+        red[wid] = mn;
+        red[4 + wid] = mx;
+    }
     __syncthreads();
-    double r = red[0];
-    for (int k = 1; k < (int)(blockDim.x >> 6); k++) r = is_max ? fmax(r, red[k]) : fmin(r, red[k]);
-    return r;
+    mn = red[0];
+    mx = red[4];
+#pragma unroll
+    for (int k = 1; k < 4; k++) {
+        mn = fmin(mn, red[k]);
+        mx = fmax(mx, red[4 + k]);
+    }
 }
 
 template <int MAXN, int MAXB>
 __global__ __launch_bounds__(256) void k_sort(int64_t nray, DevHits H, const uint32_t *__restrict__ orig,
                                               int32_t lo, int32_t hi, const int32_t *halt) {
+    constexpr int PER = MAXN / 256;  // distances per thread, held in registers
     __shared__ unsigned long long key[MAXN];
     __shared__ uint16_t idx[MAXN];
     __shared__ int32_t start[MAXB];
     __shared__ int32_t fill[MAXB];
-    __shared__ double red[4];
+    __shared__ double red[8];
     __shared__ int32_t wsum[4];
     if (halted(halt)) return;
     const int tid = threadIdx.x;
@@ -217,14 +226,21 @@ __global__ __launch_bounds__(256) void k_sort(int64_t nray, DevHits H, const uin
         const int n = H.count[r];
         if (n <= lo || n > hi) continue;
         const int64_t base = r * (int64_t)H.cap;
+        // one pass over the list: the range, the bucket counts and the
+        // scatter all use these registers
+        double dv[PER];
         double mn = INFINITY, mx = -INFINITY;
-        for (int i = tid; i < n; i += 256) {
-            const double d = H.d[base + i];
-            mn = fmin(mn, d);
-            mx = fmax(mx, d);
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = tid + k * 256;
+            dv[k] = i < n ? H.d[base + i] : 0.0;
+            if (i < n) {
+                mn = fmin(mn, dv[k]);
+                mx = fmax(mx, dv[k]);
+            }
         }
-        const double dmin = block_reduce(mn, false, red);
-        const double dmax = block_reduce(mx, true, red);
+        block_minmax(mn, mx, red);
+        const double dmin = mn, dmax = mx;
         int B = 1;
         while (B * 4 < n && B < MAXB) B <<= 1;
         const double scale = dmax > dmin ? (double)B / (dmax - dmin) : 0.0;
@@ -233,10 +249,12 @@ __global__ __launch_bounds__(256) void k_sort(int64_t nray, DevHits H, const uin
             fill[b] = 0;
         }
         __syncthreads();
-        for (int i = tid; i < n; i += 256) {
-            const double d = H.d[base + i];
-            const int b = min(B - 1, (int)((d - dmin) * scale));
-            atomicAdd(&start[b], 1);
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            if (tid + k * 256 < n) {
+                const int b = min(B - 1, (int)((dv[k] - dmin) * scale));
+                atomicAdd(&start[b], 1);
+            }
         }
         __syncthreads();
         // exclusive scan of the counts: each thread owns a contiguous run
@@ -263,12 +281,15 @@ __global__ __launch_bounds__(256) void k_sort(int64_t nray, DevHits H, const uin
             }
         }
         __syncthreads();
-        for (int i = tid; i < n; i += 256) {
-            const double d = H.d[base + i];
-            const int b = min(B - 1, (int)((d - dmin) * scale));
-            const int p = start[b] + atomicAdd(&fill[b], 1);
-            key[p] = (unsigned long long)__double_as_longlong(d);
-            idx[p] = (uint16_t)i;
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = tid + k * 256;
+            if (i < n) {
+                const int b = min(B - 1, (int)((dv[k] - dmin) * scale));
+                const int p = start[b] + atomicAdd(&fill[b], 1);
+                key[p] = (unsigned long long)__double_as_longlong(dv[k]);
+                idx[p] = (uint16_t)i;
+            }
         }
         __syncthreads();
         for (int b = tid; b < B; b += 256) {
