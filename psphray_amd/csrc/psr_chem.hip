@@ -528,13 +528,19 @@ template <int N>
 __device__ __forceinline__ void lu_solve(const double a[N * N], const int piv[N], double b[N]) {
 #pragma unroll
     for (int k = 0; k < N; k++) {
+        // the row exchange as selects over the candidate rows, so b stays in
+        // registers (an exchange written with the pivot as an index puts b
+        // in scratch memory)
+        const int p = piv[k];
+        const double bk = b[k];
+        double nk = bk;
 #pragma unroll
-        for (int q = k + 1; q < N; q++)
-            if (q == piv[k]) {
-                const double t = b[k];
-                b[k] = b[q];
-                b[q] = t;
-            }
+        for (int q = k + 1; q < N; q++) {
+            const bool sw = q == p;
+            nk = sw ? b[q] : nk;
+            b[q] = sw ? bk : b[q];
+        }
+        b[k] = nk;
 #pragma unroll
         for (int i = k + 1; i < N; i++) b[i] -= a[i * N + k] * b[k];
     }
@@ -814,6 +820,9 @@ __global__ __launch_bounds__(256) void k_cap_post(DevState S, int64_t npar, DevP
     r[MLP_IN + 3] = iso ? r[4] : mlp_lg(T1, 1e-30);
 }
 
+#ifndef CHEM_LB
+#define CHEM_LB __launch_bounds__(256)
+#endif
 #ifndef PHASE1_ATTEMPTS
 #define PHASE1_ATTEMPTS 1
 #endif
@@ -824,7 +833,7 @@ __global__ __launch_bounds__(256) void k_cap_post(DevState S, int64_t npar, DevP
 // their integrator state and are marked active = 2 for phase 2.  The split
 // bounds the wavefront divergence: step counts range from 1 to over 100.
 template <bool ISO, bool ADI, bool OTS, bool HONLY>
-__global__ __launch_bounds__(256) void k_update(DevState S, int64_t npar, const DevTick *__restrict__ tk,
+__global__ CHEM_LB void k_update(DevState S, int64_t npar, const DevTick *__restrict__ tk,
                                                 DevTables T, DevPhys P, DevStats *__restrict__ st,
                                                 double *__restrict__ saved, uint32_t *__restrict__ list,
                                                 uint32_t *__restrict__ nlist) {
@@ -890,7 +899,7 @@ __global__ __launch_bounds__(256) void k_update(DevState S, int64_t npar, const 
 // its idle lanes), and every iteration is one step attempt on every busy
 // lane.  nlist[0] is the list's length, nlist[1] the next unclaimed entry.
 template <bool ISO, bool ADI, bool OTS, bool HONLY>
-__global__ __launch_bounds__(256) void k_update_rest(DevState S, const uint32_t *__restrict__ list,
+__global__ CHEM_LB void k_update_rest(DevState S, const uint32_t *__restrict__ list,
                                                      uint32_t *__restrict__ nlist,
                                                      const DevTick *__restrict__ tk, DevTables T, DevPhys P,
                                                      DevStats *__restrict__ st, const double *__restrict__ saved) {
