@@ -64,8 +64,16 @@ __device__ __forceinline__ bool is_live(double em) { return signbit(em); }
 constexpr int FW = 3;                // wavefronts: walk, prep, apply
 constexpr int FTHREADS = 64 * FW;
 constexpr int FSEG = 128;            // hits per segment
-constexpr int FG = 8;                // checkpoint interval of the walk
-constexpr int FNCK = FSEG / FG;
+// checkpoint interval of the walk (hits per group): the H + He walk keeps
+// one group of 64-byte records in registers and its apply wave recomputes up
+// to FG - 1 steps per hit, so 4 (measured 256^3: 9.9 -> 9.3 ms per deposit);
+// the hydrogen walk's per-group work weighs more against its short steps, so
+// 8 (measured 128^3: 4 costs 152 -> 190 us)
+template <bool HE>
+struct Grp {
+    static constexpr int FG = HE ? 4 : 8;
+    static constexpr int FNCK = FSEG / FG;
+};
 
 // walk record of one hit (16-byte aligned: one LDS b128 read per pair)
 template <bool HE>
@@ -377,8 +385,8 @@ template <bool HE>
 struct DepLds {
     WalkRec<HE> rec[3][FSEG];
     HitAux<HE> aux[3][FSEG];
-    double ck[3][FNCK];       // TM before hit FG * g of the segment
-    uint8_t glive[3][FNCK];   // group g of the segment has a live hit
+    double ck[3][Grp<HE>::FNCK];      // TM before hit FG * g of the segment
+    uint8_t glive[3][Grp<HE>::FNCK];  // group g of the segment has a live hit
     int32_t stop_seg, stop_grp, last;
     double tm_end;
 };
@@ -458,6 +466,8 @@ __global__ __launch_bounds__(FTHREADS, HE ? 3 : 4) void k_deposit(const DevTick 
                                                       double *__restrict__ ray_len, double *__restrict__ ray_tm,
                                                       DevBins Bn, const int32_t *halt) {
     __shared__ DepLds<HE> L;
+    constexpr int FG = Grp<HE>::FG;
+    constexpr int FNCK = Grp<HE>::FNCK;
     if (halted(halt)) return;
     const psr_ray *__restrict__ rays = tk->rays;
     const DevState &S = C->S;
