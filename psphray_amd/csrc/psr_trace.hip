@@ -69,7 +69,7 @@ __global__ __launch_bounds__(TR_THREADS) void k_trace(
     // a tick's rays come through its tick record, parity tracing passes them
     const psr_ray *__restrict__ rays = tk ? tk->rays : rays_arg;
     __shared__ int32_t stack[TR_SCAP];
-    __shared__ int32_t leafq[TR_LCAP];
+    __shared__ int2 leafq[TR_LCAP];  // leaves to test: (first particle, count)
     __shared__ int32_t s_sp, s_lq, s_nh, s_cs, s_cl, s_ovf;
     if (halted(halt)) return;
     const int tid = threadIdx.x;
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(TR_THREADS) void k_trace(
             const Node nd = nodes[root];
             if (nd.b != 0 && box_hit(s, dir, len, nd)) {
                 if (nd.b < 0) stack[s_sp++] = nd.a;
-                else leafq[s_lq++] = root;
+                else leafq[s_lq++] = make_int2(nd.a, nd.b);
             }
         }
         for (;;) {
@@ -100,11 +100,12 @@ __global__ __launch_bounds__(TR_THREADS) void k_trace(
             if (lq >= 32 || sp == 0) {
                 const int nl = min(lq, 32);
                 const int t = tid >> 3, slot = tid & 7;
-                const int leaf = t < nl ? leafq[lq - nl + t] : -1;
-                if (leaf >= 0) {
-                    const Node L = nodes[leaf];
-                    if (slot < L.b) {
-                        const int k = L.a + slot;
+                // the leaf's particle range was queued with it, so the
+                // particle loads need no node load first
+                const int2 L = t < nl ? leafq[lq - nl + t] : make_int2(0, 0);
+                {
+                    if (slot < L.y) {
+                        const int k = L.x + slot;
                         const uint32_t o = orig[k];
                         if (!(o & PSR_INVALID_BIT)) {
                             // impact test of src/raytrace.c:171-186
@@ -150,7 +151,7 @@ __global__ __launch_bounds__(TR_THREADS) void k_trace(
                             else s_ovf = 1;
                         } else {
                             const int p = lq + atomicAdd(&s_cl, 1);
-                            leafq[p] = ch;
+                            leafq[p] = make_int2(C.a, C.b);
                         }
                     }
                 }
