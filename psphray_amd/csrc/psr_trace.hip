@@ -24,6 +24,7 @@
 #define TR_THREADS 256
 #define TR_SCAP 4096
 #define TR_LCAP 320
+#define TR_SORT 1024  // lists of up to this many hits are sorted by the trace
 
 // ------------------------------------------------------------ geometry
 // segment/AABB test of src/pluecker-sse.c:28-198 (see the oracle for the
@@ -59,118 +60,6 @@ __device__ __forceinline__ bool box_hit(const float s[3], const float dir[3], fl
         ok = ok && !(r0 < r1 || r2 > r3);
     }
     return ok;
-}
-
-__global__ __launch_bounds__(TR_THREADS) void k_trace(
-    const psr_ray *__restrict__ rays_arg, const DevTick *__restrict__ tk, int64_t nray,
-    const Node *__restrict__ nodes, const int32_t *__restrict__ froot, const float4 *__restrict__ ps,
-    const uint32_t *__restrict__ orig, DevHits H, uint8_t *__restrict__ active,
-    DevStats *__restrict__ st, const int32_t *halt) {
-    // a tick's rays come through its tick record, parity tracing passes them
-    const psr_ray *__restrict__ rays = tk ? tk->rays : rays_arg;
-    __shared__ int32_t stack[TR_SCAP];
-    __shared__ int2 leafq[TR_LCAP];  // leaves to test: (first particle, count)
-    __shared__ int32_t s_sp, s_lq, s_nh, s_cs, s_cl, s_ovf;
-    if (halted(halt)) return;
-    const int tid = threadIdx.x;
-    for (int64_t r = blockIdx.x; r < nray; r += gridDim.x) {
-        const psr_ray R = rays[r];
-        const float s[3] = {R.s[0], R.s[1], R.s[2]};
-        const float dir[3] = {R.dir[0], R.dir[1], R.dir[2]};
-        const float len = (float)R.length;
-        const int64_t base = r * (int64_t)H.cap;
-        if (tid == 0) {
-            s_sp = 0;
-            s_lq = 0;
-            s_nh = 0;
-            s_cs = 0;
-            s_cl = 0;
-            s_ovf = 0;
-            const int root = froot[R.field];
-            const Node nd = nodes[root];
-            if (nd.b != 0 && box_hit(s, dir, len, nd)) {
-                if (nd.b < 0) stack[s_sp++] = nd.a;
-                else leafq[s_lq++] = make_int2(nd.a, nd.b);
-            }
-        }
-        for (;;) {
-            __syncthreads();
-            const int sp = s_sp, lq = s_lq;
-            if (sp == 0 && lq == 0) break;
-            if (lq >= 32 || sp == 0) {
-                const int nl = min(lq, 32);
-                const int t = tid >> 3, slot = tid & 7;
-                // the leaf's particle range was queued with it, so the
-                // particle loads need no node load first
-                const int2 L = t < nl ? leafq[lq - nl + t] : make_int2(0, 0);
-                {
-                    if (slot < L.y) {
-                        const int k = L.x + slot;
-                        const uint32_t o = orig[k];
-                        if (!(o & PSR_INVALID_BIT)) {
-                            // impact test of src/raytrace.c:171-186
-                            const float4 p = ps[k];
-                            const float dx = p.x - s[0], dy = p.y - s[1], dz = p.z - s[2];
-                            const float px = dx * dir[0], py = dy * dir[1], pz = dz * dir[2];
-                            const float qx = dx * dx, qy = dy * dy, qz = dz * dz;
-                            double proj = (double)px;
-                            proj += (double)py;
-                            proj += (double)pz;
-                            double d2 = (double)qx;
-                            d2 += (double)qy;
-                            d2 += (double)qz;
-                            const double dist = sqrt(d2);
-                            const double r2 = (dist - proj) * (dist + proj);
-                            const float hh = p.w * p.w;
-                            if (!((double)hh < r2)) {
-                                const int h = atomicAdd(&s_nh, 1);
-                                if (h < H.cap) {
-                                    H.d[base + h] = dist;
-                                    H.b[base + h] = sqrt(fabs(r2));
-                                    H.slot[base + h] = (uint32_t)k;
-                                }
-                                active[k] = 1;
-                            }
-                        }
-                    }
-                }
-                __syncthreads();
-                if (tid == 0) s_lq = lq - nl;
-            } else {
-                const int nn = min(sp, 32);
-                const int t = tid >> 3, c = tid & 7;
-                const int fc = t < nn ? stack[sp - nn + t] : -1;
-                __syncthreads();
-                if (fc >= 0) {
-                    const int ch = fc + c;
-                    const Node C = nodes[ch];
-                    if (C.b != 0 && box_hit(s, dir, len, C)) {
-                        if (C.b < 0) {
-                            const int p = sp - nn + atomicAdd(&s_cs, 1);
-                            if (p < TR_SCAP) stack[p] = C.a;
-                            else s_ovf = 1;
-                        } else {
-                            const int p = lq + atomicAdd(&s_cl, 1);
-                            leafq[p] = make_int2(C.a, C.b);
-                        }
-                    }
-                }
-                __syncthreads();
-                if (tid == 0) {
-                    s_sp = min(sp - nn + s_cs, TR_SCAP);
-                    s_lq = lq + s_cl;
-                    s_cs = 0;
-                    s_cl = 0;
-                }
-            }
-        }
-        if (tid == 0) {
-            H.count[r] = s_nh;
-            if (s_nh > H.cap) atomicOr(&st->flags, PSR_DF_HITCAP);
-            if (s_ovf) atomicOr(&st->flags, PSR_DF_STACK);
-        }
-        __syncthreads();
-    }
 }
 
 // ------------------------------------------------------------ sort
@@ -212,110 +101,122 @@ __device__ __forceinline__ void block_minmax(double &mn, double &mx, double *red
 }
 
 template <int MAXN, int MAXB>
+struct SortLds {
+    unsigned long long key[MAXN];
+    uint16_t idx[MAXN];
+    int32_t start[MAXB];
+    int32_t fill[MAXB];
+    double red[8];
+    int32_t wsum[4];
+};
+
+// sorts the n hits of one ray at H[base ..] (n <= MAXN) with the 256
+// threads of the block: the permutation and the sorted slot / b columns
+template <int MAXN, int MAXB>
+__device__ __forceinline__ void sort_ray(int n, int64_t base, const DevHits &H, const uint32_t *__restrict__ orig,
+                                         SortLds<MAXN, MAXB> &L) {
+    constexpr int PER = MAXN / 256;  // distances per thread, held in registers
+    const int tid = threadIdx.x;
+    // one pass over the list: the range, the bucket counts and the
+    // scatter all use these registers
+    double dv[PER];
+    double mn = INFINITY, mx = -INFINITY;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = tid + k * 256;
+        dv[k] = i < n ? H.d[base + i] : 0.0;
+        if (i < n) {
+            mn = fmin(mn, dv[k]);
+            mx = fmax(mx, dv[k]);
+        }
+    }
+    block_minmax(mn, mx, L.red);
+    const double dmin = mn, dmax = mx;
+    int B = 1;
+    while (B * 4 < n && B < MAXB) B <<= 1;
+    const double scale = dmax > dmin ? (double)B / (dmax - dmin) : 0.0;
+    for (int b = tid; b < B; b += 256) {
+        L.start[b] = 0;
+        L.fill[b] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        if (tid + k * 256 < n) {
+            const int b = min(B - 1, (int)((dv[k] - dmin) * scale));
+            atomicAdd(&L.start[b], 1);
+        }
+    }
+    __syncthreads();
+    // exclusive scan of the counts: each thread owns a contiguous run
+    {
+        const int per = (B + 255) / 256;
+        const int b0 = tid * per, b1 = min(B, b0 + per);
+        int s = 0;
+        for (int b = b0; b < b1; b++) s += L.start[b];
+        int incl = s;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(incl, o, 64);
+            if ((tid & 63) >= o) incl += v;
+        }
+        if ((tid & 63) == 63) L.wsum[tid >> 6] = incl;
+        __syncthreads();
+        int off = 0;
+        for (int w = 0; w < (tid >> 6); w++) off += L.wsum[w];
+        int run = off + incl - s;
+        for (int b = b0; b < b1; b++) {
+            const int c = L.start[b];
+            L.start[b] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = tid + k * 256;
+        if (i < n) {
+            const int b = min(B - 1, (int)((dv[k] - dmin) * scale));
+            const int p = L.start[b] + atomicAdd(&L.fill[b], 1);
+            L.key[p] = (unsigned long long)__double_as_longlong(dv[k]);
+            L.idx[p] = (uint16_t)i;
+        }
+    }
+    __syncthreads();
+    for (int b = tid; b < B; b += 256) {
+        const int s0 = L.start[b], e0 = s0 + L.fill[b];
+        for (int j = s0 + 1; j < e0; j++) {
+            const unsigned long long kj = L.key[j];
+            const uint16_t ij = L.idx[j];
+            int q = j - 1;
+            while (q >= s0 && hit_less(kj, L.key[q], ij, L.idx[q], H, base, orig)) {
+                L.key[q + 1] = L.key[q];
+                L.idx[q + 1] = L.idx[q];
+                q--;
+            }
+            L.key[q + 1] = kj;
+            L.idx[q + 1] = ij;
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += 256) {
+        const uint32_t q = L.idx[i];
+        H.perm[base + i] = q;
+        H.sslot[base + i] = H.slot[base + q];
+        H.sb[base + i] = H.b[base + q];
+    }
+    __syncthreads();
+}
+
+template <int MAXN, int MAXB>
 __global__ __launch_bounds__(256) void k_sort(int64_t nray, DevHits H, const uint32_t *__restrict__ orig,
                                               int32_t lo, int32_t hi, const int32_t *halt) {
-    constexpr int PER = MAXN / 256;  // distances per thread, held in registers
-    __shared__ unsigned long long key[MAXN];
-    __shared__ uint16_t idx[MAXN];
-    __shared__ int32_t start[MAXB];
-    __shared__ int32_t fill[MAXB];
-    __shared__ double red[8];
-    __shared__ int32_t wsum[4];
+    __shared__ SortLds<MAXN, MAXB> L;
     if (halted(halt)) return;
-    const int tid = threadIdx.x;
     for (int64_t r = blockIdx.x; r < nray; r += gridDim.x) {
         const int n = H.count[r];
         if (n <= lo || n > hi) continue;
-        const int64_t base = r * (int64_t)H.cap;
-        // one pass over the list: the range, the bucket counts and the
-        // scatter all use these registers
-        double dv[PER];
-        double mn = INFINITY, mx = -INFINITY;
-#pragma unroll
-        for (int k = 0; k < PER; k++) {
-            const int i = tid + k * 256;
-            dv[k] = i < n ? H.d[base + i] : 0.0;
-            if (i < n) {
-                mn = fmin(mn, dv[k]);
-                mx = fmax(mx, dv[k]);
-            }
-        }
-        block_minmax(mn, mx, red);
-        const double dmin = mn, dmax = mx;
-        int B = 1;
-        while (B * 4 < n && B < MAXB) B <<= 1;
-        const double scale = dmax > dmin ? (double)B / (dmax - dmin) : 0.0;
-        for (int b = tid; b < B; b += 256) {
-            start[b] = 0;
-            fill[b] = 0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < PER; k++) {
-            if (tid + k * 256 < n) {
-                const int b = min(B - 1, (int)((dv[k] - dmin) * scale));
-                atomicAdd(&start[b], 1);
-            }
-        }
-        __syncthreads();
-        // exclusive scan of the counts: each thread owns a contiguous run
-        {
-            const int per = (B + 255) / 256;
-            const int b0 = tid * per, b1 = min(B, b0 + per);
-            int s = 0;
-            for (int b = b0; b < b1; b++) s += start[b];
-            int incl = s;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const int v = __shfl_up(incl, o, 64);
-                if ((tid & 63) >= o) incl += v;
-            }
-            if ((tid & 63) == 63) wsum[tid >> 6] = incl;
-            __syncthreads();
-            int off = 0;
-            for (int w = 0; w < (tid >> 6); w++) off += wsum[w];
-            int run = off + incl - s;
-            for (int b = b0; b < b1; b++) {
-                const int c = start[b];
-                start[b] = run;
-                run += c;
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < PER; k++) {
-            const int i = tid + k * 256;
-            if (i < n) {
-                const int b = min(B - 1, (int)((dv[k] - dmin) * scale));
-                const int p = start[b] + atomicAdd(&fill[b], 1);
-                key[p] = (unsigned long long)__double_as_longlong(dv[k]);
-                idx[p] = (uint16_t)i;
-            }
-        }
-        __syncthreads();
-        for (int b = tid; b < B; b += 256) {
-            const int s0 = start[b], e0 = s0 + fill[b];
-            for (int j = s0 + 1; j < e0; j++) {
-                const unsigned long long kj = key[j];
-                const uint16_t ij = idx[j];
-                int q = j - 1;
-                while (q >= s0 && hit_less(kj, key[q], ij, idx[q], H, base, orig)) {
-                    key[q + 1] = key[q];
-                    idx[q + 1] = idx[q];
-                    q--;
-                }
-                key[q + 1] = kj;
-                idx[q + 1] = ij;
-            }
-        }
-        __syncthreads();
-        for (int i = tid; i < n; i += 256) {
-            const uint32_t q = idx[i];
-            H.perm[base + i] = q;
-            H.sslot[base + i] = H.slot[base + q];
-            H.sb[base + i] = H.b[base + q];
-        }
-        __syncthreads();
+        sort_ray<MAXN, MAXB>(n, r * (int64_t)H.cap, H, orig, L);
     }
 }
 
@@ -419,6 +320,134 @@ __global__ void k_export_lists(int64_t nray, DevHits H, const uint32_t *__restri
     }
 }
 
+
+// 8 waves per SIMD (64 VGPRs): with its sort the trace would take 66 and 7
+__global__ __launch_bounds__(TR_THREADS, 8) void k_trace(
+    const psr_ray *__restrict__ rays_arg, const DevTick *__restrict__ tk, int64_t nray,
+    const Node *__restrict__ nodes, const int32_t *__restrict__ froot, const float4 *__restrict__ ps,
+    const uint32_t *__restrict__ orig, DevHits H, uint8_t *__restrict__ active,
+    DevStats *__restrict__ st, const int32_t *halt) {
+    // a tick's rays come through its tick record, parity tracing passes them
+    const psr_ray *__restrict__ rays = tk ? tk->rays : rays_arg;
+    // the traversal's stack and leaf queue, then (same memory) the sort of a
+    // list of up to TR_SORT hits
+    union TraceLds {
+        struct {
+            int32_t stack[TR_SCAP];
+            int2 leafq[TR_LCAP];  // leaves to test: (first particle, count)
+        } t;
+        SortLds<TR_SORT, TR_SORT / 4> s;
+    };
+    __shared__ TraceLds U;
+    int32_t *const stack = U.t.stack;
+    int2 *const leafq = U.t.leafq;
+    __shared__ int32_t s_sp, s_lq, s_nh, s_cs, s_cl, s_ovf;
+    if (halted(halt)) return;
+    const int tid = threadIdx.x;
+    for (int64_t r = blockIdx.x; r < nray; r += gridDim.x) {
+        const psr_ray R = rays[r];
+        const float s[3] = {R.s[0], R.s[1], R.s[2]};
+        const float dir[3] = {R.dir[0], R.dir[1], R.dir[2]};
+        const float len = (float)R.length;
+        const int64_t base = r * (int64_t)H.cap;
+        if (tid == 0) {
+            s_sp = 0;
+            s_lq = 0;
+            s_nh = 0;
+            s_cs = 0;
+            s_cl = 0;
+            s_ovf = 0;
+            const int root = froot[R.field];
+            const Node nd = nodes[root];
+            if (nd.b != 0 && box_hit(s, dir, len, nd)) {
+                if (nd.b < 0) stack[s_sp++] = nd.a;
+                else leafq[s_lq++] = make_int2(nd.a, nd.b);
+            }
+        }
+        for (;;) {
+            __syncthreads();
+            const int sp = s_sp, lq = s_lq;
+            if (sp == 0 && lq == 0) break;
+            if (lq >= 32 || sp == 0) {
+                const int nl = min(lq, 32);
+                const int t = tid >> 3, slot = tid & 7;
+                // the leaf's particle range was queued with it, so the
+                // particle loads need no node load first
+                const int2 L = t < nl ? leafq[lq - nl + t] : make_int2(0, 0);
+                {
+                    if (slot < L.y) {
+                        const int k = L.x + slot;
+                        const uint32_t o = orig[k];
+                        if (!(o & PSR_INVALID_BIT)) {
+                            // impact test of src/raytrace.c:171-186
+                            const float4 p = ps[k];
+                            const float dx = p.x - s[0], dy = p.y - s[1], dz = p.z - s[2];
+                            const float px = dx * dir[0], py = dy * dir[1], pz = dz * dir[2];
+                            const float qx = dx * dx, qy = dy * dy, qz = dz * dz;
+                            double proj = (double)px;
+                            proj += (double)py;
+                            proj += (double)pz;
+                            double d2 = (double)qx;
+                            d2 += (double)qy;
+                            d2 += (double)qz;
+                            const double dist = sqrt(d2);
+                            const double r2 = (dist - proj) * (dist + proj);
+                            const float hh = p.w * p.w;
+                            if (!((double)hh < r2)) {
+                                const int h = atomicAdd(&s_nh, 1);
+                                if (h < H.cap) {
+                                    H.d[base + h] = dist;
+                                    H.b[base + h] = sqrt(fabs(r2));
+                                    H.slot[base + h] = (uint32_t)k;
+                                }
+                                active[k] = 1;
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+                if (tid == 0) s_lq = lq - nl;
+            } else {
+                const int nn = min(sp, 32);
+                const int t = tid >> 3, c = tid & 7;
+                const int fc = t < nn ? stack[sp - nn + t] : -1;
+                __syncthreads();
+                if (fc >= 0) {
+                    const int ch = fc + c;
+                    const Node C = nodes[ch];
+                    if (C.b != 0 && box_hit(s, dir, len, C)) {
+                        if (C.b < 0) {
+                            const int p = sp - nn + atomicAdd(&s_cs, 1);
+                            if (p < TR_SCAP) stack[p] = C.a;
+                            else s_ovf = 1;
+                        } else {
+                            const int p = lq + atomicAdd(&s_cl, 1);
+                            leafq[p] = make_int2(C.a, C.b);
+                        }
+                    }
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    s_sp = min(sp - nn + s_cs, TR_SCAP);
+                    s_lq = lq + s_cl;
+                    s_cs = 0;
+                    s_cl = 0;
+                }
+            }
+        }
+        const int nh = s_nh;
+        if (tid == 0) {
+            H.count[r] = nh;
+            if (nh > H.cap) atomicOr(&st->flags, PSR_DF_HITCAP);
+            if (s_ovf) atomicOr(&st->flags, PSR_DF_STACK);
+        }
+        // a short list is sorted here, while it is in the cache, by the
+        // threads that found it; longer ones by k_sort's tiers
+        if (nh > 0 && nh <= TR_SORT) sort_ray<TR_SORT, TR_SORT / 4>(nh, base, H, orig, U.s);
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------ launchers
 int psr_ensure_hits(psr_ctx *c, int64_t nray, int32_t cap) {
     if (nray <= c->hit_slots && cap <= c->H.cap && c->H.d) return PSR_OK;
@@ -514,13 +543,11 @@ int psr_launch_sort(psr_ctx *c, int64_t nray) {
     const unsigned grid = (unsigned)std::min<int64_t>(nray, 65535 * 4);
     const int32_t *halt = &c->d_stats->flags;
     PSR_EV(c, 1, 0);
-    // LDS tiers by list length: 10 B per hit + 8 B per bucket (a wavefront-per-
-    // ray tier for short lists saved 15 us on the batched fields and cost 20 us
-    // on 128^3, its extra launch included, so there is none)
-    k_sort<1024, 256><<<grid, 256, 0, c->stream>>>(nray, c->H, c->d_orig, -1, 1024, halt);
-    PSR_HIP(c, hipGetLastError());
-    if (c->H.cap > 1024) {
-        k_sort<4096, 1024><<<grid, 256, 0, c->stream>>>(nray, c->H, c->d_orig, 1024, 4096, halt);
+    // lists of up to TR_SORT hits were sorted by the trace; the longer ones
+    // here, in LDS tiers by list length (10 B per hit + 8 B per bucket)
+    static_assert(TR_SORT == 1024, "the first tier starts above the trace's sort");
+    if (c->H.cap > TR_SORT) {
+        k_sort<4096, 1024><<<grid, 256, 0, c->stream>>>(nray, c->H, c->d_orig, TR_SORT, 4096, halt);
         PSR_HIP(c, hipGetLastError());
     }
     if (c->H.cap > 4096) {
