@@ -123,6 +123,8 @@ def main():
     ap.add_argument("--train-ticks", type=int, default=8,
                     help="exact ticks whose steps train the surrogate (--chem surrogate)")
     ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--mlp-residual", type=float, default=None,
+                    help="surrogate residual threshold (--chem surrogate; library default 1e-4)")
     ap.add_argument("--accum", choices=["auto", "atomic", "binned"], default="auto",
                     help="deposit accumulation (auto: timed on the first ticks)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -201,6 +203,8 @@ def main():
         loss = ctx.mlp_train(n=0, epochs=args.epochs, batch=8192, lr=2e-3, seed=1)
         t_train = time.perf_counter() - t0
         ctx.mlp_capture(0)
+        if args.mlp_residual is not None:
+            ctx.mlp_set_residual(args.mlp_residual)
         log(f"surrogate: {ntup} tuples, {args.epochs} epochs in {t_train:.2f}s, loss {loss:.3e}")
         # the plain method over the timed ticks, from the same state
         snap = wl.ps.copy()
@@ -214,6 +218,7 @@ def main():
         ctx.state_download(ex)
         ctx.state_upload(snap)
         sur = {"tuples": int(ntup), "epochs": args.epochs, "train_s": t_train, "train_loss": loss,
+               "residual": args.mlp_residual if args.mlp_residual is not None else 1e-4,
                "exact_ms_per_step": wall_exact / args.steps * 1e3}
     ctx.sync()
     ctx.reset_stats()
