@@ -117,6 +117,54 @@ def test_trace_lists_bit_exact(ctx, cfg, n, nray):
     assert np.array_equal(b, oh["b"])
 
 
+def long_ray_field():
+    """a 64^3 lattice with smoothing lengths tripled and diagonal rays through
+    it: about 11,800 hits per ray, past the LDS sort tiers (8192), beside
+    the workload's ordinary rays"""
+    wl = make_workload(1, RATES, n=64, nray=24)
+    wl.ps.sml[:] *= np.float32(3.0)
+    box = wl.boxsize
+    r = wl.rays(1)
+    rng = np.random.default_rng(7)
+    d = np.array([1.0, 1.0, 1.0]) + rng.normal(0, 0.02, (8, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    r[:8]["s"] = np.float32(0.01 * box)
+    r[:8]["dir"] = d.astype(np.float32)
+    r[:8]["length"] = box * np.sqrt(3.0)
+    return wl, r
+
+
+def test_lists_longer_than_the_lds_sort(ctx):
+    """lists of more than 8192 hits take the global-memory bitonic sort:
+    bit-exact lists, then one whole tick against the oracle"""
+    print_case("long lists")
+    wl, rays = long_ray_field()
+    load(ctx, wl)
+    ctx.rays_upload(rays)
+    off, ip, d, b = ctx.trace_lists()
+    t = po.Tree(wl.ps.pos, wl.ps.sml, wl.boxsize)
+    ooff, oh = po.trace_rays(t, wl.ps, rays)
+    assert np.diff(ooff).max() > 8192 and np.diff(ooff).min() < 4096
+    assert np.array_equal(off, ooff)
+    assert np.array_equal(ip, oh["ipar"])
+    assert np.array_equal(d, oh["d"])
+    assert np.array_equal(b, oh["b"])
+    orc = oracle_for(wl)
+    ctx.reset_stats()
+    ctx.tick(1, wl.tick_time)
+    out = wl.ps.copy()
+    ctx.state_download(out)
+    ref = wl.ps.copy()
+    po.deposit(orc, ref, rays, ooff, oh)
+    act = po.merge(ref, ooff, oh)
+    rc, _ = po.update(orc, ref, act, 1, wl.tick_time)
+    assert rc == 0
+    assert ctx.stats()["total_deposit_count"] == ooff[-1]
+    assert np.array_equal(out.hits, ref.hits)
+    assert np.array_equal(out.lasthit, ref.lasthit)
+    assert np.abs(out.lambdaH - ref.lambdaH).max() < 1e-6
+
+
 def test_trace_batched_fields(ctx):
     print_case("trace batched")
     wl = make_workload(5, RATES, nfield=6, nray=64)
