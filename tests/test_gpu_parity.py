@@ -178,6 +178,43 @@ def test_trace_batched_fields(ctx):
     assert np.array_equal(d, oh["d"])
 
 
+def test_empty_field_and_empty_tick(ctx):
+    """batched fields with one field emptied: its rays find nothing, the
+    others' lists are the oracle's; a tick with no rays leaves the state as
+    it was; a whole tick then counts every listed hit"""
+    print_case("empty field / empty tick")
+    wl = make_workload(5, RATES, nfield=4, nray=32)
+    fo = np.asarray(wl.field_offsets, np.int64)
+    keep = np.ones(wl.ps.npar, bool)
+    keep[fo[1]:fo[2]] = False
+    ps = psr.ParticleSystem(int(keep.sum()))
+    for f in ps.fields():
+        setattr(ps, f, np.ascontiguousarray(getattr(wl.ps, f)[keep]))
+    fo2 = fo.copy()
+    fo2[2:] -= fo[2] - fo[1]
+    wl.ps, wl.field_offsets = ps, fo2
+    load(ctx, wl)
+    rays = wl.rays(1)
+    ctx.rays_upload(rays)
+    off, ip, d, b = ctx.trace_lists()
+    empty = rays["field"] == 1
+    assert empty.any()
+    assert (np.diff(off)[empty] == 0).all()
+    ooff, oh = oracle_lists(wl, rays[~empty])
+    assert off[-1] > 0 and np.array_equal(np.diff(off)[~empty], np.diff(ooff))
+    assert np.array_equal(ip, oh["ipar"]) and np.array_equal(d, oh["d"]) and np.array_equal(b, oh["b"])
+    before = wl.ps.copy()
+    ctx.state_download(before)
+    ctx.tick(1, wl.tick_time, ray_first=0, nray=0)
+    after = wl.ps.copy()
+    ctx.state_download(after)
+    for f in before.fields():
+        assert np.array_equal(getattr(before, f), getattr(after, f)), f
+    ctx.reset_stats()
+    ctx.tick(2, wl.tick_time)
+    assert ctx.stats()["total_deposit_count"] == off[-1]
+
+
 def test_trace_invalid_and_outside_particles(ctx):
     print_case("trace flags")
     wl = make_workload(1, RATES, n=16, nray=64)
