@@ -506,6 +506,45 @@ def test_full_tick_parity(ctx, cfg, n, nray):
     assert (out.yGdepHI == 0).all() and (out.heat == 0).all()
 
 
+FLAG_CASES = {
+    "h-caseA-iso": (1, dict(isothermal=True, on_the_spot=False, h_only=True)),
+    "he-caseA-T": (3, dict(isothermal=False, on_the_spot=False, h_only=False)),
+    "he-adiabatic": (3, dict(isothermal=False, adiabatic=True, h_only=False)),
+    "he-comoving-z3": (3, dict(isothermal=False, h_only=False, comoving=True, redshift=3.0)),
+    "h-fake-T": (1, dict(isothermal=True, h_only=True, fake_temperature=2.0e4)),
+    "he-no-secondary": (3, dict(isothermal=False, h_only=False, secondary_ionization=False)),
+}
+
+
+@pytest.mark.parametrize("name", list(FLAG_CASES))
+def test_full_tick_parity_flags(ctx, name):
+    """the other run flags (case-A recombination with its counters, frozen
+    energy, comoving scaling, a fixed temperature, no secondary ionization):
+    a whole tick against the oracle"""
+    print_case("tick flags", name)
+    cfg, kw = FLAG_CASES[name]
+    wl = make_workload(cfg, RATES, n=16, nray=64)
+    u = wl.physics.units
+    base = dict(isothermal=True, on_the_spot=True, h_only=True, secondary_ionization=True,
+                comoving=False)
+    base.update(kw)
+    wl.physics = psr.Physics(RATES, u, psr.make_flags(**base), with_secion=not base["h_only"])
+    orc = oracle_for(wl)
+    load(ctx, wl)
+    out, ref, rays, _, off = run_one_tick(ctx, wl, orc, psr.PSR_PHASE_ALL)
+    assert np.array_equal(out.hits, ref.hits)
+    assert np.array_equal(out.lasthit, ref.lasthit)
+    fr = ("lambdaH",) if base["h_only"] else ("lambdaH", "lambdaHeI", "lambdaHeII")
+    for f in fr + ("yGrecHII",):
+        dx = np.abs(getattr(out, f).astype(np.float64) - getattr(ref, f).astype(np.float64))
+        scale = 1.0 if f.startswith("lambda") else max(np.abs(getattr(ref, f)).max(), 1e-30)
+        assert dx.max() / scale < 1e-3 and np.median(dx) / scale < 1e-6, f
+    if base.get("adiabatic"):
+        assert np.array_equal(out.ie, ref.ie)
+    else:
+        assert np.median(np.abs(out.ie / ref.ie - 1)) < 1e-6
+
+
 def test_solve_steps_matches_oracle(ctx, golden):
     print_case("solve_steps")
     c = np.load(f"{golden}/chem_lsoda.npz")
