@@ -215,6 +215,32 @@ def test_empty_field_and_empty_tick(ctx):
     assert ctx.stats()["total_deposit_count"] == off[-1]
 
 
+def test_trace_only_tick(ctx):
+    """a tick with the trace phase alone (traceOnly, src/loop.c:131-148):
+    the hits are counted, the particles are left as they were and the
+    active set is dropped, so a whole tick afterwards equals one from the
+    start"""
+    print_case("trace-only tick")
+    wl = make_workload(2, RATES, n=24, nray=256)
+    orc = oracle_for(wl)
+    load(ctx, wl)
+    rays = wl.rays(1)
+    ctx.rays_upload(rays)
+    ctx.reset_stats()
+    ctx.tick(1, wl.tick_time, phases=psr.PSR_PHASE_TRACE)
+    s = ctx.stats()
+    t = po.Tree(wl.ps.pos, wl.ps.sml, wl.boxsize)
+    ooff, _ = po.trace_rays(t, wl.ps, rays)
+    assert s["total_deposit_count"] == ooff[-1] and s["active_count"] == 0
+    mid = wl.ps.copy()
+    ctx.state_download(mid)
+    for f in mid.fields():
+        assert np.array_equal(getattr(mid, f), getattr(wl.ps, f)), f
+    out, ref, _, _, _ = run_one_tick(ctx, wl, orc, psr.PSR_PHASE_ALL)
+    assert np.array_equal(out.lasthit, ref.lasthit)
+    assert np.abs(out.lambdaH - ref.lambdaH).max() < 1e-6
+
+
 def test_trace_invalid_and_outside_particles(ctx):
     print_case("trace flags")
     wl = make_workload(1, RATES, n=16, nray=64)
