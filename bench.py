@@ -30,8 +30,10 @@ BYTES_DEPOSIT_H = 36
 BYTES_DEPOSIT_HE = 68
 # HBM bytes per k_deposit launch from rocprofv3 FETCH_SIZE + WRITE_SIZE
 # passes, scaled by the same counters on a copy of known size (FETCH_SIZE x 2,
-# WRITE_SIZE x 1): 85,049 KiB x 2 + 50,911 KiB for workload 2
-TRAFFIC_DEPOSIT = {2: (84968 * 2 + 50806) * 1024}
+# WRITE_SIZE x 1), the timed-tick dispatch of each workload (the 128^3,
+# 256^3 and batched-field traffic summaries under profiles/)
+TRAFFIC_DEPOSIT = {2: (84968 * 2 + 50806) * 1024, 3: (5984385 * 2 + 4237966) * 1024,
+                   5: (151006 * 2 + 92589) * 1024}
 WORKLOADS = {
     1: "32^3 jittered lattice, H-only, isothermal 1e4 K, one source",
     2: "128^3 lognormal, H-only, isothermal 1e4 K, 64 sources",
