@@ -374,35 +374,36 @@ __global__ __launch_bounds__(TR_THREADS, 8) void k_trace(
                 // the leaf's particle range was queued with it, so the
                 // particle loads need no node load first
                 const int2 L = t < nl ? leafq[lq - nl + t] : make_int2(0, 0);
-                {
-                    if (slot < L.y) {
-                        const int k = L.x + slot;
-                        const uint32_t o = orig[k];
-                        if (!(o & PSR_INVALID_BIT)) {
-                            // impact test of src/raytrace.c:171-186
-                            const float4 p = ps[k];
-                            const float dx = p.x - s[0], dy = p.y - s[1], dz = p.z - s[2];
-                            const float px = dx * dir[0], py = dy * dir[1], pz = dz * dir[2];
-                            const float qx = dx * dx, qy = dy * dy, qz = dz * dz;
-                            double proj = (double)px;
-                            proj += (double)py;
-                            proj += (double)pz;
-                            double d2 = (double)qx;
-                            d2 += (double)qy;
-                            d2 += (double)qz;
-                            const double dist = sqrt(d2);
-                            const double r2 = (dist - proj) * (dist + proj);
-                            const float hh = p.w * p.w;
-                            if (!((double)hh < r2)) {
-                                const int h = atomicAdd(&s_nh, 1);
-                                if (h < H.cap) {
-                                    H.d[base + h] = dist;
-                                    H.b[base + h] = sqrt(fabs(r2));
-                                    H.slot[base + h] = (uint32_t)k;
-                                }
-                                active[k] = 1;
-                            }
+                const int k = L.x + slot;
+                if (slot < L.y) {
+                    // the flag and the particle are loaded together and the
+                    // test is computed for both (its result is dropped for an
+                    // invalid particle), so one memory round trip serves both
+                    const uint32_t o = orig[k];
+                    const float4 p = ps[k];
+                    // impact test of src/raytrace.c:171-186
+                    const float dx = p.x - s[0], dy = p.y - s[1], dz = p.z - s[2];
+                    const float px = dx * dir[0], py = dy * dir[1], pz = dz * dir[2];
+                    const float qx = dx * dx, qy = dy * dy, qz = dz * dz;
+                    double proj = (double)px;
+                    proj += (double)py;
+                    proj += (double)pz;
+                    double d2 = (double)qx;
+                    d2 += (double)qy;
+                    d2 += (double)qz;
+                    const double dist = sqrt(d2);
+                    const double r2 = (dist - proj) * (dist + proj);
+                    const float hh = p.w * p.w;
+                    const bool valid = !(o & PSR_INVALID_BIT);
+                    const bool inside = !((double)hh < r2);
+                    if (valid & inside) {  // not &&: no branch between the two loads
+                        const int h = atomicAdd(&s_nh, 1);
+                        if (h < H.cap) {
+                            H.d[base + h] = dist;
+                            H.b[base + h] = sqrt(fabs(r2));
+                            H.slot[base + h] = (uint32_t)k;
                         }
+                        active[k] = 1;
                     }
                 }
                 __syncthreads();
