@@ -25,6 +25,11 @@
 #define TR_SCAP 4096
 #define TR_LCAP 320
 #define TR_SORT 1024  // lists of up to this many hits are sorted by the trace
+// threads per ray of the k_sort tiers: their LDS allows 3 (4096 hits) and 1
+// (8192) workgroups per CU, so more threads per ray fill the CU (256^3 sort
+// phase: 2.17 ms at 256 and 256, 1.67 at 512 and 512, 3.4 at 1024 and 1024)
+#define SORT_NT4 512
+#define SORT_NT8 512
 
 // ------------------------------------------------------------ geometry
 // segment/AABB test of src/pluecker-sse.c:28-198 (see the oracle for the
@@ -77,8 +82,10 @@ __device__ __forceinline__ bool hit_less(unsigned long long ka, unsigned long lo
     return pa < pb;
 }
 
-// the range [min, max] of one value per thread over the 256-thread block
+// the range [min, max] of one value per thread over the NT-thread block
+template <int NT>
 __device__ __forceinline__ void block_minmax(double &mn, double &mx, double *red) {
+    constexpr int NW = NT / 64;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         mn = fmin(mn, __shfl_xor(mn, o, 64));
@@ -88,34 +95,35 @@ __device__ __forceinline__ void block_minmax(double &mn, double &mx, double *red
     __syncthreads();
     if ((threadIdx.x & 63) == 0) {
         red[wid] = mn;
-        red[4 + wid] = mx;
+        red[NW + wid] = mx;
     }
     __syncthreads();
     mn = red[0];
-    mx = red[4];
+    mx = red[NW];
 #pragma unroll
-    for (int k = 1; k < 4; k++) {
+    for (int k = 1; k < NW; k++) {
         mn = fmin(mn, red[k]);
-        mx = fmax(mx, red[4 + k]);
+        mx = fmax(mx, red[NW + k]);
     }
 }
 
-template <int MAXN, int MAXB>
+template <int MAXN, int MAXB, int NT>
 struct SortLds {
     unsigned long long key[MAXN];
     uint16_t idx[MAXN];
     int32_t start[MAXB];
     int32_t fill[MAXB];
-    double red[8];
-    int32_t wsum[4];
+    double red[2 * (NT / 64)];
+    int32_t wsum[NT / 64];
 };
 
-// sorts the n hits of one ray at H[base ..] (n <= MAXN) with the 256
+// sorts the n hits of one ray at H[base ..] (n <= MAXN) with the NT
 // threads of the block: the permutation and the sorted slot / b columns
-template <int MAXN, int MAXB>
+template <int MAXN, int MAXB, int NT>
 __device__ __forceinline__ void sort_ray(int n, int64_t base, const DevHits &H, const uint32_t *__restrict__ orig,
-                                         SortLds<MAXN, MAXB> &L) {
-    constexpr int PER = MAXN / 256;  // distances per thread, held in registers
+                                         SortLds<MAXN, MAXB, NT> &L) {
+    static_assert(MAXN % NT == 0 && NT % 64 == 0, "whole wavefronts, whole rows of distances");
+    constexpr int PER = MAXN / NT;  // distances per thread, held in registers
     const int tid = threadIdx.x;
     // one pass over the list: the range, the bucket counts and the
     // scatter all use these registers
@@ -123,26 +131,26 @@ __device__ __forceinline__ void sort_ray(int n, int64_t base, const DevHits &H, 
     double mn = INFINITY, mx = -INFINITY;
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-        const int i = tid + k * 256;
+        const int i = tid + k * NT;
         dv[k] = i < n ? H.d[base + i] : 0.0;
         if (i < n) {
             mn = fmin(mn, dv[k]);
             mx = fmax(mx, dv[k]);
         }
     }
-    block_minmax(mn, mx, L.red);
+    block_minmax<NT>(mn, mx, L.red);
     const double dmin = mn, dmax = mx;
     int B = 1;
     while (B * 4 < n && B < MAXB) B <<= 1;
     const double scale = dmax > dmin ? (double)B / (dmax - dmin) : 0.0;
-    for (int b = tid; b < B; b += 256) {
+    for (int b = tid; b < B; b += NT) {
         L.start[b] = 0;
         L.fill[b] = 0;
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-        if (tid + k * 256 < n) {
+        if (tid + k * NT < n) {
             const int b = min(B - 1, (int)((dv[k] - dmin) * scale));
             atomicAdd(&L.start[b], 1);
         }
@@ -150,7 +158,7 @@ __device__ __forceinline__ void sort_ray(int n, int64_t base, const DevHits &H, 
     __syncthreads();
     // exclusive scan of the counts: each thread owns a contiguous run
     {
-        const int per = (B + 255) / 256;
+        const int per = (B + NT - 1) / NT;
         const int b0 = tid * per, b1 = min(B, b0 + per);
         int s = 0;
         for (int b = b0; b < b1; b++) s += L.start[b];
@@ -174,7 +182,7 @@ __device__ __forceinline__ void sort_ray(int n, int64_t base, const DevHits &H, 
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-        const int i = tid + k * 256;
+        const int i = tid + k * NT;
         if (i < n) {
             const int b = min(B - 1, (int)((dv[k] - dmin) * scale));
             const int p = L.start[b] + atomicAdd(&L.fill[b], 1);
@@ -183,7 +191,7 @@ __device__ __forceinline__ void sort_ray(int n, int64_t base, const DevHits &H, 
         }
     }
     __syncthreads();
-    for (int b = tid; b < B; b += 256) {
+    for (int b = tid; b < B; b += NT) {
         const int s0 = L.start[b], e0 = s0 + L.fill[b];
         for (int j = s0 + 1; j < e0; j++) {
             const unsigned long long kj = L.key[j];
@@ -199,7 +207,7 @@ __device__ __forceinline__ void sort_ray(int n, int64_t base, const DevHits &H, 
         }
     }
     __syncthreads();
-    for (int i = tid; i < n; i += 256) {
+    for (int i = tid; i < n; i += NT) {
         const uint32_t q = L.idx[i];
         H.perm[base + i] = q;
         H.sslot[base + i] = H.slot[base + q];
@@ -208,15 +216,15 @@ __device__ __forceinline__ void sort_ray(int n, int64_t base, const DevHits &H, 
     __syncthreads();
 }
 
-template <int MAXN, int MAXB>
-__global__ __launch_bounds__(256) void k_sort(int64_t nray, DevHits H, const uint32_t *__restrict__ orig,
-                                              int32_t lo, int32_t hi, const int32_t *halt) {
-    __shared__ SortLds<MAXN, MAXB> L;
+template <int MAXN, int MAXB, int NT>
+__global__ __launch_bounds__(NT) void k_sort(int64_t nray, DevHits H, const uint32_t *__restrict__ orig,
+                                             int32_t lo, int32_t hi, const int32_t *halt) {
+    __shared__ SortLds<MAXN, MAXB, NT> L;
     if (halted(halt)) return;
     for (int64_t r = blockIdx.x; r < nray; r += gridDim.x) {
         const int n = H.count[r];
         if (n <= lo || n > hi) continue;
-        sort_ray<MAXN, MAXB>(n, r * (int64_t)H.cap, H, orig, L);
+        sort_ray<MAXN, MAXB, NT>(n, r * (int64_t)H.cap, H, orig, L);
     }
 }
 
@@ -336,7 +344,7 @@ __global__ __launch_bounds__(TR_THREADS, 8) void k_trace(
             int32_t stack[TR_SCAP];
             int2 leafq[TR_LCAP];  // leaves to test: (first particle, count)
         } t;
-        SortLds<TR_SORT, TR_SORT / 4> s;
+        SortLds<TR_SORT, TR_SORT / 4, TR_THREADS> s;
     };
     __shared__ TraceLds U;
     int32_t *const stack = U.t.stack;
@@ -444,7 +452,7 @@ __global__ __launch_bounds__(TR_THREADS, 8) void k_trace(
         }
         // a short list is sorted here, while it is in the cache, by the
         // threads that found it; longer ones by k_sort's tiers
-        if (nh > 0 && nh <= TR_SORT) sort_ray<TR_SORT, TR_SORT / 4>(nh, base, H, orig, U.s);
+        if (nh > 0 && nh <= TR_SORT) sort_ray<TR_SORT, TR_SORT / 4, TR_THREADS>(nh, base, H, orig, U.s);
         __syncthreads();
     }
 }
@@ -548,11 +556,11 @@ int psr_launch_sort(psr_ctx *c, int64_t nray) {
     // here, in LDS tiers by list length (10 B per hit + 8 B per bucket)
     static_assert(TR_SORT == 1024, "the first tier starts above the trace's sort");
     if (c->H.cap > TR_SORT) {
-        k_sort<4096, 1024><<<grid, 256, 0, c->stream>>>(nray, c->H, c->d_orig, TR_SORT, 4096, halt);
+        k_sort<4096, 1024, SORT_NT4><<<grid, SORT_NT4, 0, c->stream>>>(nray, c->H, c->d_orig, TR_SORT, 4096, halt);
         PSR_HIP(c, hipGetLastError());
     }
     if (c->H.cap > 4096) {
-        k_sort<8192, 2048><<<grid, 256, 0, c->stream>>>(nray, c->H, c->d_orig, 4096, 8192, halt);
+        k_sort<8192, 2048, SORT_NT8><<<grid, SORT_NT8, 0, c->stream>>>(nray, c->H, c->d_orig, 4096, 8192, halt);
         PSR_HIP(c, hipGetLastError());
     }
     if (c->H.cap > 8192) {
