@@ -247,3 +247,37 @@ def test_surrogate_tick_against_exact_solver():
         else:
             assert acc > 0.95 * (acc + fb)
             assert np.median(dx) <= 1e-3
+
+
+@pytest.mark.gpu
+def test_surrogate_tick_off_the_spot_runs_the_exact_solver():
+    """the network predicts no recombination counters, so a run without
+    on-the-spot recombination takes the exact solver in surrogate ticks: no
+    update is counted as the network's, and the state equals an exact
+    tick's up to the f32 summation order of the two runs' deposits (which
+    an adaptive step decision can carry to the solver's tolerance)"""
+    print_case("surrogate off-the-spot")
+    wl = make_workload(1, RATES, n=16, nray=64)
+    wl.physics = psr.Physics(RATES, wl.physics.units,
+                             psr.make_flags(isothermal=True, h_only=True, on_the_spot=False))
+    rays = wl.rays(1)
+    out = {}
+    for chem in (psr.PSR_CHEM_EXACT, psr.PSR_CHEM_SURROGATE):
+        with psr.Context(0) as c:
+            c.set_physics(wl.physics)
+            c.load(wl.ps, boxsize=wl.boxsize)
+            rng = np.random.default_rng(2)
+            c.mlp_set_params(rng.normal(0, 0.05, _abi.lib().psr_mlp_param_count()).astype(np.float32))
+            c.mlp_set_norm(np.zeros(22), np.ones(22), np.full(22, -1e30), np.full(22, 1e30))
+            c.rays_upload(rays)
+            c.tick(1, wl.tick_time, chem=chem)
+            st = wl.ps.copy()
+            c.state_download(st)
+            out[chem] = (st, c.stats())
+    (a, sa), (b, sb) = out[psr.PSR_CHEM_EXACT], out[psr.PSR_CHEM_SURROGATE]
+    assert sb["surrogate_count"] == 0 and sb["active_count"] == sa["active_count"] > 0
+    assert np.array_equal(a.hits, b.hits) and np.array_equal(a.lasthit, b.lasthit)
+    for f in ("lambdaH", "yGrecHII"):
+        va, vb = getattr(a, f).astype(np.float64), getattr(b, f).astype(np.float64)
+        d = np.abs(va - vb)
+        assert np.median(d) < 1e-9 and (d <= 1e-3 * np.maximum(np.abs(va), np.abs(vb)) + 1e-6).all(), f
