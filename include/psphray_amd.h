@@ -162,6 +162,10 @@ int psr_units_init(psr_units *u, double h, double hmf, double hemf, double cmh,
 int psr_tables_build(psr_tables *t, const psr_units *u, const char *rates_path,
                      int with_secion, char *err, size_t errlen);
 void psr_tables_free(psr_tables *t);
+/* replaces the quadrature SPH table by the reference's data file
+ * src/kernel.akline (one value per line, comma-terminated: the array
+ * initialiser of src/sph-depth.c:3-6) */
+int psr_tables_load_kernel(psr_tables *t, const char *path, char *err, size_t errlen);
 /* host table lookups (for tests): ar_get, xs_get, sph_depth */
 double psr_tab_ar_get(const psr_tables *t, int col, double logT);
 double psr_tab_xs_get(const psr_tables *t, int id, double eng);

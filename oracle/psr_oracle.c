@@ -1339,7 +1339,8 @@ int32_t or_step_evolve(const or_ctx *c, or_step *s, double rtol, const double *a
         nsteps = 1;
         t = 1.0;
     }
-    while (t < 1.0 && nsteps < ROS3_MAXSTEPS) {
+    int dead = 0;
+    while (!dead && t < 1.0 && nsteps < ROS3_MAXSTEPS) {
         if (h > 1.0 - t) h = 1.0 - t;
         jac_active(c, s, act, n, x, f0, J);
         for (;;) {
@@ -1356,7 +1357,15 @@ int32_t or_step_evolve(const or_ctx *c, or_step *s, double rtol, const double *a
                 E = sqrt(acc / n);
                 if (!(E == E)) E = INFINITY;
             }
-            if (E <= 1.0 || h < 1e-14) {
+            /* a singular matrix or a non-finite error estimate never
+             * advances the state; at the smallest step it ends the
+             * integration with x unchanged (a solver failure, t < 1) */
+            const int good = E < INFINITY;
+            if (!good && h < 1e-14) {
+                dead = 1;
+                break;
+            }
+            if (good && (E <= 1.0 || h < 1e-14)) {
                 t += h;
                 memcpy(x, xn, sizeof(x));
                 double fac = 0.9 / ctl_cbrt(fmin(fmax(E, 1e-10), 1e10));

@@ -7,6 +7,7 @@ only marshals numpy arrays.  Names follow the reference: a particle system
 src/loop.c:21-35), ticks and epochs.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -57,7 +58,10 @@ class Physics:
     """Units, run flags and host-built tables (rates file, cross sections,
     SPH column kernel, secondary ionization)."""
 
-    def __init__(self, rates_path, units=None, flags=None, with_secion=False):
+    def __init__(self, rates_path, units=None, flags=None, with_secion=False, kernel_path="default"):
+        """kernel_path: the reference's SPH column table (src/kernel.akline
+        format); "default" takes the copy beside the rates file when there is
+        one, None keeps the quadrature table psr_tables_build computes"""
         self.units = units if units is not None else default_units()
         self.flags = flags if flags is not None else make_flags()
         self.tables = _abi.Tables()
@@ -66,7 +70,16 @@ class Physics:
                                          str(rates_path).encode(), int(with_secion), err, 512)
         if rc:
             raise PsrError(rc, err.value.decode())
+        if kernel_path == "default":
+            cand = os.path.join(os.path.dirname(os.path.abspath(str(rates_path))), "kernel.akline")
+            kernel_path = cand if os.path.exists(cand) else None
+        if kernel_path is not None:
+            rc = _abi.lib().psr_tables_load_kernel(C.byref(self.tables), str(kernel_path).encode(),
+                                                   err, 512)
+            if rc:
+                raise PsrError(rc, err.value.decode())
         self.rates_path = str(rates_path)
+        self.kernel_path = kernel_path
 
     def __del__(self):
         try:

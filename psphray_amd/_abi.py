@@ -103,6 +103,7 @@ SIGNATURES = {
     "psr_tables_build": (C.c_int, [C.POINTER(Tables), C.POINTER(Units), C.c_char_p, C.c_int,
                                    C.c_char_p, C.c_size_t]),
     "psr_tables_free": (None, [C.POINTER(Tables)]),
+    "psr_tables_load_kernel": (C.c_int, [C.POINTER(Tables), C.c_char_p, C.c_char_p, C.c_size_t]),
     "psr_tab_ar_get": (_D, [C.POINTER(Tables), C.c_int, _D]),
     "psr_tab_xs_get": (_D, [C.POINTER(Tables), C.c_int, _D]),
     "psr_tab_sph_depth": (_D, [C.POINTER(Tables), _D]),

@@ -824,6 +824,17 @@ extern "C" int psr_rays_upload(psr_ctx *c, const psr_ray *rays, int64_t nray) {
     PSR_HIP(c, hipSetDevice(c->device));
     // a replay of a pending tick needs the pool it was launched with
     if (int rc = psr_resolve(c)) return rc;
+    // the trace indexes the field roots with each ray's field: every index is
+    // checked before the pool is touched, and a refused upload leaves no pool
+    int32_t maxf = -1;
+    for (int64_t i = 0; i < nray; i++) {
+        if (rays[i].field < 0 || rays[i].field >= std::max(c->nfield, 1)) {
+            c->nray_pool = 0;
+            c->ray_field_max = -1;
+            PSR_FAIL(c, PSR_ERR_ARG, "ray field index out of range");
+        }
+        maxf = std::max(maxf, rays[i].field);
+    }
     if (nray > c->ray_cap) {
         dfree(c->d_rays);
         PSR_HIP(c, hipMalloc(&c->d_rays, sizeof(psr_ray) * nray));
@@ -832,9 +843,7 @@ extern "C" int psr_rays_upload(psr_ctx *c, const psr_ray *rays, int64_t nray) {
     if (nray) PSR_HIP(c, hipMemcpyAsync(c->d_rays, rays, sizeof(psr_ray) * nray, hipMemcpyHostToDevice, c->stream));
     PSR_HIP(c, hipStreamSynchronize(c->stream));
     c->nray_pool = nray;
-    for (int64_t i = 0; i < nray; i++)
-        if (rays[i].field < 0 || rays[i].field >= std::max(c->nfield, 1))
-            PSR_FAIL(c, PSR_ERR_ARG, "ray field index out of range");
+    c->ray_field_max = maxf;
     return PSR_OK;
 }
 

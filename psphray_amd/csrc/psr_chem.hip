@@ -565,7 +565,7 @@ struct Integ {
     StepIn s;
     Rates Riso;
     double t, h;
-    bool rejected, need_fj;
+    bool rejected, need_fj, dead;
     int nsteps;
     double f0[7];
     double J[N * N];
@@ -579,6 +579,7 @@ struct Integ {
         h = 1.0;
         rejected = false;
         need_fj = true;
+        dead = false;
         nsteps = 0;
     }
 
@@ -588,7 +589,7 @@ struct Integ {
         double f1[7], xt[7], M[N * N], k1[N], k2[N], k3[N];
         int piv[N];
         if (need_fj) {
-            if (!(t < 1.0 && nsteps < ROS3_MAXSTEPS)) return true;
+            if (dead || !(t < 1.0 && nsteps < ROS3_MAXSTEPS)) return true;
             if (h > 1.0 - t) h = 1.0 - t;
             rhs_jac<ISO, ADI, OTS, HONLY>(x, f0, J, s, T, P, Riso);
             need_fj = false;
@@ -632,7 +633,15 @@ struct Integ {
             E = sqrt(acc / N);
             if (!(E == E)) E = INFINITY;
         }
-        if (E <= 1.0 || h < 1e-14) {
+        // a singular matrix or a non-finite error estimate never advances
+        // the state; at the smallest step it ends the integration with x
+        // unchanged (t < 1 counts it as a solver failure)
+        const bool good = ok && E < INFINITY;
+        if (!good && h < 1e-14) {
+            dead = true;
+            return true;
+        }
+        if (good && (E <= 1.0 || h < 1e-14)) {
             t += h;
 #pragma unroll
             for (int q = 0; q < 7; q++) x[q] = xn[q];
@@ -668,6 +677,7 @@ struct Integ {
         nsteps = (int)w[N + 2];
         rejected = false;
         need_fj = true;
+        dead = false;
     }
 
     // clamps of src/solve.c:208-213

@@ -183,6 +183,7 @@ struct psr_ctx {
     // rays
     psr_ray *d_rays = nullptr;
     int64_t nray_pool = 0;
+    int32_t ray_field_max = -1;   // largest field index in the pool
     int64_t ray_cap = 0;
     // hit slots
     DevHits H{};

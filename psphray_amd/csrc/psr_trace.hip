@@ -610,6 +610,9 @@ static int check_ready(psr_ctx *c, int64_t ray_first, int64_t nray) {
     if (!c->have_state) PSR_FAIL(c, PSR_ERR_STATE, "no particle state: call psr_state_upload");
     if (ray_first < 0 || nray < 0 || ray_first + nray > c->nray_pool)
         PSR_FAIL(c, PSR_ERR_ARG, "ray range outside the uploaded pool");
+    // a tree rebuilt with fewer fields leaves earlier rays pointing past the roots
+    if (nray > 0 && c->ray_field_max >= std::max(c->nfield, 1))
+        PSR_FAIL(c, PSR_ERR_ARG, "ray pool names a field the current tree does not have");
     return PSR_OK;
 }
 

@@ -366,6 +366,49 @@ int psr_tables_build(psr_tables *t, const psr_units *u, const char *rates_path, 
     return PSR_OK;
 }
 
+/* the SPH column table as data: src/kernel.akline, one value per line,
+ * each followed by a comma (the file is #included as an array initialiser
+ * at src/sph-depth.c:3-6).  Replaces the quadrature table of
+ * psr_tables_build; the entry count is the file's (sph_depth indexes it with
+ * n = (int)(N r_h), src/sph-depth.c:26-33). */
+int psr_tables_load_kernel(psr_tables *t, const char *path, char *err, size_t errlen) {
+    if (!t || !path) return PSR_ERR_ARG;
+    FILE *f = fopen(path, "r");
+    if (!f) {
+        if (err && errlen) snprintf(err, errlen, "cannot open %s", path);
+        return PSR_ERR_ARG;
+    }
+    size_t cap = 4096, n = 0;
+    double *v = malloc(sizeof(double) * cap);
+    char line[256];
+    int bad = 0;
+    while (fgets(line, sizeof line, f)) {
+        char *p = line;
+        while (*p == ' ' || *p == '\t') p++;
+        if (*p == '\n' || *p == '\r' || *p == 0) continue;
+        char *end = NULL;
+        const double x = strtod(p, &end);
+        if (end == p) { bad = 1; break; }
+        while (*end == ' ' || *end == '\t') end++;
+        if (*end != ',' && *end != '\n' && *end != '\r' && *end != 0) { bad = 1; break; }
+        if (n == cap) {
+            cap *= 2;
+            v = realloc(v, sizeof(double) * cap);
+        }
+        v[n++] = x;
+    }
+    fclose(f);
+    if (bad || n == 0 || n > (1u << 24)) {
+        free(v);
+        if (err && errlen) snprintf(err, errlen, "%s: not a kernel table (one number per line)", path);
+        return PSR_ERR_ARG;
+    }
+    free((void *)t->kernel);
+    t->kernel_n = (int32_t)n;
+    t->kernel = v;
+    return PSR_OK;
+}
+
 void psr_tables_free(psr_tables *t) {
     if (!t) return;
     free((void *)t->ar);

@@ -1,8 +1,11 @@
 """Counter-based ray lists (Philox4x32-10, Salmon et al. 2011).
 
-Replaces the GSL streams of emit_rays (src/loop.c:163-373) with a list that
-is a pure function of (seed, tick, ray), so the GPU path and the CPU oracle
-consume identical rays.  The source choice follows the reference's weights
+Replaces the GSL streams of emit_rays (src/loop.c:163-373) with counter-based
+streams keyed by (seed, tick, ray), so the GPU path and the CPU oracle consume
+identical rays.  The list of a tick is a function of those keys and of each
+source's lastemit tick (src/sources.c:67-78), which emit() advances like the
+reference: a RayEmitter must emit ticks 1, 2, 3, ... in order, once each
+(a fresh emitter replays the same sequence).  The source choice follows the reference's weights
 Ngamma_dot * (tick - lastemit) * tick_time (src/sources.c:67-78), rays from
 one source share its weight equally (src/loop.c:305-321), directions are
 isotropic and the length is 2 * boxsize (src/loop.c:267).

@@ -66,12 +66,36 @@ def test_cross_sections_equal_oracle(phys, orc):
             assert phys.xs_get(i, e * u.U_EV) == orc.xs_get(i, e * u.U_EV)
 
 
-def test_kernel_table_matches_reference_table():
-    """the product computes the column kernel by quadrature; it reproduces
-    src/kernel.akline bit for bit in 99% of the entries and to 2.1e-7
-    elsewhere, except entry 1431 where the reference table itself is
-    2.3e-6 off the closed form (the oracle test pins that closed form)"""
+def test_kernel_table_loaded_from_the_reference_file():
+    """Physics loads src/kernel.akline (text, comma-terminated lines) as data:
+    every entry equals the oracle's fixture of the same file bit for bit, and
+    sph_depth indexes it as src/sph-depth.c:26-33 does"""
     p = psr.Physics(RATES)
+    assert p.kernel_path is not None and p.kernel_path.endswith("kernel.akline")
+    tab = p.kernel_table()
+    ref = po.reference_kernel_table()
+    assert tab.shape == (4096,) and np.array_equal(tab, ref)
+    for r in (0.0, 0.3, 0.66, 0.9999, 1.0, -0.1, 0.35 / 4096):
+        n = int(4096 * r)
+        assert p.sph_depth(r) == (ref[n] if 0 <= n < 4096 else 0.0)
+
+
+def test_kernel_file_errors(tmp_path):
+    bad = tmp_path / "k.akline"
+    bad.write_text("1.0,\nnot-a-number,\n")
+    with pytest.raises(psr.PsrError):
+        psr.Physics(RATES, kernel_path=str(bad))
+    with pytest.raises(psr.PsrError):
+        psr.Physics(RATES, kernel_path=str(tmp_path / "missing"))
+
+
+def test_quadrature_kernel_table_matches_reference_table():
+    """without the file the product computes the column kernel by
+    quadrature; it reproduces src/kernel.akline bit for bit in 99% of the
+    entries and to 2.1e-7 elsewhere, except entry 1431 where the reference
+    table itself is 2.3e-6 off the closed form (the oracle test pins that
+    closed form)"""
+    p = psr.Physics(RATES, kernel_path=None)
     mine = p.kernel_table()
     ref = po.reference_kernel_table()
     m = ref != 0

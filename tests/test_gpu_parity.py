@@ -26,8 +26,9 @@ def oracle_for(wl):
     ou = po.units(hmf=u.C_HMF, hemf=u.C_HEMF)
     of = po.OrFlags(f.isothermal, f.adiabatic, f.on_the_spot, f.h_only, f.secondary_ionization,
                     f.trace_only, f.no_photon, f.comoving, f.redshift, f.fake_temperature)
-    # the product's kernel table (it equals src/kernel.akline to 1.2e-7)
-    return po.Oracle(ou, of, rates=RATES, kernel=wl.physics.kernel_table())
+    # the oracle's own copy of src/kernel.akline (the product loads the same
+    # file as data, psr_tables_load_kernel)
+    return po.Oracle(ou, of, rates=RATES, kernel=po.reference_kernel_table())
 
 
 def load(ctx, wl):
@@ -581,7 +582,7 @@ def test_solve_steps_matches_oracle(ctx, golden):
         ctx.set_physics(phys)
         dev = ctx.solve_steps(c[f"{key}_in"])
         orc = po.Oracle(po.units(hmf=hmf), po.flags(isothermal=iso, h_only=iso),
-                        kernel=phys.kernel_table())
+                        kernel=po.reference_kernel_table())
         ref, bad = orc.step_evolve(c[f"{key}_in"])
         assert bad == 0
         tol = 1e-7 if iso else 1e-3
@@ -611,3 +612,29 @@ def test_iliev_multi_tick_ionized_fraction(ctx):
     dx = np.abs(out.lambdaH - ref.lambdaH)
     assert (1 - ref.lambdaH).max() > 0.5      # an ionized region formed
     assert dx.max() < 1e-3
+
+
+def test_refused_ray_upload_leaves_no_pool(ctx):
+    """a ray pool with a field index outside the tree is refused before any
+    copy, and a tick over the refused pool returns PSR_ERR_ARG without a
+    launch; a tree rebuilt with fewer fields refuses the older pool too"""
+    print_case("refused ray upload")
+    wl = make_workload(5, RATES, n=None, nray=None, nfield=3)
+    load(ctx, wl)
+    rays = wl.rays(1)
+    assert int(rays["field"].max()) == 2
+    bad = rays.copy()
+    bad["field"][len(bad) // 2] = 3
+    with pytest.raises(psr.PsrError) as e:
+        ctx.rays_upload(bad)
+    assert e.value.code == 1
+    rc = ctx._lib.psr_tick(ctx.h, 0, len(rays), 1, wl.tick_time, psr.PSR_PHASE_ALL, psr.PSR_CHEM_EXACT)
+    assert rc == 1
+    ctx.rays_upload(rays)
+    ctx.tick(1, wl.tick_time, nray=len(rays))
+    ctx.sync()
+    # the same pool after a rebuild with one field only
+    one = make_workload(1, RATES, n=8, nray=8)
+    load(ctx, one)
+    rc = ctx._lib.psr_tick(ctx.h, 0, len(rays), 2, wl.tick_time, psr.PSR_PHASE_ALL, psr.PSR_CHEM_EXACT)
+    assert rc == 1

@@ -290,6 +290,18 @@ def test_riccati_tick_matches_tight_integration():
     assert L.or_riccati_tick(0.9, g, m, G, A) == pytest.approx(a / q, rel=1e-12)
 
 
+def test_step_evolve_ends_on_a_non_finite_state(orc_he):
+    """a state whose right-hand side is NaN (ie = NaN) never advances: the
+    step size shrinks below 1e-14, the integration ends as a solver failure
+    and the fractions are written back unchanged (clamped), not NaN"""
+    s = _step(orc_he, lambdaH=0.3, lambdaHeI=0.6, lambdaHeII=0.3, nH=1e-3, T=1e4, time=1e3,
+              ie=np.nan)
+    o, bad = orc_he.step_evolve(s)
+    assert bad == 1
+    assert o["lambdaH"][0] == 0.3 and o["lambdaHeI"][0] == 0.6 and o["lambdaHeII"][0] == 0.3
+    assert o["nsteps"][0] < 100
+
+
 def test_chemistry_equilibrium_case_a_formula():
     """src/test-solve.c:41-47: with case A and no photons the neutral fraction
     relaxes to P/q = alpha/(gamma + alpha)"""
