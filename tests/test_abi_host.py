@@ -154,3 +154,16 @@ def test_philox_known_answer():
     from psphray_amd.rays import philox4x32
     out = philox4x32([0], [0], [0], [0], 0, 0)
     assert [int(x[0]) for x in out] == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+
+
+def test_void_rate_file_tables_equal_oracle():
+    """the reference's test/atomic_rates_void.txt (two rows, every rate zero):
+    the product's parser gives the oracle's table, Brem synthesized"""
+    vp = os.path.join(ROOT, "tests", "golden", "atomic_rates_void.txt")
+    u = psr.default_units(hmf=0.76)
+    p = psr.Physics(vp, u)
+    o = po.Oracle(po.units(hmf=0.76), po.flags(), rates=vp)
+    for col in range(15):
+        for lt in (0.0, 1.3, 4.0, 8.99, 9.0, 9.5):
+            assert p.ar_get(col, lt) == o.ar_get(col, lt)
+    assert p.ar_get(13, 4.5) > 0

@@ -638,3 +638,37 @@ def test_refused_ray_upload_leaves_no_pool(ctx):
     load(ctx, one)
     rc = ctx._lib.psr_tick(ctx.h, 0, len(rays), 2, wl.tick_time, psr.PSR_PHASE_ALL, psr.PSR_CHEM_EXACT)
     assert rc == 1
+
+
+def test_void_rate_table_steps_on_the_device(ctx):
+    """psr_solve_steps with the reference's void rate table: the fractions
+    lose exactly their deposits (x = x0 - yGdep, He lines included) and a
+    neutral particle gains exactly its heat, as the oracle pins show"""
+    import os
+    from conftest import GOLDEN
+    print_case("void rate table solve_steps")
+    vp = os.path.join(GOLDEN, "atomic_rates_void.txt")
+    for ots in (True, False):
+        u = psr.default_units(hmf=0.76)
+        ctx.set_physics(psr.Physics(vp, u, psr.make_flags(isothermal=False, h_only=False,
+                                                          on_the_spot=ots)))
+        phys = psr.Physics(vp, u)
+        s = np.zeros(2, psr.STEP_DTYPE)
+        nH = 1e-3 / u.U_CM ** 3
+        s["nH"] = nH
+        s["T"] = 1e4
+        s["time"] = 3.0
+        for k, v in dict(lambdaH=0.7, lambdaHeI=0.5, lambdaHeII=0.3, yGdepHI=0.2, yGdepHeI=0.1,
+                         yGdepHeII=0.05).items():
+            s[k][0] = v
+        s["ie"][0] = phys.Tye2ie(1e4, 0.5)
+        ie0 = phys.Tye2ie(1e4, 0.0)
+        s["lambdaH"][1], s["lambdaHeI"][1], s["lambdaHeII"][1] = 1.0, 1.0, 0.0
+        s["ie"][1] = ie0
+        s["heat"][1] = 0.37 * ie0
+        o = ctx.solve_steps(s)
+        assert abs(o["lambdaH"][0] - 0.5) < 1e-14
+        assert abs(o["lambdaHeI"][0] - 0.4) < 1e-14
+        assert abs(o["lambdaHeII"][0] - 0.25) < 1e-14
+        assert o["ie"][1] == pytest.approx(ie0 * 1.37, rel=1e-14)
+        assert o["lambdaH"][1] == 1.0 and o["lambdaHeI"][1] == 1.0
