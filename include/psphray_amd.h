@@ -253,6 +253,15 @@ int psr_get_tuning(psr_ctx *ctx, int32_t param, int32_t *value);
 /* step_evolve on the device for n independent steps (src/solve.c:317-323),
  * in place; rtol <= 0 selects the reference tolerances (src/solve.c:151) */
 int psr_solve_steps(psr_ctx *ctx, psr_step *steps, int64_t n, double rtol);
+/* psr_solve_steps without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The steps are staged
+ * through a fixed per-GPU device array of PSR_NOALLOC_MAX_STEPS steps
+ * (4.7 MB, held until the process ends); larger n returns PSR_ERR_CAPACITY
+ * without touching the device.  Calls on one GPU share that array: calls
+ * that overlap on one GPU (several host threads or contexts) race. */
+#define PSR_NOALLOC_MAX_STEPS 32768
+int psr_solve_steps_noalloc(psr_ctx *ctx, psr_step *steps, int64_t n, double rtol);
 
 /* ---- learned chemistry (MLP surrogate of step_evolve) ----
  * No counterpart in the reference: a network 22 -> 64 -> 64 -> 64 -> 4

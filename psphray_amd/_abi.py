@@ -20,6 +20,7 @@ PSR_PHASE_ALL = 7
 PSR_CHEM_EXACT = 0
 PSR_CHEM_SURROGATE = 1
 PSR_MLP_IN = 22
+PSR_NOALLOC_MAX_STEPS = 32768
 PSR_MLP_OUT = 4
 PSR_TUNE_DEPOSIT_ACCUM = 1
 PSR_ACCUM_AUTO = 0
@@ -124,6 +125,7 @@ SIGNATURES = {
     "psr_set_tuning": (C.c_int, [_VP, C.c_int32, C.c_int32]),
     "psr_get_tuning": (C.c_int, [_VP, C.c_int32, _VP]),
     "psr_solve_steps": (C.c_int, [_VP, _VP, _I64, _D]),
+    "psr_solve_steps_noalloc": (C.c_int, [_VP, _VP, _I64, _D]),
     "psr_mlp_features": (C.c_int, [_VP, C.POINTER(Units), _VP]),
     "psr_mlp_train": (C.c_int, [_VP, _VP, _VP, _I64, _I32, _I32, C.c_float, C.c_uint64, _VP]),
     "psr_mlp_forward": (C.c_int, [_VP, _VP, _I64, _VP]),

@@ -118,7 +118,9 @@ extern "C" int psr_create(psr_ctx **out, int device) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
     if (hipMalloc(&c->d_stats, sizeof(DevStats) * PSR_NSTAT) != hipSuccess ||
-        hipMemset(c->d_stats, 0, sizeof(DevStats) * PSR_NSTAT) != hipSuccess) {
+        hipMemset(c->d_stats, 0, sizeof(DevStats) * PSR_NSTAT) != hipSuccess ||
+        psr_noalloc_init(device) != PSR_OK) {
+        (void)hipFree(c->d_stats);
         (void)hipStreamDestroy(c->stream);
         delete c;
         return PSR_ERR_HIP;

@@ -13,6 +13,7 @@
 // rate table's interpolation), one table pass per step where finite
 // differences took N + 1.  The x_HI variable and x_HII = 1 - x_HI are both
 // kept in FP64.
+#include <mutex>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1247,6 +1248,60 @@ int psr_solve_steps_dev(psr_ctx *c, psr_step *d_steps, int64_t n, double rtol) {
     const unsigned grid = (unsigned)((n + 255) / 256);
     CHEM_DISPATCH(k_solve, d_steps, n, c->T, P, rtol);
     PSR_HIP(c, hipGetLastError());
+    return PSR_OK;
+}
+
+// ---- allocation-free step_evolve batches (psr_solve_steps_noalloc)
+// The steps travel through this fixed device array instead of a hipMalloc'd
+// staging buffer.  Each GPU holds its own copy for the life of the process;
+// host code never uses the symbol itself, only the per-device address that
+// psr_noalloc_init resolves (hipGetSymbolAddress) when a context is created.
+static __device__ __align__(256) psr_step scratch[PSR_NOALLOC_MAX_STEPS];
+static constexpr int kMaxDevices = 64;
+static void *g_scratch_addr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// the calling thread's current device must be `device`
+int psr_noalloc_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return PSR_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_addr[device]) return PSR_OK;
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return PSR_ERR_HIP;
+    g_scratch_addr[device] = ptr;
+    return PSR_OK;
+}
+
+extern "C" int psr_solve_steps_noalloc(psr_ctx *c, psr_step *steps, int64_t n, double rtol) {
+    if (!c || (n > 0 && !steps) || n < 0) return PSR_ERR_ARG;
+    if (n > PSR_NOALLOC_MAX_STEPS)
+        PSR_FAIL(c, PSR_ERR_CAPACITY, "psr_solve_steps_noalloc: at most PSR_NOALLOC_MAX_STEPS (32768) "
+                                      "steps per call");
+    if (!c->have_physics) PSR_FAIL(c, PSR_ERR_STATE, "no physics: call psr_set_physics");
+    if (rtol <= 0) rtol = 1e-4;
+    if (n == 0) return PSR_OK;
+    PSR_HIP(c, hipSetDevice(c->device));
+    psr_step *d = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (c->device < kMaxDevices) d = static_cast<psr_step *>(g_scratch_addr[c->device]);
+    }
+    if (!d) PSR_FAIL(c, PSR_ERR_STATE, "psr_solve_steps_noalloc: device scratch address not resolved");
+    // every entry the kernel reads is overwritten by this call's input first
+    hipError_t e = hipMemcpyAsync(d, steps, sizeof(psr_step) * n, hipMemcpyHostToDevice, c->stream);
+    int rc = PSR_OK;
+    if (e == hipSuccess) rc = psr_solve_steps_dev(c, d, n, rtol);
+    if (e == hipSuccess && rc == PSR_OK)
+        e = hipMemcpyAsync(steps, d, sizeof(psr_step) * n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess)
+        PSR_FAIL(c, PSR_ERR_HIP, std::string("psr_solve_steps_noalloc: ") + hipGetErrorString(e));
+    if (rc) return rc;
+    for (int64_t i = 0; i < n; i++)
+        if (steps[i].nsteps < 0) {
+            c->err = "psr_solve_steps_noalloc: a step did not reach t = 1";
+            return PSR_ERR_SOLVER;
+        }
     return PSR_OK;
 }
 

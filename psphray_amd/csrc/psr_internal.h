@@ -273,6 +273,7 @@ void psr_graph_reset(psr_ctx *c);
 // validate (and if needed replay) every pending tick; blocks
 int psr_resolve(psr_ctx *c);
 int psr_solve_steps_dev(psr_ctx *c, psr_step *d_steps, int64_t n, double rtol);
+int psr_noalloc_init(int device);
 DevPhys psr_dev_phys(const psr_ctx *c);
 int psr_sync_dev_ctx(psr_ctx *c);
 int psr_ensure_scratch(psr_ctx *c, size_t bytes);
