@@ -36,4 +36,8 @@ gcc -std=gnu99 -O2 -fPIC -ffp-contract=off -I"$ROOT/include" -c "$HERE/host/psr_
 pids+=($!)
 for p in "${pids[@]}"; do wait "$p" || { echo "build.sh: a compile step failed" >&2; exit 1; }; done
 "$HIPCC" --offload-arch=gfx950 -shared -fPIC -o "$HERE/libpsphray_amd.so" "$OUT"/*.o -lm
+# a plain C caller of the ABI (tests/test_c_driver.py runs it)
+mkdir -p "$ROOT/tests/_bin"
+gcc -std=c99 -O2 -Wall -I"$ROOT/include" "$ROOT/tests/c_driver.c" -L"$HERE" -lpsphray_amd \
+    -Wl,-rpath,'$ORIGIN/../../psphray_amd' -o "$ROOT/tests/_bin/c_driver"
 make -s -C "$ROOT/oracle" all

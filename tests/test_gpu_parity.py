@@ -179,6 +179,53 @@ def test_trace_batched_fields(ctx):
     assert np.array_equal(d, oh["d"])
 
 
+def _field_view(ps, a, b):
+    sub = ps.copy()
+    for name in sub.fields():
+        setattr(sub, name, np.ascontiguousarray(getattr(ps, name)[a:b]))
+    sub.npar = b - a
+    return sub
+
+
+def test_batched_fields_whole_ticks_against_per_field_oracle(ctx):
+    """batched uneven fields (workload 5 at 12 fields) over three whole
+    ticks (trace, deposit, merge, update) in one launch per phase, against
+    the oracle run field by field on each field's own rays and tree:
+    hits and last hits bit-exact, x_HI to 1e-6 (isothermal hydrogen, the
+    closed-form update), deposits of the last tick within the f32 summation
+    bound"""
+    print_case("batched whole ticks")
+    wl = make_workload(5, RATES, nfield=12, nray=128)
+    orc = oracle_for(wl)
+    load(ctx, wl)
+    fo = np.asarray(wl.field_offsets, np.int64)
+    refs = [_field_view(wl.ps, fo[f], fo[f + 1]) for f in range(len(fo) - 1)]
+    trees = [po.Tree(r.pos, r.sml, wl.boxsizes[f]) for f, r in enumerate(refs)]
+    for tick in (1, 2, 3):
+        rays = wl.rays(tick)
+        ctx.rays_upload(rays)
+        ctx.tick(tick, wl.tick_time, phases=psr.PSR_PHASE_TRACE | psr.PSR_PHASE_DEPOSIT
+                 if tick == 3 else psr.PSR_PHASE_ALL)
+        for f, ref in enumerate(refs):
+            fr = rays[rays["field"] == f]
+            if tick == 3:
+                off, hits = po.trace_rays(trees[f], ref, fr)
+                po.deposit(orc, ref, fr, off, hits)
+            else:
+                rc, _ = po.tick(orc, trees[f], ref, fr, tick, wl.tick_time)
+                assert rc == 0
+    out = ctx.state_download(wl.ps.copy())
+    for f, ref in enumerate(refs):
+        a, b = fo[f], fo[f + 1]
+        sub = _field_view(out, a, b)
+        assert np.array_equal(sub.hits, ref.hits), f
+        assert np.array_equal(sub.lasthit, ref.lasthit), f
+        assert np.abs(sub.lambdaH - ref.lambdaH).max() < 1e-6, f
+        for name in ("yGdepHI", "heat"):
+            assert_f32_sums(getattr(sub, name), getattr(ref, name), ref.hits, name)
+    assert (out.hits > 0).sum() > 1000
+
+
 def test_empty_field_and_empty_tick(ctx):
     """batched fields with one field emptied: its rays find nothing, the
     others' lists are the oracle's; a tick with no rays leaves the state as
