@@ -180,6 +180,8 @@ def main():
                     help="surrogate residual threshold (library default 1e-4)")
     ap.add_argument("--accum", choices=["auto", "atomic", "binned"], default="auto",
                     help="deposit accumulation (auto: timed on the first ticks)")
+    ap.add_argument("--walk", choices=["auto", "single", "multi"], default="auto",
+                    help="deposit: one ray or four rays per workgroup (auto: timed)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mlp-forward-rows", type=int, default=0,
                     help="only time the surrogate network's forward kernel on this many random "
@@ -228,6 +230,8 @@ def main():
 
     ctx = psr.Context(0)
     accum_ids = {"auto": psr.PSR_ACCUM_AUTO, "atomic": psr.PSR_ACCUM_ATOMIC, "binned": psr.PSR_ACCUM_BINNED}
+    ctx.set_tuning(psr.PSR_TUNE_DEPOSIT_WALK, {"auto": psr.PSR_WALK_AUTO, "single": psr.PSR_WALK_SINGLE,
+                                               "multi": psr.PSR_WALK_MULTI}[args.walk])
     ctx.set_tuning(psr.PSR_TUNE_DEPOSIT_ACCUM, accum_ids[args.accum])
     ctx.set_physics(wl.physics)
     t0 = time.perf_counter()
@@ -246,6 +250,7 @@ def main():
 
     def decided():
         return (ctx.tuning(psr.PSR_TUNE_DEPOSIT_ACCUM) != psr.PSR_ACCUM_AUTO
+                and ctx.tuning(psr.PSR_TUNE_DEPOSIT_WALK) != psr.PSR_WALK_AUTO
                 and ctx.tuning(psr.PSR_TUNE_TICK_GRAPH) != -1)
 
     run(args.warmup, psr.PSR_CHEM_EXACT)
@@ -259,8 +264,10 @@ def main():
         raise SystemExit("autotuner undecided after the warm-up ticks; no line is printed")
     warmup_ticks = args.warmup + extra
     accum = {v: k for k, v in accum_ids.items()}[ctx.tuning()]
+    walk = {psr.PSR_WALK_SINGLE: "one ray per workgroup",
+            psr.PSR_WALK_MULTI: "four rays per workgroup"}[ctx.tuning(psr.PSR_TUNE_DEPOSIT_WALK)]
     graph = {0: "kernel by kernel", 1: "graph"}[ctx.tuning(psr.PSR_TUNE_TICK_GRAPH)]
-    log(f"{warmup_ticks} warm-up ticks; deposit accumulation: {accum}; ticks: {graph}")
+    log(f"{warmup_ticks} warm-up ticks; deposit: {walk}, {accum} accumulation; ticks: {graph}")
 
     sur = None
     if surrogate:
@@ -376,6 +383,7 @@ def main():
         "deposit_walked_per_tick": st["deposit_processed"] / K,
         "solver_steps_per_update": st["solver_steps"] / max(st["active_count"], 1),
         "deposit_accumulation": accum,
+        "deposit_walk": walk,
         "tick_launch": graph,
         "graph_launches_timed": graph_launches,
         "surrogate": sur,

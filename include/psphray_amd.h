@@ -247,6 +247,21 @@ int psr_get_timers(psr_ctx *ctx, double *ms);
  * PSR_TUNE_GRAPH_LAUNCHES (read only): ticks run as graph launches so far. */
 #define PSR_TUNE_TICK_GRAPH 2
 #define PSR_TUNE_GRAPH_LAUNCHES 3
+/* PSR_TUNE_DEPOSIT_WALK: how many rays a deposit workgroup carries (their
+ * serial transmission walks, src/loop.c:439-589, run one per lane).
+ *   PSR_WALK_AUTO    timed together with the accumulation choices (two
+ *                    trial deposits each) on the first asynchronous ticks
+ *   PSR_WALK_SINGLE  one ray per 3-wavefront workgroup (prep / walk / apply
+ *                    pipelined over 128-hit segments in LDS)
+ *   PSR_WALK_MULTI   four rays per 4-wavefront workgroup (32-hit segments;
+ *                    the four walks in four lanes, two apply wavefronts),
+ *                    rays batched by hit count; f32 atomics only
+ * The values are the same bits either way (deposits within f32 summation
+ * order).  PSR_WALK_MULTI with PSR_ACCUM_BINNED is PSR_ERR_NOTSUP. */
+#define PSR_TUNE_DEPOSIT_WALK 4
+#define PSR_WALK_AUTO 0
+#define PSR_WALK_SINGLE 1
+#define PSR_WALK_MULTI 2
 int psr_set_tuning(psr_ctx *ctx, int32_t param, int32_t value);
 int psr_get_tuning(psr_ctx *ctx, int32_t param, int32_t *value);
 

@@ -28,6 +28,10 @@ PSR_ACCUM_ATOMIC = 1
 PSR_ACCUM_BINNED = 2
 PSR_TUNE_TICK_GRAPH = 2
 PSR_TUNE_GRAPH_LAUNCHES = 3
+PSR_TUNE_DEPOSIT_WALK = 4
+PSR_WALK_AUTO = 0
+PSR_WALK_SINGLE = 1
+PSR_WALK_MULTI = 2
 
 _D = C.c_double
 _I32 = C.c_int32
@@ -149,10 +153,12 @@ def lib():
     """Load the in-tree HIP library; fails loudly when it is not built."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"{LIB_PATH} is missing: build it with psphray_amd/build.sh "
+        # PSR_LIBRARY: another build of the same ABI (profiling builds)
+        path = os.environ.get("PSR_LIBRARY") or LIB_PATH
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: build it with psphray_amd/build.sh "
                                "(or __graft_entry__.build())")
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(path)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)
             f.restype = res

@@ -164,6 +164,7 @@ extern "C" void psr_destroy(psr_ctx *c) {
     dfree(c->H.perm);
     dfree(c->H.sslot);
     dfree(c->H.sb);
+    dfree(c->H.order);
     dfree(c->d_ray_len);
     dfree(c->d_ray_tm);
     dfree(c->d_bin_rec);
@@ -175,7 +176,7 @@ extern "C" void psr_destroy(psr_ctx *c) {
     dfree(c->d_ctx);
     for (int k = 0; k < 10; k++)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
-    for (int k = 0; k < 2 * psr_ctx::TUNE_N; k++)
+    for (int k = 0; k < 2 * psr_ctx::DTUNE_N; k++)
         if (c->tune_ev[k]) (void)hipEventDestroy(c->tune_ev[k]);
     for (int k = 0; k < 2 * psr_ctx::TUNE_N; k++)
         if (c->gtune_ev[k]) (void)hipEventDestroy(c->gtune_ev[k]);

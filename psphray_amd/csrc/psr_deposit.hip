@@ -848,6 +848,276 @@ __global__ __launch_bounds__(PLAN_THREADS) void k_bin_plan(uint32_t *__restrict_
     for (int64_t b = b0; b < b1; b++) fill[b] = 0;
 }
 
+// ---- several rays per workgroup
+// The kernel above gives each ray a workgroup of three single-wavefront
+// roles; each role is bound by its own latency chain, and the LDS ring
+// keeps four rays per CU.  Here a workgroup carries MR rays with shorter
+// segments (MSEG hits per ray, the same 128 hits per iteration): the MR walks
+// run in MR lanes of one wavefront (the same instructions, so MR times the
+// walk throughput at the same latency), one wavefront preps the MR * MSEG
+// hits, and two apply wavefronts take one hit per lane.  The walk stores the
+// TM each hit saw, so the apply reads it instead of stepping again from a
+// checkpoint.  Values are those of the kernel above bit for bit (same
+// prep_hit, walk_step and apply_values); the cut is tested after each live
+// hit.  Rays come in batches of similar hit counts (k_ray_order).
+constexpr int ORD_THREADS = 1024;
+constexpr int MR = 4;                 // rays per workgroup
+constexpr int MSEG = 32;              // hits per ray and segment
+constexpr int MW = 4;                 // wavefronts: walk, prep, apply, apply
+constexpr int MTHREADS = 64 * MW;
+static_assert(MR * MSEG == 128, "one prep wavefront: two hits per lane; one hit per apply lane");
+
+// rays by descending hit count: buckets of equal bit length (count 0 last)
+__global__ __launch_bounds__(ORD_THREADS) void k_ray_order(const int32_t *__restrict__ count, int64_t nray,
+                                                           uint32_t *__restrict__ order, const int32_t *halt) {
+    __shared__ uint32_t hist[33], base[33];
+    if (halted(halt)) return;
+    if (threadIdx.x < 33) hist[threadIdx.x] = 0;
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < nray; i += ORD_THREADS) atomicAdd(&hist[__clz(count[i])], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int k = 0; k < 33; k++) {
+            base[k] = run;
+            run += hist[k];
+        }
+    }
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < nray; i += ORD_THREADS) {
+        const uint32_t pos = atomicAdd(&base[__clz(count[i])], 1u);
+        order[pos] = (uint32_t)i;
+    }
+}
+
+template <bool HE>
+struct MLds {
+    WalkRec<HE> rec[3][MR * MSEG];
+    HitAux<HE> aux[3][MR * MSEG];
+    double tmh[3][MR * MSEG];  // TM before each hit (walk -> apply)
+    int64_t ray[MR];           // ray index, -1 for an empty place in the batch
+    int32_t n[MR];             // hits of the ray (0: nothing to deposit)
+    int32_t endseg[MR];        // last segment to deposit (the cut's, once found)
+    int32_t last[MR];          // last hit deposited
+    double tm_end[MR];
+};
+
+// hit k of a ray's list for the prep wavefront: ray place p, hit index in
+// the list (clamped into it; a ray without hits reads particle 0)
+template <bool HE>
+__device__ __forceinline__ void mr_index(HitGather<HE> &g, const DevHits &H, const MLds<HE> &L, int p, int k) {
+    const int n = L.n[p];
+    if (n > 0) {
+        const int64_t at = L.ray[p] * (int64_t)H.cap + min(k, n - 1);
+        g.slot = gmem(H.sslot)[at];
+        g.b = gmem(H.sb)[at];
+    } else {
+        g.slot = 0;
+        g.b = 0.0;
+    }
+}
+
+template <bool HE>
+__global__ __launch_bounds__(MTHREADS, 3) void k_deposit_multi(const DevTick *__restrict__ tk, int64_t nray,
+                                                               DevHits H, const DevCtx *__restrict__ C,
+                                                               DevStats *__restrict__ st0,
+                                                               double *__restrict__ ray_len,
+                                                               double *__restrict__ ray_tm, const int32_t *halt) {
+    __shared__ MLds<HE> L;
+    if (halted(halt)) return;
+    const psr_ray *__restrict__ rays = tk->rays;
+    const DevState &S = C->S;
+    const DevTables &T = C->T;
+    const DevPhys &P = C->P;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    ApplySums acc{0, 0, 0, 0, 0, 0};
+    unsigned long long cnt = 0, walked = 0;
+    double lost = 0.0;
+    const int64_t nbatch = (nray + MR - 1) / MR;
+    for (int64_t bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
+        if (threadIdx.x < MR) {
+            const int p = threadIdx.x;
+            const int64_t q = bt * MR + p;
+            int64_t r = -1;
+            int n = 0;
+            if (q < nray) {
+                r = H.order[q];
+                n = H.count[r];
+                if (rays[r].Nph == 0 || n == 0) {
+                    // nothing to deposit (src/loop.c:408): the ray keeps its length
+                    ray_len[r] = rays[r].length;
+                    ray_tm[r] = rays[r].Nph;
+                    n = 0;
+                }
+            }
+            L.ray[p] = r;
+            L.n[p] = n;
+            L.endseg[p] = n > 0 ? (n - 1) / MSEG : -1;
+            L.last[p] = n - 1;
+            L.tm_end[p] = 0.0;
+        }
+        __syncthreads();
+        if (wv == 1) {
+            // prep: hit j = h * 64 + lane of the iteration is hit (j % MSEG)
+            // of ray place j / MSEG; the slot / b of segment i + 2 and the
+            // particle gathers of segment i + 1 are in flight while segment i
+            // is computed (indexes clamped into each list)
+            constexpr int HPL = 2;
+            RayConst kc[HPL];
+            uint32_t a_slot[HPL];
+            double a_b[HPL];
+            HitGather<HE> gc[HPL];
+#pragma unroll
+            for (int h = 0; h < HPL; h++) {
+                const int j = h * 64 + lane, p = j / MSEG, k = j % MSEG;
+                kc[h] = ray_const<HE>(L.n[p] > 0 ? rays[L.ray[p]].freq : 0.0, T, P);
+                mr_index<HE>(gc[h], H, L, p, k);
+                gather_hit<HE>(gc[h], S);
+                HitGather<HE> t1;
+                mr_index<HE>(t1, H, L, p, MSEG + k);
+                a_slot[h] = t1.slot;
+                a_b[h] = t1.b;
+            }
+            for (int i = 0;; i++) {
+                int mend = -1;
+#pragma unroll
+                for (int p = 0; p < MR; p++) mend = max(mend, L.endseg[p]);
+                if (i > mend + 2) break;
+                const int ring = i % 3;
+                double sd[HPL];
+#pragma unroll
+                for (int h = 0; h < HPL; h++) sd[h] = sph_depth(T, gc[h].b * (1.0 / (double)gc[h].sml));
+#pragma unroll
+                for (int h = 0; h < HPL; h++) {
+                    const int j = h * 64 + lane, p = j / MSEG, k = j % MSEG;
+                    WalkRec<HE> rr;
+                    HitAux<HE> au;
+                    prep_hit<HE>(gc[h], sd[h], P, kc[h], rr, au);
+                    if (i * MSEG + k >= L.n[p]) noop_rec<HE>(rr, au);
+                    L.rec[ring][j] = rr;
+                    L.aux[ring][j] = au;
+                }
+#pragma unroll
+                for (int h = 0; h < HPL; h++) {
+                    const int j = h * 64 + lane, p = j / MSEG, k = j % MSEG;
+                    gc[h].slot = a_slot[h];
+                    gc[h].b = a_b[h];
+                    gather_hit<HE>(gc[h], S);
+                    HitGather<HE> t2;
+                    mr_index<HE>(t2, H, L, p, (i + 2) * MSEG + k);
+                    a_slot[h] = t2.slot;
+                    a_b[h] = t2.b;
+                }
+                __syncthreads();
+            }
+        } else if (wv == 0) {
+            // the walks: lane p < MR walks ray place p, one segment behind the
+            // prep; the next record is read while a step runs
+            __builtin_amdgcn_s_setprio(3);
+            const int p = lane < MR ? lane : 0;
+            double t = 0.0, T0 = 0.0;
+            if (lane < MR && L.n[p] > 0) {
+                const int64_t r = L.ray[p];
+                const double Nph = rays[r].Nph;
+                T0 = cutoff_threshold(Nph);
+                t = (P.no_photon && rays[r].type < 0) ? 0.0 : Nph;
+            }
+            for (int i = 0;; i++) {
+                int mend = -1;
+#pragma unroll
+                for (int q = 0; q < MR; q++) mend = max(mend, L.endseg[q]);
+                if (i > mend + 2) break;
+                const int s = i - 1;
+                if (lane < MR && s >= 0 && s <= L.endseg[p]) {
+                    const int ring = s % 3;
+                    const int kn = min(MSEG, L.n[p] - s * MSEG);
+                    const WalkRec<HE> *R = &L.rec[ring][p * MSEG];
+                    double *TMH = &L.tmh[ring][p * MSEG];
+                    WalkRec<HE> A = R[0];
+                    for (int k = 0; k < kn; k++) {
+                        const WalkRec<HE> v = A;
+                        A = R[min(k + 1, MSEG - 1)];
+                        TMH[k] = t;
+                        t = walk_step<HE>(t, v);
+                        if (t < T0 && is_live(v.em)) {
+                            L.endseg[p] = s;
+                            L.last[p] = s * MSEG + k;
+                            break;
+                        }
+                    }
+                    L.tm_end[p] = t;
+                }
+                __syncthreads();
+            }
+            __builtin_amdgcn_s_setprio(0);
+        } else {
+            // apply: one hit per lane of segment i - 2 (its walk and cut are done)
+            const int j = (wv - 2) * 64 + lane, p = j / MSEG, k = j % MSEG;
+            const RayConst kc = ray_const<HE>(L.n[p] > 0 ? rays[L.ray[p]].freq : 0.0, T, P);
+            for (int i = 0;; i++) {
+                int mend = -1;
+#pragma unroll
+                for (int q = 0; q < MR; q++) mend = max(mend, L.endseg[q]);
+                if (i > mend + 2) break;
+                const int s = i - 2;
+                if (s >= 0 && s <= L.endseg[p] && s * MSEG + k <= L.last[p]) {
+                    const int ring = s % 3;
+                    const WalkRec<HE> v = L.rec[ring][j];
+                    if (is_live(v.em)) {
+                        const DepRec<HE> rr = apply_values<HE>(L.tmh[ring][j], v, L.aux[ring][j], T, P, kc, acc);
+                        rec_atomic<HE>(S, rr);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (threadIdx.x < MR && L.n[threadIdx.x] > 0) {
+            const int p = threadIdx.x;
+            const int64_t r = L.ray[p];
+            const int last = L.last[p];
+            const int64_t base = r * (int64_t)H.cap;
+            ray_len[r] = H.d[base + H.perm[base + last]];
+            ray_tm[r] = L.tm_end[p];
+            cnt += (unsigned long long)L.n[p];
+            walked += (unsigned long long)(last + 1);
+            lost += L.tm_end[p];
+        }
+        __syncthreads();
+    }
+    // counters: wave 0 holds the ray totals, waves 2 and 3 the deposit sums
+    if (wv == 0) {
+        const unsigned long long c1 = wave_sum_u64(cnt), c2 = wave_sum_u64(walked);
+        const double l1 = wave_sum(lost);
+        if (lane == 0 && c1) {
+            DevStats *st = stat_slot(st0);
+            atomicAdd(&st->total_deposit_count, c1);
+            atomicAdd(&st->deposit_processed, c2);
+            atomicAdd(&st->lost_photon_count_sum, l1);
+        }
+    } else if (wv >= 2) {
+        const int ns = wave_sum_i(acc.nsat);
+        const double v1 = wave_sum(acc.fiHI), v2 = wave_sum(acc.siHI);
+        double v3 = 0, v4 = 0, v5 = 0;
+        if (HE) {
+            v3 = wave_sum(acc.fiHeI);
+            v4 = wave_sum(acc.fiHeII);
+            v5 = wave_sum(acc.siHeI);
+        }
+        if (lane == 0) {
+            DevStats *st = stat_slot(st0);
+            if (ns) atomicAdd(&st->saturated_deposit_count, (unsigned long long)ns);
+            if (v1 != 0) atomicAdd(&st->first_ionization_HI, v1);
+            if (v2 != 0) atomicAdd(&st->secondary_ionization_HI, v2);
+            if (HE) {
+                if (v3 != 0) atomicAdd(&st->first_ionization_HeI, v3);
+                if (v4 != 0) atomicAdd(&st->first_ionization_HeII, v4);
+                if (v5 != 0) atomicAdd(&st->secondary_ionization_HeI, v5);
+            }
+        }
+    }
+}
+
 }  // namespace
 
 // bins sized for the context's particles and hit slots; reallocated (after
@@ -880,19 +1150,36 @@ static int ensure_bins(psr_ctx *c, bool he) {
     return PSR_OK;
 }
 
-int psr_prepare_deposit(psr_ctx *c, bool binned) {
+int psr_prepare_deposit(psr_ctx *c, int mode) {
     int rc = psr_sync_dev_ctx(c);
     if (rc) return rc;
-    return binned ? ensure_bins(c, !c->F.h_only) : PSR_OK;
+    return mode == PSR_DEP_WG_BINNED ? ensure_bins(c, !c->F.h_only) : PSR_OK;
 }
 
 // the rays of the tick come from the tick record (ray_first is kept for the
 // call's shape checks)
-int psr_launch_deposit(psr_ctx *c, int64_t, int64_t nray, bool binned) {
-    int rc = psr_prepare_deposit(c, binned);
+int psr_launch_deposit(psr_ctx *c, int64_t, int64_t nray, int mode) {
+    int rc = psr_prepare_deposit(c, mode);
     if (rc) return rc;
     const bool he = !c->F.h_only;
     const DevTick *rays = c->d_tick;
+    if (mode == PSR_DEP_MULTI) {
+        const int32_t *halt = &c->d_stats->flags;
+        PSR_EV(c, 2, 0);
+        k_ray_order<<<1, ORD_THREADS, 0, c->stream>>>(c->H.count, nray, c->H.order, halt);
+        PSR_HIP(c, hipGetLastError());
+        const unsigned gm = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nray + MR - 1) / MR, 65535 * 4));
+        if (he)
+            k_deposit_multi<true><<<gm, MTHREADS, 0, c->stream>>>(rays, nray, c->H, c->d_ctx, c->d_stats,
+                                                                 c->d_ray_len, c->d_ray_tm, halt);
+        else
+            k_deposit_multi<false><<<gm, MTHREADS, 0, c->stream>>>(rays, nray, c->H, c->d_ctx, c->d_stats,
+                                                                  c->d_ray_len, c->d_ray_tm, halt);
+        PSR_HIP(c, hipGetLastError());
+        PSR_EV(c, 2, 1);
+        return PSR_OK;
+    }
+    const bool binned = mode == PSR_DEP_WG_BINNED;
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nray, 65535 * 4));
     const int32_t *halt = &c->d_stats->flags;
     const int64_t nbins = binned ? c->bin_n : 0;

@@ -92,8 +92,13 @@ struct DevHits {
     uint32_t *perm;
     uint32_t *sslot;
     double *sb;
+    uint32_t *order;  // several rays per workgroup: rays by descending hit count
     int32_t cap;
 };
+
+// how a tick's deposit runs: one ray per 3-wavefront workgroup with f32
+// atomics or binned records, or several rays per workgroup (atomics)
+enum { PSR_DEP_WG_ATOMIC = 0, PSR_DEP_WG_BINNED = 1, PSR_DEP_MULTI = 2 };
 
 // device counters: PSR_NSTAT copies, one 128-byte line each; a block adds
 // into copy blockIdx.x % PSR_NSTAT and the host folds the copies.  Device-
@@ -244,9 +249,12 @@ struct psr_ctx {
     std::vector<int64_t> graph_key;
     int64_t graph_launches = 0;
     int32_t accum_req = PSR_ACCUM_AUTO;
-    int32_t accum_use = PSR_ACCUM_AUTO;
-    hipEvent_t tune_ev[2 * TUNE_N] = {};
-    int32_t tune_slot[TUNE_N] = {};  // ring slot of the trial's tick; -1: a synchronous replay
+    int32_t walk_req = PSR_WALK_AUTO;
+    int32_t dep_use = -1;            // the decided PSR_DEP_* mode; -1 while timing
+    static constexpr int DTUNE_N = 6;  // deposit trials: two per candidate mode
+    hipEvent_t tune_ev[2 * DTUNE_N] = {};
+    int32_t tune_slot[DTUNE_N] = {};  // ring slot of the trial's tick; -1: a synchronous replay
+    int32_t tune_mode[DTUNE_N] = {};
     int tune_n = 0;
 
     DevState state() const;
@@ -259,14 +267,14 @@ int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray, bool sync, con
 int psr_launch_sort(psr_ctx *c, int64_t nray);
 // hit slots for nray rays of cap hits each (reallocates after a sync)
 int psr_ensure_hits(psr_ctx *c, int64_t nray, int32_t cap);
-int psr_launch_deposit(psr_ctx *c, int64_t ray_first, int64_t nray, bool binned);
+int psr_launch_deposit(psr_ctx *c, int64_t ray_first, int64_t nray, int mode);
 // restart the accumulation trials (tree or physics changed)
 void psr_tune_reset(psr_ctx *c);
 // timed: record the update phase's event pair (the surrogate records its own);
 // the tick and its length come from c->d_tick
 int psr_launch_update(psr_ctx *c, bool timed = true);
 // allocations of the deposit / update phases ahead of a graph recording
-int psr_prepare_deposit(psr_ctx *c, bool binned);
+int psr_prepare_deposit(psr_ctx *c, int mode);
 int psr_prepare_update(psr_ctx *c);
 // drop the recorded tick graph (physics, tree, state or options changed)
 void psr_graph_reset(psr_ctx *c);

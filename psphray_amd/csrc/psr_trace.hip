@@ -471,6 +471,7 @@ int psr_ensure_hits(psr_ctx *c, int64_t nray, int32_t cap) {
     if (c->H.perm) (void)hipFree(c->H.perm);
     if (c->H.sslot) (void)hipFree(c->H.sslot);
     if (c->H.sb) (void)hipFree(c->H.sb);
+    if (c->H.order) (void)hipFree(c->H.order);
     if (c->d_ray_len) (void)hipFree(c->d_ray_len);
     if (c->d_ray_tm) (void)hipFree(c->d_ray_tm);
     c->H = DevHits{};
@@ -483,6 +484,7 @@ int psr_ensure_hits(psr_ctx *c, int64_t nray, int32_t cap) {
     PSR_HIP(c, hipMalloc(&c->H.perm, sizeof(uint32_t) * tot));
     PSR_HIP(c, hipMalloc(&c->H.sslot, sizeof(uint32_t) * tot));
     PSR_HIP(c, hipMalloc(&c->H.sb, sizeof(double) * tot));
+    PSR_HIP(c, hipMalloc(&c->H.order, sizeof(uint32_t) * ns));
     PSR_HIP(c, hipMalloc(&c->H.count, sizeof(int32_t) * ns));
     PSR_HIP(c, hipMalloc(&c->d_ray_len, sizeof(double) * ns));
     PSR_HIP(c, hipMalloc(&c->d_ray_tm, sizeof(double) * ns));
@@ -670,23 +672,36 @@ extern "C" int psr_trace_lists(psr_ctx *c, int64_t ray_first, int64_t nray, int6
 
 void psr_tune_reset(psr_ctx *c) {
     c->tune_n = 0;
-    c->accum_use = PSR_ACCUM_AUTO;
+    c->dep_use = -1;
     c->gtune_n = 0;
     c->graph_use = -1;
 }
 
-// once every trial has run: keep the accumulation whose fastest trial
-// deposit was faster.  A trial on an asynchronous tick that overflowed (its
-// kernels stopped early, its replay ran later) does not count; trials on
-// replays do.  Without a valid trial of each kind the trials start again.
+// the deposit modes the requests leave open, in a fixed order
+static int dep_candidates(const psr_ctx *c, int *cand) {
+    int n = 0;
+    const bool wg = c->walk_req != PSR_WALK_MULTI;
+    const bool lane = c->walk_req != PSR_WALK_SINGLE && c->accum_req != PSR_ACCUM_BINNED;
+    if (wg && c->accum_req != PSR_ACCUM_BINNED) cand[n++] = PSR_DEP_WG_ATOMIC;
+    if (wg && c->accum_req != PSR_ACCUM_ATOMIC) cand[n++] = PSR_DEP_WG_BINNED;
+    if (lane) cand[n++] = PSR_DEP_MULTI;
+    return n;
+}
+
+// once every trial has run: keep the mode whose fastest trial deposit was
+// fastest.  A trial on an asynchronous tick that overflowed (its kernels
+// stopped early, its replay ran later) does not count; trials on replays
+// do.  Without a valid trial of each candidate the trials start again.
 // Blocks only when `block`.
 static int tune_decide(psr_ctx *c, bool block) {
-    if (c->accum_use != PSR_ACCUM_AUTO || c->tune_n < psr_ctx::TUNE_N) return PSR_OK;
-    const hipEvent_t last = c->tune_ev[2 * psr_ctx::TUNE_N - 1];
+    int cand[3];
+    const int nc = dep_candidates(c, cand);
+    if (c->dep_use >= 0 || nc < 2 || c->tune_n < 2 * nc) return PSR_OK;
+    const hipEvent_t last = c->tune_ev[2 * (2 * nc) - 1];
     if (block) PSR_HIP(c, hipEventSynchronize(last));
     else if (hipEventQuery(last) != hipSuccess) return PSR_OK;
-    float best[2] = {INFINITY, INFINITY};
-    for (int k = 0; k < psr_ctx::TUNE_N; k++) {
+    float best[3] = {INFINITY, INFINITY, INFINITY};
+    for (int k = 0; k < 2 * nc; k++) {
         if (c->tune_slot[k] >= 0) {
             const int32_t v = ((volatile int32_t *)c->h_ring)[c->tune_slot[k]];
             if (v == 0 && !block) return PSR_OK;
@@ -694,13 +709,17 @@ static int tune_decide(psr_ctx *c, bool block) {
         }
         float ms = 0;
         PSR_HIP(c, hipEventElapsedTime(&ms, c->tune_ev[2 * k], c->tune_ev[2 * k + 1]));
-        best[k & 1] = std::min(best[k & 1], ms);
+        best[c->tune_mode[k]] = std::min(best[c->tune_mode[k]], ms);
     }
-    if (!(best[0] < INFINITY && best[1] < INFINITY)) {
-        psr_tune_reset(c);
-        return PSR_OK;
+    int pick = -1;
+    for (int i = 0; i < nc; i++) {
+        if (!(best[cand[i]] < INFINITY)) {
+            psr_tune_reset(c);
+            return PSR_OK;
+        }
+        if (pick < 0 || best[cand[i]] < best[pick]) pick = cand[i];
     }
-    c->accum_use = best[1] < best[0] ? PSR_ACCUM_BINNED : PSR_ACCUM_ATOMIC;
+    c->dep_use = pick;
     return PSR_OK;
 }
 
@@ -715,17 +734,20 @@ __global__ void k_fill_u8(uint8_t *p, int64_t n) {
         p[i] = 0;
 }
 
-// the accumulation of this tick's deposit: the requested or decided one,
-// else the next trial (odd trials binned); replays use binning meanwhile
-static bool tick_binned(const psr_ctx *c, int *trial) {
+// the deposit mode of this tick: the only candidate, the decided one, else
+// the next trial (candidates in turn); after the last trial and before the
+// decision, the first candidate
+static int tick_mode(const psr_ctx *c, int *trial) {
     *trial = -1;
-    if (c->accum_req != PSR_ACCUM_AUTO) return c->accum_req == PSR_ACCUM_BINNED;
-    if (c->accum_use != PSR_ACCUM_AUTO) return c->accum_use == PSR_ACCUM_BINNED;
-    if (c->tune_n < psr_ctx::TUNE_N) {
+    int cand[3];
+    const int nc = dep_candidates(c, cand);
+    if (nc == 1) return cand[0];
+    if (c->dep_use >= 0) return c->dep_use;
+    if (c->tune_n < 2 * nc) {
         *trial = c->tune_n;
-        return *trial & 1;
+        return cand[*trial % nc];
     }
-    return true;
+    return cand[0];
 }
 
 // the tick-graph trials: whole ticks alternately kernel by kernel (even
@@ -770,16 +792,17 @@ static int tick_body(psr_ctx *c, const PendingTick &t, bool sync, bool set = tru
         if (t.phases & PSR_PHASE_DEPOSIT) {
             if ((rc = psr_launch_sort(c, t.nray))) return rc;
             int trial;
-            const bool binned = tick_binned(c, &trial);
+            const int mode = tick_mode(c, &trial);
             if (trial >= 0) {
                 if (!c->tune_ev[0])
-                    for (int k = 0; k < 2 * psr_ctx::TUNE_N; k++) PSR_HIP(c, hipEventCreate(&c->tune_ev[k]));
+                    for (int k = 0; k < 2 * psr_ctx::DTUNE_N; k++) PSR_HIP(c, hipEventCreate(&c->tune_ev[k]));
                 PSR_HIP(c, hipEventRecord(c->tune_ev[2 * trial], c->stream));
             }
-            if ((rc = psr_launch_deposit(c, t.ray_first, t.nray, binned))) return rc;
+            if ((rc = psr_launch_deposit(c, t.ray_first, t.nray, mode))) return rc;
             if (trial >= 0) {
                 PSR_HIP(c, hipEventRecord(c->tune_ev[2 * trial + 1], c->stream));
                 c->tune_slot[trial] = sync ? -1 : (int32_t)t.slot;
+                c->tune_mode[trial] = mode;
                 c->tune_n++;
             }
         } else {
@@ -821,17 +844,17 @@ void psr_graph_reset(psr_ctx *c) {
 static bool graph_usable(const psr_ctx *c, const PendingTick &t) {
     if (c->timing || t.nray <= 0) return false;
     int trial;
-    (void)tick_binned(c, &trial);
+    (void)tick_mode(c, &trial);
     if (trial >= 0) return false;
     return c->H.cap > 0 && c->H.cap <= 8192;
 }
 
-static std::vector<int64_t> graph_key(const psr_ctx *c, const PendingTick &t, bool binned) {
+static std::vector<int64_t> graph_key(const psr_ctx *c, const PendingTick &t, int mode) {
     const MlpDev M = psr_mlp_dev(c);
     return {t.nray,
             t.phases,
             t.chem_mode,
-            (int64_t)binned,
+            (int64_t)mode,
             c->H.cap,
             c->hit_slots,
             (int64_t)(intptr_t)c->H.d,
@@ -871,11 +894,11 @@ static int tick_graph(psr_ctx *c, const PendingTick &t, bool *done, int *trial) 
         }
     }
     int atrial;
-    const bool binned = tick_binned(c, &atrial);
+    const int mode = tick_mode(c, &atrial);
     if ((rc = psr_sync_dev_ctx(c))) return rc;
-    if ((t.phases & PSR_PHASE_DEPOSIT) && (rc = psr_prepare_deposit(c, binned))) return rc;
+    if ((t.phases & PSR_PHASE_DEPOSIT) && (rc = psr_prepare_deposit(c, mode))) return rc;
     if ((t.phases & PSR_PHASE_UPDATE) && (rc = psr_prepare_update(c))) return rc;
-    const std::vector<int64_t> key = graph_key(c, t, binned);
+    const std::vector<int64_t> key = graph_key(c, t, mode);
     if (!c->graph_exec || key != c->graph_key) {
         psr_graph_reset(c);
         PSR_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -1012,7 +1035,16 @@ extern "C" int psr_set_tuning(psr_ctx *c, int32_t param, int32_t value) {
     if (param == PSR_TUNE_DEPOSIT_ACCUM) {
         if (value != PSR_ACCUM_AUTO && value != PSR_ACCUM_ATOMIC && value != PSR_ACCUM_BINNED)
             PSR_FAIL(c, PSR_ERR_ARG, "unknown deposit accumulation");
+        if (value == PSR_ACCUM_BINNED && c->walk_req == PSR_WALK_MULTI)
+            PSR_FAIL(c, PSR_ERR_NOTSUP, "several rays per workgroup accumulate with atomics only");
         c->accum_req = value;
+        psr_tune_reset(c);
+    } else if (param == PSR_TUNE_DEPOSIT_WALK) {
+        if (value != PSR_WALK_AUTO && value != PSR_WALK_SINGLE && value != PSR_WALK_MULTI)
+            PSR_FAIL(c, PSR_ERR_ARG, "unknown deposit walk");
+        if (value == PSR_WALK_MULTI && c->accum_req == PSR_ACCUM_BINNED)
+            PSR_FAIL(c, PSR_ERR_NOTSUP, "several rays per workgroup accumulate with atomics only");
+        c->walk_req = value;
         psr_tune_reset(c);
     } else if (param == PSR_TUNE_TICK_GRAPH) {
         if (value < -1 || value > 1) PSR_FAIL(c, PSR_ERR_ARG, "tick graph: -1 (timed trials), 0 or 1");
@@ -1027,7 +1059,19 @@ extern "C" int psr_set_tuning(psr_ctx *c, int32_t param, int32_t value) {
 
 extern "C" int psr_get_tuning(psr_ctx *c, int32_t param, int32_t *value) {
     if (!c || !value) return PSR_ERR_ARG;
-    if (param == PSR_TUNE_DEPOSIT_ACCUM) *value = c->accum_req != PSR_ACCUM_AUTO ? c->accum_req : c->accum_use;
+    int cand[3];
+    const int nc = dep_candidates(c, cand);
+    const int mode = nc == 1 ? cand[0] : c->dep_use;
+    if (param == PSR_TUNE_DEPOSIT_ACCUM)
+        *value = c->accum_req != PSR_ACCUM_AUTO ? c->accum_req
+                 : mode < 0                     ? PSR_ACCUM_AUTO
+                 : mode == PSR_DEP_WG_BINNED    ? PSR_ACCUM_BINNED
+                                                : PSR_ACCUM_ATOMIC;
+    else if (param == PSR_TUNE_DEPOSIT_WALK)
+        *value = c->walk_req != PSR_WALK_AUTO ? c->walk_req
+                 : mode < 0                   ? PSR_WALK_AUTO
+                 : mode == PSR_DEP_MULTI       ? PSR_WALK_MULTI
+                                              : PSR_WALK_SINGLE;
     else if (param == PSR_TUNE_TICK_GRAPH) *value = c->graph_req >= 0 ? c->graph_req : c->graph_use;
     else if (param == PSR_TUNE_GRAPH_LAUNCHES) *value = (int32_t)std::min<int64_t>(c->graph_launches, INT32_MAX);
     else PSR_FAIL(c, PSR_ERR_ARG, "unknown tuning parameter");

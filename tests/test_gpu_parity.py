@@ -72,15 +72,23 @@ def assert_f32_sums(a, r, hits, name):
 
 
 CASES = [(1, 16, 64), (1, 32, 32), (2, 24, 256), (3, 20, 256)]
-ACCUM = [psr.PSR_ACCUM_ATOMIC, psr.PSR_ACCUM_BINNED]
+# deposit modes: one ray per workgroup with atomics or binned records, and
+# four rays per workgroup (atomics)
+ACCUM = [(psr.PSR_ACCUM_ATOMIC, psr.PSR_WALK_SINGLE), (psr.PSR_ACCUM_BINNED, psr.PSR_WALK_SINGLE),
+         (psr.PSR_ACCUM_ATOMIC, psr.PSR_WALK_MULTI)]
+ACCUM_IDS = ["atomic", "binned", "multi"]
 
 
 @pytest.fixture
 def accum(ctx, request):
-    """the deposit's accumulation, fixed for the test, automatic afterwards"""
-    ctx.set_tuning(psr.PSR_TUNE_DEPOSIT_ACCUM, request.param)
+    """the deposit's mode, fixed for the test, automatic afterwards"""
+    a, w = request.param
+    ctx.set_tuning(psr.PSR_TUNE_DEPOSIT_ACCUM, psr.PSR_ACCUM_AUTO)
+    ctx.set_tuning(psr.PSR_TUNE_DEPOSIT_WALK, w)
+    ctx.set_tuning(psr.PSR_TUNE_DEPOSIT_ACCUM, a)
     yield request.param
     ctx.set_tuning(psr.PSR_TUNE_DEPOSIT_ACCUM, psr.PSR_ACCUM_AUTO)
+    ctx.set_tuning(psr.PSR_TUNE_DEPOSIT_WALK, psr.PSR_WALK_AUTO)
 
 
 @pytest.mark.parametrize("cfg,n,nray", CASES)
@@ -342,7 +350,7 @@ def run_one_tick(ctx, wl, orc, phases, tick=1):
     return out, ref, rays, (ln, tm, st), off
 
 
-@pytest.mark.parametrize("accum", ACCUM, indirect=True)
+@pytest.mark.parametrize("accum", ACCUM, indirect=True, ids=ACCUM_IDS)
 @pytest.mark.parametrize("cfg,n,nray", [(1, 32, 32), (2, 24, 256), (3, 20, 128)])
 def test_deposit_parity(ctx, cfg, n, nray, accum):
     print_case("deposit", cfg, n, nray, accum)
@@ -363,7 +371,7 @@ def test_deposit_parity(ctx, cfg, n, nray, accum):
     assert s["saturated_deposit_count"] == st.saturated_deposit_count
 
 
-@pytest.mark.parametrize("accum", ACCUM, indirect=True)
+@pytest.mark.parametrize("accum", ACCUM, indirect=True, ids=ACCUM_IDS)
 @pytest.mark.parametrize("he", [False, True])
 def test_deposit_no_photon_and_cutoff(ctx, he, accum):
     """rays of negative type start with TM = 0 under no_photon (src/loop.c:411);
@@ -438,9 +446,9 @@ def test_binned_deposit_full_bins(he):
 
 
 def test_deposit_accumulation_is_tuned(ctx):
-    """automatic accumulation: the first four deposit ticks alternate atomic
-    and binned sums (each timed); a choice is made once they are done, and
-    every tick's sums are right whichever ran"""
+    """automatic deposit mode: the first six deposit ticks cycle through
+    atomic and binned sums and four rays per workgroup (each timed); a choice is made
+    once they are done, and every tick's sums are right whichever ran"""
     print_case("autotune")
     wl = make_workload(2, RATES, n=24, nray=256)
     orc = oracle_for(wl)
@@ -454,6 +462,7 @@ def test_deposit_accumulation_is_tuned(ctx):
         ctx.tick(t, wl.tick_time, phases=psr.PSR_PHASE_TRACE | psr.PSR_PHASE_DEPOSIT)
     ctx.sync()
     assert ctx.tuning() in (psr.PSR_ACCUM_ATOMIC, psr.PSR_ACCUM_BINNED)
+    assert ctx.tuning(psr.PSR_TUNE_DEPOSIT_WALK) in (psr.PSR_WALK_SINGLE, psr.PSR_WALK_MULTI)
     out = wl.ps.copy()
     ctx.state_download(out)
     ref = wl.ps.copy()
@@ -465,15 +474,17 @@ def test_deposit_accumulation_is_tuned(ctx):
         assert_f32_sums(getattr(out, fld), nt * getattr(ref, fld).astype(np.float64), nt * ref.hits, fld)
 
 
-@pytest.mark.parametrize("accum", [psr.PSR_ACCUM_AUTO, psr.PSR_ACCUM_ATOMIC])
-def test_overflowed_tick_is_replayed(accum):
+@pytest.mark.parametrize("accum,walk", [(psr.PSR_ACCUM_AUTO, psr.PSR_WALK_AUTO),
+                                        (psr.PSR_ACCUM_ATOMIC, psr.PSR_WALK_SINGLE),
+                                        (psr.PSR_ACCUM_ATOMIC, psr.PSR_WALK_MULTI)])
+def test_overflowed_tick_is_replayed(accum, walk):
     """ticks are launched without waiting for the trace's capacity check; a
     tick whose lists outgrow the capacity (2048 hits per ray at first) stops
     every later kernel and is replayed, with the ticks after it, at the next
     call -- the result equals three sequential oracle ticks.  With a fixed
     accumulation the ticks run as recorded graphs, including the overflowed
     one."""
-    print_case("overflow-replay", accum)
+    print_case("overflow-replay", accum, walk)
     wl = make_workload(1, RATES, n=104, nray=16)
     box = wl.boxsize
     r1, r2, r3 = wl.rays(1), wl.rays(2), wl.rays(3)
@@ -487,6 +498,7 @@ def test_overflowed_tick_is_replayed(accum):
     orc = oracle_for(wl)
     with psr.Context(0) as c2:
         c2.set_tuning(psr.PSR_TUNE_DEPOSIT_ACCUM, accum)
+        c2.set_tuning(psr.PSR_TUNE_DEPOSIT_WALK, walk)
         load(c2, wl)
         c2.rays_upload(pool)
         for t, (a, rr) in enumerate(((0, r1), (len(r1), r2), (len(r1) + len(r2), r3)), start=1):
@@ -509,9 +521,11 @@ def test_overflowed_tick_is_replayed(accum):
     assert np.abs(out.lambdaH - ref.lambdaH).max() < 1e-6
 
 
-@pytest.mark.parametrize("cfg,accum", [(2, psr.PSR_ACCUM_ATOMIC), (2, psr.PSR_ACCUM_BINNED),
-                                       (3, psr.PSR_ACCUM_BINNED)])
-def test_tick_graph_matches_kernel_by_kernel(cfg, accum):
+@pytest.mark.parametrize("cfg,accum,walk", [(2, psr.PSR_ACCUM_ATOMIC, psr.PSR_WALK_SINGLE),
+                                            (2, psr.PSR_ACCUM_BINNED, psr.PSR_WALK_SINGLE),
+                                            (3, psr.PSR_ACCUM_BINNED, psr.PSR_WALK_SINGLE),
+                                            (3, psr.PSR_ACCUM_ATOMIC, psr.PSR_WALK_MULTI)])
+def test_tick_graph_matches_kernel_by_kernel(cfg, accum, walk):
     """ten ticks recorded once as a hipGraph and replayed with each tick's
     record against the same ten ticks launched kernel by kernel, from the
     same state.  Two runs differ only through the f32 summation order of the
@@ -519,7 +533,7 @@ def test_tick_graph_matches_kernel_by_kernel(cfg, accum):
     which hit ends a few rays at the 1e-10 cut-off: full hit counts are
     equal, per-particle hits and last hits nearly everywhere, fractions and
     energies to rounding."""
-    print_case("tick-graph", cfg, accum)
+    print_case("tick-graph", cfg, accum, walk)
     wl = make_workload(cfg, RATES, n=24 if cfg == 2 else 16, nray=256)
     ticks = [wl.rays(t) for t in range(1, 11)]
     pool = np.concatenate(ticks)
@@ -528,6 +542,7 @@ def test_tick_graph_matches_kernel_by_kernel(cfg, accum):
     for graph in (0, 1):
         with psr.Context(0) as c2:
             c2.set_tuning(psr.PSR_TUNE_DEPOSIT_ACCUM, accum)
+            c2.set_tuning(psr.PSR_TUNE_DEPOSIT_WALK, walk)
             c2.set_tuning(psr.PSR_TUNE_TICK_GRAPH, graph)
             load(c2, wl)
             c2.rays_upload(pool)
