@@ -43,6 +43,8 @@ DevState psr_ctx::state() const {
     s.lasthit = d_lasthit;
     s.hits = d_hits;
     s.active = d_active;
+    s.dv = d_dview;
+    s.dv_he = dview_he;
     return s;
 }
 
@@ -141,6 +143,9 @@ static void free_tree(psr_ctx *c) {
 static void free_state(psr_ctx *c) {
     dfree(c->d_f32);
     dfree(c->d_f64);
+    dfree(c->d_dview);
+    c->dview_valid = false;
+    c->dview_he = -1;
     dfree(c->d_lasthit);
     dfree(c->d_hits);
     dfree(c->d_active);
@@ -753,6 +758,7 @@ extern "C" int psr_state_upload(psr_ctx *c, const psr_state *s) {
     if (!c->have_state) {
         PSR_HIP(c, hipMalloc(&c->d_f32, sizeof(float) * 11 * nal));
         PSR_HIP(c, hipMalloc(&c->d_f64, sizeof(double) * 3 * nal));
+        PSR_HIP(c, hipMalloc(&c->d_dview, sizeof(DepViewHe) * nal));
         PSR_HIP(c, hipMalloc(&c->d_lasthit, sizeof(int64_t) * nal));
         PSR_HIP(c, hipMalloc(&c->d_hits, sizeof(int32_t) * nal));
         PSR_HIP(c, hipMalloc(&c->d_active, nal));
@@ -783,6 +789,7 @@ extern "C" int psr_state_upload(psr_ctx *c, const psr_state *s) {
 #undef UP
     PSR_HIP(c, hipStreamSynchronize(c->stream));
     c->have_state = true;
+    c->dview_valid = false;   // rebuilt before the next deposit
     c->ctx_dirty = true;
     psr_graph_reset(c);
     return PSR_OK;

@@ -741,6 +741,14 @@ __device__ __forceinline__ void store_step(const DevState &S, uint32_t i, int64_
     S.lH[i] = nlH;
     S.lHeI[i] = nlHeI;
     S.lHeII[i] = nlHeII;
+    if (S.dv_he == 1) {
+        DepViewHe *v = (DepViewHe *)S.dv + i;
+        v->xHI = nlH;
+        v->xHeI = nlHeI;
+        v->xHeII = nlHeII;
+    } else if (S.dv_he == 0) {
+        ((DepViewH *)S.dv)[i].xHI = nlH;
+    }
     float ie_new = ADI ? S.ie[i] : (float)x[6];
     S.lasthit[i] = tick;
     if (ISO) ie_new = (float)Tye2ie(P, T, lambda_ye(P, nlH, nlHeI, nlHeII));

@@ -166,12 +166,39 @@ struct HitGather {
 
 template <bool HE>
 __device__ __forceinline__ void gather_hit(HitGather<HE> &g, const DevState &S) {
-    g.sml = gmem(S.ps)[g.slot].w;
-    g.mass = gmem(S.mass)[g.slot];
-    g.xHI = gmem(S.lH)[g.slot];
+    // one or two 16-byte loads: (sml, mass, xHI) and (xHeI, xHeII)
+    const auto *w = gmem((const uint32_t *)__builtin_assume_aligned(S.dv, 32)) +
+                    (HE ? 8 * (size_t)g.slot : 4 * (size_t)g.slot);
+    const uint32_t a0 = w[0], a1 = w[1], a2 = w[2], a3 = w[3];
+    g.sml = __uint_as_float(a0);
+    g.mass = __uint_as_float(a1);
+    g.xHI = __hiloint2double((int)a3, (int)a2);
     if constexpr (HE) {
-        g.xHeI = gmem(S.lHeI)[g.slot];
-        g.xHeII = gmem(S.lHeII)[g.slot];
+        const uint32_t b0 = w[4], b1 = w[5], b2 = w[6], b3 = w[7];
+        g.xHeI = __hiloint2double((int)b1, (int)b0);
+        g.xHeII = __hiloint2double((int)b3, (int)b2);
+    }
+}
+
+// the packed view from the particle arrays (after an upload or a change of
+// the H / H + He layout)
+__global__ void k_build_dview(DevState S, int64_t n, int he) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (he) {
+            DepViewHe v;
+            v.sml = S.ps[i].w;
+            v.mass = S.mass[i];
+            v.xHI = S.lH[i];
+            v.xHeI = S.lHeI[i];
+            v.xHeII = S.lHeII[i];
+            ((DepViewHe *)S.dv)[i] = v;
+        } else {
+            DepViewH v;
+            v.sml = S.ps[i].w;
+            v.mass = S.mass[i];
+            v.xHI = S.lH[i];
+            ((DepViewH *)S.dv)[i] = v;
+        }
     }
 }
 
@@ -243,16 +270,51 @@ struct DepRec<true> {
     float dHI, dHeI, dHeII, dheat;
 };
 
-// src/loop.c:479-573 for one live hit that saw TM = tm
+// the parts of src/loop.c:516-552 that depend on the particle state only:
+// x = ye NH / (NH + NHe) (capped at 1) and the secondary-ionization table
+// values at (dE_HI, log10 x) and (dE_HeI, log10 x)
+struct SecVals {
+    double x;
+    double vH[3], vHe[3];
+};
+
 template <bool HE>
-__device__ __forceinline__ DepRec<HE> apply_values(double tm, const WalkRec<HE> &v, const HitAux<HE> &au,
-                                                   const DevTables &T, const DevPhys &P, const RayConst &k,
-                                                   ApplySums &acc) {
+__device__ __forceinline__ SecVals hit_secion(double xHI, double xHeI, double xHeII, float massf,
+                                              const DevTables &T, const DevPhys &P, const RayConst &k) {
+    SecVals sv;
+    sv.x = 0.0;
+#pragma unroll
+    for (int t = 0; t < 3; t++) sv.vH[t] = sv.vHe[t] = 0.0;
+    if (P.secondary_ionization) {
+        const double mass = (double)massf;
+        const double NH = mass * P.C_H_PER_MASS;
+        const double NHe = mass * P.C_HE_PER_MASS;
+        if (!HE) xHeI = xHeII = 0.0;
+        const double ye = (1.0 - xHI) + P.C_HEMF / P.C_HMF * 0.25 * (xHeII + 2 * (1.0 - xHeII - xHeI));
+        double x = ye * NH / (NH + NHe);
+        if (x > 1.0) x = 1.0;
+        sv.x = x;
+        if (x > 0.0 && k.secion_terms) {
+            const double log10x = log10(x);
+            // phi_HI, phi_HeI, E_h at (dE_HI, log10 x) and at (dE_HeI, log10 x);
+            // hydrogen only, deltaHeI is 0 and so is its term for any table value
+            tab2_get3(T, k.dEHI, log10x, sv.vH);
+            if (HE) tab2_get3(T, k.dEHeI, log10x, sv.vHe);
+        }
+    }
+    return sv;
+}
+
+// src/loop.c:479-573 for one live hit that saw TM = tm, with the state-only
+// values sv of hit_secion
+template <bool HE>
+__device__ __forceinline__ DepRec<HE> apply_core(double tm, const WalkRec<HE> &v, uint32_t slot, float massf,
+                                                 const SecVals &sv, const DevPhys &P, const RayConst &k,
+                                                 ApplySums &acc) {
     double yHI = 0.0, yHeI = 0.0, yHeII = 0.0, ht = 0.0;
-    const uint32_t slot = au.slot;
     const double em = v.em;
     const double NHI = v.nhi;
-    const double mass = (double)au.mass;
+    const double mass = (double)massf;
     const double NH = mass * P.C_H_PER_MASS;
     const double NHe = mass * P.C_HE_PER_MASS;
     const double NH_inv = 1.0 / NH;
@@ -292,39 +354,22 @@ __device__ __forceinline__ DepRec<HE> apply_values(double tm, const WalkRec<HE> 
     const double deltaHI = aHI * NH_inv;
     const double deltaHeI = he ? aHeI * NHe_inv : 0.0;
     const double deltaHeII = he ? aHeII * NHe_inv : 0.0;
-    double x = 0.0;
-    if (P.secondary_ionization) {
-        const double xHI = au.xHI;
-        double xHeI = 0.0, xHeII = 0.0;
-        if constexpr (HE) {
-            xHeI = au.xHeI;
-            xHeII = au.xHeII;
-        }
-        const double ye = (1.0 - xHI) + P.C_HEMF / P.C_HMF * 0.25 * (xHeII + 2 * (1.0 - xHeII - xHeI));
-        x = ye * NH / (NH + NHe);
-        if (x > 1.0) x = 1.0;
-    }
-    if (P.secondary_ionization && x > 0.0) {
+    if (P.secondary_ionization && sv.x > 0.0) {
         yHI += deltaHI;
         acc.fiHI += deltaHI * NH;
         if (k.secion_terms) {
-            const double log10x = log10(x);
-            // phi_HI, phi_HeI, E_h at (dE_HI, log10 x) and at (dE_HeI, log10 x);
-            // hydrogen only, deltaHeI is 0 and so is its term for any table value
-            double vH[3], vHe[3] = {0.0, 0.0, 0.0};
-            tab2_get3(T, k.dEHI, log10x, vH);
-            if (HE) tab2_get3(T, k.dEHeI, log10x, vHe);
-            const double sec = deltaHI * k.dEHI / P.C_HI_ENERGY * vH[0] + deltaHeI * k.dEHeI / P.C_HI_ENERGY * vHe[0];
+            const double sec =
+                deltaHI * k.dEHI / P.C_HI_ENERGY * sv.vH[0] + deltaHeI * k.dEHeI / P.C_HI_ENERGY * sv.vHe[0];
             yHI += sec;
             acc.siHI += sec * NH;
-            ht += k.hf_HI * deltaHI * vH[2];
+            ht += k.hf_HI * deltaHI * sv.vH[2];
             if (he) {
                 const double sec2 =
-                    deltaHI * k.dEHI / P.C_HEI_ENERGY * vH[1] + deltaHeI * k.dEHeI / P.C_HEI_ENERGY * vHe[1];
+                    deltaHI * k.dEHI / P.C_HEI_ENERGY * sv.vH[1] + deltaHeI * k.dEHeI / P.C_HEI_ENERGY * sv.vHe[1];
                 yHeI += deltaHeI + sec2;
                 acc.siHeI += sec2 * NHe;
                 acc.fiHeI += deltaHeI * NHe;
-                ht += k.hf_HeI * deltaHeI * vHe[2];
+                ht += k.hf_HeI * deltaHeI * sv.vHe[2];
                 yHeII += deltaHeII;
                 acc.fiHeII += deltaHeII * NHe;
             }
@@ -353,6 +398,20 @@ __device__ __forceinline__ DepRec<HE> apply_values(double tm, const WalkRec<HE> 
         out.dHeII = (float)yHeII;
     }
     return out;
+}
+
+// src/loop.c:479-573 for one live hit that saw TM = tm
+template <bool HE>
+__device__ __forceinline__ DepRec<HE> apply_values(double tm, const WalkRec<HE> &v, const HitAux<HE> &au,
+                                                   const DevTables &T, const DevPhys &P, const RayConst &k,
+                                                   ApplySums &acc) {
+    double xHeI = 0.0, xHeII = 0.0;
+    if constexpr (HE) {
+        xHeI = au.xHeI;
+        xHeII = au.xHeII;
+    }
+    const SecVals sv = hit_secion<HE>(au.xHI, xHeI, xHeII, au.mass, T, P, k);
+    return apply_core<HE>(tm, v, au.slot, au.mass, sv, P, k, acc);
 }
 
 // a record added straight to its particle (records beyond a bin's room)
@@ -890,11 +949,17 @@ __global__ __launch_bounds__(ORD_THREADS) void k_ray_order(const int32_t *__rest
     }
 }
 
+// a ray's row of records (and of TMs) is padded by one entry, so that the
+// MR walking lanes, which read the same record index of different rays, hit
+// different LDS banks
+constexpr int MROW = MSEG + 1;
+
 template <bool HE>
 struct MLds {
-    WalkRec<HE> rec[3][MR * MSEG];
+    WalkRec<HE> rec[3][MR * MROW];
     HitAux<HE> aux[3][MR * MSEG];
-    double tmh[3][MR * MSEG];  // TM before each hit (walk -> apply)
+    double tmh[3][MR * MROW];  // TM before each hit (walk -> apply)
+    RayConst kc[MR];           // per-ray constants (cross sections, energies)
     int64_t ray[MR];           // ray index, -1 for an empty place in the batch
     int32_t n[MR];             // hits of the ray (0: nothing to deposit)
     int32_t endseg[MR];        // last segment to deposit (the cut's, once found)
@@ -951,6 +1016,7 @@ __global__ __launch_bounds__(MTHREADS, 3) void k_deposit_multi(const DevTick *__
                     n = 0;
                 }
             }
+            L.kc[p] = ray_const<HE>(n > 0 ? rays[r].freq : 0.0, T, P);
             L.ray[p] = r;
             L.n[p] = n;
             L.endseg[p] = n > 0 ? (n - 1) / MSEG : -1;
@@ -960,24 +1026,28 @@ __global__ __launch_bounds__(MTHREADS, 3) void k_deposit_multi(const DevTick *__
         __syncthreads();
         if (wv == 1) {
             // prep: hit j = h * 64 + lane of the iteration is hit (j % MSEG)
-            // of ray place j / MSEG; the slot / b of segment i + 2 and the
-            // particle gathers of segment i + 1 are in flight while segment i
-            // is computed (indexes clamped into each list)
+            // of ray place j / MSEG.  Four stages, one iteration apart: the
+            // slot / b of segment i + 3, the particle gathers of segment
+            // i + 2, the SPH table value of segment i + 1 (from the gathered
+            // b / sml) are in flight while segment i is computed (indexes
+            // clamped into each list)
             constexpr int HPL = 2;
-            RayConst kc[HPL];
             uint32_t a_slot[HPL];
             double a_b[HPL];
-            HitGather<HE> gc[HPL];
+            HitGather<HE> g1[HPL], g2[HPL];   // gathered: segment i (+ sd), segment i + 1
+            double sd[HPL];
 #pragma unroll
             for (int h = 0; h < HPL; h++) {
                 const int j = h * 64 + lane, p = j / MSEG, k = j % MSEG;
-                kc[h] = ray_const<HE>(L.n[p] > 0 ? rays[L.ray[p]].freq : 0.0, T, P);
-                mr_index<HE>(gc[h], H, L, p, k);
-                gather_hit<HE>(gc[h], S);
+                mr_index<HE>(g1[h], H, L, p, k);
+                gather_hit<HE>(g1[h], S);
+                mr_index<HE>(g2[h], H, L, p, MSEG + k);
+                gather_hit<HE>(g2[h], S);
                 HitGather<HE> t1;
-                mr_index<HE>(t1, H, L, p, MSEG + k);
+                mr_index<HE>(t1, H, L, p, 2 * MSEG + k);
                 a_slot[h] = t1.slot;
                 a_b[h] = t1.b;
+                sd[h] = sph_depth(T, g1[h].b * (1.0 / (double)g1[h].sml));
             }
             for (int i = 0;; i++) {
                 int mend = -1;
@@ -985,27 +1055,27 @@ __global__ __launch_bounds__(MTHREADS, 3) void k_deposit_multi(const DevTick *__
                 for (int p = 0; p < MR; p++) mend = max(mend, L.endseg[p]);
                 if (i > mend + 2) break;
                 const int ring = i % 3;
-                double sd[HPL];
-#pragma unroll
-                for (int h = 0; h < HPL; h++) sd[h] = sph_depth(T, gc[h].b * (1.0 / (double)gc[h].sml));
 #pragma unroll
                 for (int h = 0; h < HPL; h++) {
                     const int j = h * 64 + lane, p = j / MSEG, k = j % MSEG;
                     WalkRec<HE> rr;
                     HitAux<HE> au;
-                    prep_hit<HE>(gc[h], sd[h], P, kc[h], rr, au);
+                    const RayConst &kc = L.kc[p];
+                    prep_hit<HE>(g1[h], sd[h], P, kc, rr, au);
                     if (i * MSEG + k >= L.n[p]) noop_rec<HE>(rr, au);
-                    L.rec[ring][j] = rr;
+                    L.rec[ring][p * MROW + k] = rr;
                     L.aux[ring][j] = au;
                 }
 #pragma unroll
                 for (int h = 0; h < HPL; h++) {
                     const int j = h * 64 + lane, p = j / MSEG, k = j % MSEG;
-                    gc[h].slot = a_slot[h];
-                    gc[h].b = a_b[h];
-                    gather_hit<HE>(gc[h], S);
+                    g1[h] = g2[h];
+                    sd[h] = sph_depth(T, g1[h].b * (1.0 / (double)g1[h].sml));
+                    g2[h].slot = a_slot[h];
+                    g2[h].b = a_b[h];
+                    gather_hit<HE>(g2[h], S);
                     HitGather<HE> t2;
-                    mr_index<HE>(t2, H, L, p, (i + 2) * MSEG + k);
+                    mr_index<HE>(t2, H, L, p, (i + 3) * MSEG + k);
                     a_slot[h] = t2.slot;
                     a_b[h] = t2.b;
                 }
@@ -1030,20 +1100,45 @@ __global__ __launch_bounds__(MTHREADS, 3) void k_deposit_multi(const DevTick *__
                 if (i > mend + 2) break;
                 const int s = i - 1;
                 if (lane < MR && s >= 0 && s <= L.endseg[p]) {
+                    // groups of WG records in registers, each record reloaded
+                    // with the next group's as soon as its step has read it;
+                    // past the list end the records are no-ops (TM unchanged).
+                    // TM changes only at live hits, so the cut (the first
+                    // live hit after which TM < T0) is looked for only in a
+                    // group that ends below T0, from the TMs after each step
+                    constexpr int WG = 4;
                     const int ring = s % 3;
                     const int kn = min(MSEG, L.n[p] - s * MSEG);
-                    const WalkRec<HE> *R = &L.rec[ring][p * MSEG];
-                    double *TMH = &L.tmh[ring][p * MSEG];
-                    WalkRec<HE> A = R[0];
-                    for (int k = 0; k < kn; k++) {
-                        const WalkRec<HE> v = A;
-                        A = R[min(k + 1, MSEG - 1)];
-                        TMH[k] = t;
-                        t = walk_step<HE>(t, v);
-                        if (t < T0 && is_live(v.em)) {
-                            L.endseg[p] = s;
-                            L.last[p] = s * MSEG + k;
-                            break;
+                    const WalkRec<HE> *R = &L.rec[ring][p * MROW];
+                    double *TMH = &L.tmh[ring][p * MROW];
+                    WalkRec<HE> A[WG];
+#pragma unroll
+                    for (int q = 0; q < WG; q++) A[q] = R[q];
+                    for (int g = 0; g * WG < kn; g++) {
+                        double post[WG];
+                        bool lv[WG];
+                        const int g1 = min(g + 1, MSEG / WG - 1);
+#pragma unroll
+                        for (int q = 0; q < WG; q++) {
+                            TMH[g * WG + q] = t;
+                            lv[q] = is_live(A[q].em);
+                            t = walk_step<HE>(t, A[q]);
+                            post[q] = t;
+                            A[q] = R[g1 * WG + q];
+                        }
+                        if (t < T0) {
+                            int qc = -1;
+#pragma unroll
+                            for (int q = WG - 1; q >= 0; q--)
+                                if (lv[q] && post[q] < T0) qc = q;
+                            if (qc >= 0) {
+                                L.endseg[p] = s;
+                                L.last[p] = s * MSEG + g * WG + qc;
+#pragma unroll
+                                for (int q = 0; q < WG; q++)
+                                    if (q == qc) t = post[q];
+                                break;
+                            }
                         }
                     }
                     L.tm_end[p] = t;
@@ -1052,20 +1147,39 @@ __global__ __launch_bounds__(MTHREADS, 3) void k_deposit_multi(const DevTick *__
             }
             __builtin_amdgcn_s_setprio(0);
         } else {
-            // apply: one hit per lane of segment i - 2 (its walk and cut are done)
+            // apply: one hit per lane of segment i - 2 (its walk and cut are
+            // done).  The secondary-ionization values depend on the particle
+            // state only, so those of segment i - 1 (prepped one iteration
+            // ago) are looked up here and used next iteration: their loads
+            // are in flight meanwhile, and the wait for them never covers
+            // this iteration's atomics (issued after them)
             const int j = (wv - 2) * 64 + lane, p = j / MSEG, k = j % MSEG;
-            const RayConst kc = ray_const<HE>(L.n[p] > 0 ? rays[L.ray[p]].freq : 0.0, T, P);
+            const RayConst &kc = L.kc[p];
+            SecVals nxt{};
             for (int i = 0;; i++) {
                 int mend = -1;
 #pragma unroll
                 for (int q = 0; q < MR; q++) mend = max(mend, L.endseg[q]);
                 if (i > mend + 2) break;
+                const SecVals cur = nxt;
+                const int s1 = i - 1;
+                if (s1 >= 0 && L.n[p] > 0) {
+                    const HitAux<HE> &au = L.aux[s1 % 3][j];
+                    double xHeI = 0.0, xHeII = 0.0;
+                    if constexpr (HE) {
+                        xHeI = au.xHeI;
+                        xHeII = au.xHeII;
+                    }
+                    nxt = hit_secion<HE>(au.xHI, xHeI, xHeII, au.mass, T, P, kc);
+                }
                 const int s = i - 2;
                 if (s >= 0 && s <= L.endseg[p] && s * MSEG + k <= L.last[p]) {
                     const int ring = s % 3;
-                    const WalkRec<HE> v = L.rec[ring][j];
+                    const WalkRec<HE> v = L.rec[ring][p * MROW + k];
                     if (is_live(v.em)) {
-                        const DepRec<HE> rr = apply_values<HE>(L.tmh[ring][j], v, L.aux[ring][j], T, P, kc, acc);
+                        const HitAux<HE> &au = L.aux[ring][j];
+                        const DepRec<HE> rr =
+                            apply_core<HE>(L.tmh[ring][p * MROW + k], v, au.slot, au.mass, cur, P, kc, acc);
                         rec_atomic<HE>(S, rr);
                     }
                 }
@@ -1151,6 +1265,15 @@ static int ensure_bins(psr_ctx *c, bool he) {
 }
 
 int psr_prepare_deposit(psr_ctx *c, int mode) {
+    const int he = !c->F.h_only;
+    if (c->have_state && c->npar > 0 && (!c->dview_valid || c->dview_he != he)) {
+        c->dview_he = he;
+        c->ctx_dirty = true;
+        k_build_dview<<<(unsigned)std::min<int64_t>((c->npar + 255) / 256, 16384), 256, 0, c->stream>>>(
+            c->state(), c->npar, he);
+        PSR_HIP(c, hipGetLastError());
+        c->dview_valid = true;
+    }
     int rc = psr_sync_dev_ctx(c);
     if (rc) return rc;
     return mode == PSR_DEP_WG_BINNED ? ensure_bins(c, !c->F.h_only) : PSR_OK;

@@ -60,6 +60,19 @@ struct DevPhys {
 };
 
 // particle state pointers (tree order)
+// packed per-particle inputs of the deposit in tree order (sml, mass and the
+// fractions the optical depths need), so that a hit gathers one line instead
+// of one per array; the chemistry's write-back keeps the fractions current
+struct __attribute__((aligned(16))) DepViewH {
+    float sml, mass;
+    double xHI;
+};
+struct __attribute__((aligned(32))) DepViewHe {
+    float sml, mass;
+    double xHI, xHeI, xHeII;
+};
+static_assert(sizeof(DepViewH) == 16 && sizeof(DepViewHe) == 32, "the deposit reads the view as 16-byte words");
+
 struct DevState {
     const float4 *ps;
     const uint32_t *orig;
@@ -71,6 +84,8 @@ struct DevState {
     int64_t *lasthit;
     int32_t *hits;
     uint8_t *active;
+    void *dv;        // DepViewHe (dv_he = 1) or DepViewH (0) per particle; -1: not built
+    int32_t dv_he;
 };
 
 // state, tables and constants as one device-resident block, read by the
@@ -182,6 +197,9 @@ struct psr_ctx {
     uint32_t *d_orig = nullptr;
     float *d_f32 = nullptr;   // mass rho ie yeMET yGdep*3 yGrec*3 heat : 11 arrays
     double *d_f64 = nullptr;  // lambdaH lambdaHeI lambdaHeII
+    void *d_dview = nullptr;  // the deposit's packed view (32 B per particle)
+    bool dview_valid = false;
+    int32_t dview_he = -1;
     int64_t *d_lasthit = nullptr;
     int32_t *d_hits = nullptr;
     uint8_t *d_active = nullptr;
