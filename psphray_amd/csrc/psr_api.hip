@@ -12,7 +12,6 @@
 // pos -+ sml (order independent, so bit-exact), empty leaves keep their
 // geometric octant, internal boxes are the union of the 8 children.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cstdio>
@@ -533,18 +532,18 @@ extern "C" int psr_tree_build(psr_ctx *c, const float *pos, const float *sml, co
     if (n) {
         k_keys<<<nblk(n), 256, 0, st>>>(d_pos, n, d_fo, nfield, d_boxf, d_k0, d_i0, d_f0);
         TB_HIP(hipGetLastError());
-        size_t tb = 0;
-        TB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, d_k0, d_k1, d_i0, d_i1, (int)n, 0, 64, st));
-        size_t tb2 = 0;
-        TB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, d_f1, d_f0, d_i1, d_i0, (int)n, 0, 16, st));
-        if (psr_ensure_scratch(c, std::max(tb, tb2))) { cleanup(); return PSR_ERR_HIP; }
-        TB_HIP(hipcub::DeviceRadixSort::SortPairs(c->d_scratch, tb, d_k0, d_k1, d_i0, d_i1, (int)n, 0, 64, st));
+        // the keys are octree paths of 3 bits per level below the root
+        // (PSR_MAXDEPTH levels, 63 bits); particles outside the root carry
+        // all 64 bits set and sort last
+        const int kbits = 64;
+        const size_t tb = std::max(psr_sort_scratch_u64(n), psr_sort_scratch_u16(n));
+        if (psr_ensure_scratch(c, tb)) { cleanup(); return PSR_ERR_HIP; }
+        TB_HIP(psr_sort_pairs_u64(d_k0, d_i0, d_k1, d_i1, n, 0, kbits, c->d_scratch, st));
         if (nfield > 1) {
             k_gather_fid<<<nblk(n), 256, 0, st>>>(d_i1, d_f0, d_f1, n);
             int bits = 1;
             while ((1 << bits) < nfield) bits++;
-            tb2 = c->scratch_bytes;
-            TB_HIP(hipcub::DeviceRadixSort::SortPairs(c->d_scratch, tb2, d_f1, d_f0, d_i1, d_i0, (int)n, 0, bits, st));
+            TB_HIP(psr_sort_pairs_u16(d_f1, d_i1, d_f0, d_i0, n, 0, bits, c->d_scratch, st));
             // d_i0 holds the final order; regather keys from the unsorted copy
             k_gather_keys<<<nblk(n), 256, 0, st>>>(d_i0, d_k0, d_k1, n);
             TB_HIP(hipMemcpyAsync(d_i1, d_i0, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
@@ -622,10 +621,8 @@ extern "C" int psr_tree_build(psr_ctx *c, const float *pos, const float *sml, co
             TB_HIP(hipMalloc(&d_scan, sizeof(int32_t) * flag_cap));
         }
         k_flag_internal<<<nblk(nl), 256, 0, st>>>(nd_count, ls, nl, d_flagi);
-        size_t tb = 0;
-        TB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, d_flagi, d_scan, nl, st));
-        if (psr_ensure_scratch(c, tb)) { cleanup(); cleanup2(); return PSR_ERR_HIP; }
-        TB_HIP(hipcub::DeviceScan::ExclusiveSum(c->d_scratch, tb, d_flagi, d_scan, nl, st));
+        if (psr_ensure_scratch(c, psr_scan_scratch_i32(nl))) { cleanup(); cleanup2(); return PSR_ERR_HIP; }
+        TB_HIP(psr_exclusive_sum_i32(d_flagi, d_scan, nl, c->d_scratch, st));
         int32_t last_scan = 0, last_flag = 0;
         TB_HIP(hipMemcpyAsync(&last_scan, d_scan + nl - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
         TB_HIP(hipMemcpyAsync(&last_flag, d_flagi + nl - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));

@@ -300,6 +300,20 @@ void psr_graph_reset(psr_ctx *c);
 int psr_resolve(psr_ctx *c);
 int psr_solve_steps_dev(psr_ctx *c, psr_step *d_steps, int64_t n, double rtol);
 int psr_noalloc_init(int device);
+// stable LSD radix sort of (key, value) pairs by key bits [begin, end) and
+// an exclusive scan (psr_sort.hip); scratch of the size the *_scratch
+// functions give, inputs left unchanged
+size_t psr_sort_scratch_u64(int64_t n);
+size_t psr_sort_scratch_u16(int64_t n);
+size_t psr_sort_scratch_u32(int64_t n);
+size_t psr_scan_scratch_i32(int64_t n);
+hipError_t psr_sort_pairs_u64(const unsigned long long *kin, const uint32_t *vin, unsigned long long *kout,
+                              uint32_t *vout, int64_t n, int begin, int end, void *scratch, hipStream_t st);
+hipError_t psr_sort_pairs_u16(const uint16_t *kin, const uint32_t *vin, uint16_t *kout, uint32_t *vout, int64_t n,
+                              int begin, int end, void *scratch, hipStream_t st);
+hipError_t psr_sort_pairs_u32(const uint32_t *kin, const uint32_t *vin, uint32_t *kout, uint32_t *vout, int64_t n,
+                              int begin, int end, void *scratch, hipStream_t st);
+hipError_t psr_exclusive_sum_i32(const int32_t *in, int32_t *out, int64_t n, void *scratch, hipStream_t st);
 DevPhys psr_dev_phys(const psr_ctx *c);
 int psr_sync_dev_ctx(psr_ctx *c);
 int psr_ensure_scratch(psr_ctx *c, size_t bytes);
