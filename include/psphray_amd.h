@@ -199,6 +199,37 @@ int psr_state_download(psr_ctx *ctx, psr_state *s);
  * r[] array, src/loop.c:69) */
 int psr_rays_upload(psr_ctx *ctx, const psr_ray *rays, int64_t nray);
 
+/* ---- ray emission on the device (emit_rays, src/loop.c:163-373) ----
+ * psr_sources_upload: point sources (Source, src/psystem.h:7-24): positions
+ * [nsrc][3] (internal length), Ngamma_dot (photons per internal time),
+ * field (batched fields; NULL: 0), spectrum index (NULL: 0), lastemit
+ * (NULL: 0; src/sources.c:67-78).  Replaces the context's sources. */
+int psr_sources_upload(psr_ctx *ctx, int64_t nsrc, const double *pos, const double *ngamma_dot,
+                       const int32_t *field, const int32_t *spec, const int64_t *lastemit);
+/* spectra (src/spectra.c:50-60): spectrum k has the bins [offsets[k],
+ * offsets[k+1]) of photon energies (internal units) and cumulative weights
+ * cdf (non-decreasing, last 1); a monochromatic source has one bin */
+int psr_spectra_upload(psr_ctx *ctx, int32_t nspec, const int64_t *offsets, const double *energy,
+                       const double *cdf);
+/* the source rays of one tick, written into the ray pool at [pool_first,
+ * pool_first + *nray_out) (the pool grows as needed; a pending tick using
+ * that range is finished first).  per_field == 0: nray rays drawn over all
+ * sources; else per_field rays for each field with a source.  Streams:
+ * Philox4x32-10 keyed (seed, 0x50535248), counter (ray id, tick, block):
+ * block 0 word 0 the source, with weight Ngamma_dot * (tick - lastemit) *
+ * tick_time; blocks 1.. the isotropic direction (gsl_ran_dir_3d's
+ * rejection method, src/loop.c:250); block 0 word 3 the spectrum bin.  Rays
+ * in order of source, stably; a source's rays share its weight equally
+ * (src/loop.c:305-321); length 2 * boxsize; type -1.  Sources that emitted
+ * get lastemit = tick.  Ticks must increase from call to call. */
+int psr_emit_rays(psr_ctx *ctx, int64_t tick, double tick_time, int64_t nray, int64_t per_field,
+                  uint32_t seed, int64_t pool_first, int64_t *nray_out);
+/* the sources' lastemit ticks */
+int psr_sources_lastemit(psr_ctx *ctx, int64_t *lastemit);
+/* a copy of nray rays of the pool from first (the reference keeps r[] on
+ * the host, src/loop.c:69; here for checks and ray logs, src/loop.c:80-85) */
+int psr_rays_download(psr_ctx *ctx, int64_t first, int64_t nray, psr_ray *rays);
+
 /* parity mode: rt_trace + qsort for each ray of the pool range
  * [ray_first, ray_first + nray) (src/raytrace.c:153-210,
  * src/loop.c:375-386).  Lists are sorted ascending by (d, ipar).  offsets

@@ -1517,3 +1517,141 @@ int32_t or_tick(or_ctx *c, const or_tree *t, or_psys *p, const or_ray *rays, int
     free(offsets);
     return rc;
 }
+
+
+/* ------------------------------------------------------------- emission */
+/* Philox4x32-10 (Salmon et al., SC'11): ten rounds of the two multiplies
+ * and the Weyl key schedule */
+static void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; r++) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0, n1 = (uint32_t)p1;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1, n3 = (uint32_t)p0;
+        c[0] = n0;
+        c[1] = n1;
+        c[2] = n2;
+        c[3] = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+static double unif(uint32_t x) { return ((double)x + 0.5) * (1.0 / 4294967296.0); }
+
+/* count of the n ascending values a[i] / div that are <= u */
+static int64_t count_le(const double *a, int64_t n, double div, double u) {
+    int64_t k = 0;
+    while (k < n && a[k] / div <= u) k++;
+    return k;
+}
+
+int64_t or_emit_rays(int64_t nsrc, const double *pos, const double *ngd, const int32_t *field,
+                     const int32_t *spec, int64_t *lastemit, const int64_t *spec_off,
+                     const double *spec_eng, const double *spec_cdf, int32_t nfield,
+                     const double *boxsizes, int64_t tick, double tick_time, int64_t nray,
+                     int64_t per_field, uint32_t seed, or_ray *out, int64_t cap) {
+    const uint32_t K1 = 0x50535248u;
+    const uint32_t tlo = (uint32_t)tick, thi = (uint32_t)((uint64_t)tick >> 32);
+    int64_t *grp = malloc(sizeof(int64_t) * (nsrc > 0 ? nsrc : 1));
+    double *w = malloc(sizeof(double) * (nsrc > 0 ? nsrc : 1));
+    double *cdf = malloc(sizeof(double) * (nsrc > 0 ? nsrc : 1));
+    uint32_t *cnt = calloc(nsrc > 0 ? nsrc : 1, sizeof(uint32_t));
+    uint8_t *emitted = calloc(nsrc > 0 ? nsrc : 1, 1);
+    int64_t nout = 0;
+    const int32_t ngroups = per_field == 0 ? 1 : nfield;
+    /* the source weights, src/sources.c:67-78 */
+    for (int64_t i = 0; i < nsrc; i++) w[i] = ngd[i] * (double)(tick - lastemit[i]) * tick_time;
+    for (int32_t g = 0; g < ngroups; g++) {
+        int64_t ng = 0;
+        for (int64_t i = 0; i < nsrc; i++)
+            if (per_field == 0 || field[i] == g) grp[ng++] = i;
+        const int64_t nr = per_field == 0 ? nray : per_field;
+        const int64_t id0 = per_field == 0 ? 0 : (int64_t)g * per_field;
+        if (nr == 0 || ng == 0) continue;
+        double run = 0.0;
+        for (int64_t k = 0; k < ng; k++) {
+            run += w[grp[k]];
+            cdf[k] = run;
+        }
+        if (!(cdf[ng - 1] > 0)) continue;
+        if (nout + nr > cap) {
+            nout = -1;
+            break;
+        }
+        /* each ray's source (a discrete draw by weight, src/loop.c:174-190) */
+        int64_t *src_of = malloc(sizeof(int64_t) * nr);
+        for (int64_t k = 0; k < ng; k++) cnt[grp[k]] = 0;
+        for (int64_t r = 0; r < nr; r++) {
+            uint32_t c[4] = {(uint32_t)(id0 + r), tlo, thi, 0u};
+            philox(c, seed, K1);
+            int64_t pick = count_le(cdf, ng, cdf[ng - 1], unif(c[0]));
+            if (pick > ng - 1) pick = ng - 1;
+            src_of[r] = grp[pick];
+            cnt[grp[pick]]++;
+        }
+        /* rays ordered by source, each source's in ray-id order (the
+         * reference sorts by (type, isrc), src/loop.c:236) */
+        int64_t *start = malloc(sizeof(int64_t) * (nsrc > 0 ? nsrc : 1));
+        int64_t acc = 0;
+        for (int64_t k = 0; k < ng; k++) {
+            start[grp[k]] = acc;
+            acc += cnt[grp[k]];
+        }
+        for (int64_t r = 0; r < nr; r++) {
+            const int64_t s = src_of[r];
+            or_ray *o = out + nout + start[s]++;
+            uint32_t c[4] = {(uint32_t)(id0 + r), tlo, thi, 0u};
+            philox(c, seed, K1);
+            /* isotropic direction: gsl_ran_dir_3d's rejection method
+             * (src/loop.c:250), two tries per block of the stream */
+            double x = 0.0, y = 0.0, z = 1.0;
+            int done = 0;
+            for (uint32_t blk = 1; blk <= 32 && !done; blk++) {
+                uint32_t d[4] = {(uint32_t)(id0 + r), tlo, thi, blk};
+                philox(d, seed, K1);
+                for (int h = 0; h < 2 && !done; h++) {
+                    const double xx = -1.0 + 2.0 * unif(d[2 * h]);
+                    const double yy = -1.0 + 2.0 * unif(d[2 * h + 1]);
+                    const double ss = xx * xx + yy * yy;
+                    if (ss <= 1.0) {
+                        const double sc = 2.0 * sqrt(1.0 - ss);
+                        x = xx * sc;
+                        y = yy * sc;
+                        z = -1.0 + 2.0 * ss;
+                        done = 1;
+                    }
+                }
+            }
+            for (int k = 0; k < 3; k++) o->s[k] = (float)pos[3 * s + k];
+            o->dir[0] = (float)x;
+            o->dir[1] = (float)y;
+            o->dir[2] = (float)z;
+            /* a source's rays share its weight, src/loop.c:305-321 */
+            o->Nph = w[s] * (1.0 / (double)cnt[s]);
+            /* photon energy: a bin of the source's spectrum, src/spectra.c:50-60 */
+            const int64_t a = spec_off[spec[s]], nb = spec_off[spec[s] + 1] - a;
+            int64_t ib = 0;
+            const double u3 = unif(c[3]);
+            while (ib < nb && spec_cdf[a + ib] <= u3) ib++;
+            if (ib > nb - 1) ib = nb - 1;
+            o->freq = spec_eng[a + ib];
+            o->length = boxsizes[field[s]] * 2;  /* src/loop.c:267 */
+            o->type = -1;
+            o->field = field[s];
+            o->src_or_par = s;
+            emitted[s] = 1;
+        }
+        nout += nr;
+        free(start);
+        free(src_of);
+    }
+    if (nout >= 0)
+        for (int64_t i = 0; i < nsrc; i++)
+            if (emitted[i]) lastemit[i] = tick;  /* src/loop.c:335-338 */
+    free(grp);
+    free(w);
+    free(cdf);
+    free(cnt);
+    free(emitted);
+    return nout;
+}

@@ -210,6 +210,18 @@ int32_t or_ros3_fixed(const or_ctx *c, const or_step *s, int32_t nsub, double *x
 int32_t or_step_evolve_many(const or_ctx *c, or_step *s, int64_t n, double rtol,
                             int32_t nthreads);
 
+/* emit_rays (src/loop.c:163-373) for point sources with counter-based
+ * streams (Philox4x32-10, key (seed, 0x50535248), counter (ray id, tick lo,
+ * tick hi, block)) in place of the GSL ones.  Groups: every source
+ * (per_field == 0, nray rays, ids from 0) or each field's sources
+ * (per_field rays, ids from field * per_field).  Writes the rays into out
+ * (capacity cap), advances lastemit, returns the ray count (-1: cap). */
+int64_t or_emit_rays(int64_t nsrc, const double *pos, const double *ngd, const int32_t *field,
+                     const int32_t *spec, int64_t *lastemit, const int64_t *spec_off,
+                     const double *spec_eng, const double *spec_cdf, int32_t nfield,
+                     const double *boxsizes, int64_t tick, double tick_time, int64_t nray,
+                     int64_t per_field, uint32_t seed, or_ray *out, int64_t cap);
+
 /* units helpers: src/config.h:295-303 */
 double or_ieye2T(const or_ctx *c, double ie, double ye);
 double or_Tye2ie(const or_ctx *c, double T, double ye);

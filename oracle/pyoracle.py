@@ -94,6 +94,8 @@ _SIG = {
     "or_ieye2T": (_D, [_VP, _D, _D]),
     "or_Tye2ie": (_D, [_VP, _D, _D]),
     "or_set_flags": (None, [_VP, C.POINTER(OrFlags)]),
+    "or_emit_rays": (_I64, [_I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _I32, _VP, _I64, _D, _I64,
+                            _I64, C.c_uint32, _VP, _I64]),
 }
 
 _lib = None
@@ -349,3 +351,33 @@ def tick(orc, tree, ps, rays, tick_no, tick_time, nthreads=8):
     rc = lib().or_tick(orc.h, tree.h, C.byref(s), p(rays), len(rays), tick_no, tick_time,
                        C.byref(st), nthreads)
     return rc, st
+
+
+class Emitter:
+    """the oracle's emit_rays over point sources (or_emit_rays): the same
+    sources, spectra and lastemit bookkeeping as a RayEmitter"""
+
+    def __init__(self, seed, pos, ngd, field, spec, spectra, boxsizes, tick_time):
+        self.seed = int(seed) & 0xFFFFFFFF
+        self.pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        self.ngd = np.ascontiguousarray(ngd, np.float64)
+        self.field = np.ascontiguousarray(field, np.int32)
+        self.spec = np.ascontiguousarray(spec, np.int32)
+        self.off = np.zeros(len(spectra) + 1, np.int64)
+        self.off[1:] = np.cumsum([len(e) for e, _ in spectra])
+        self.eng = np.ascontiguousarray(np.concatenate([e for e, _ in spectra]), np.float64)
+        self.cdf = np.ascontiguousarray(np.concatenate([c for _, c in spectra]), np.float64)
+        self.boxsizes = np.ascontiguousarray(boxsizes, np.float64)
+        self.tick_time = float(tick_time)
+        self.lastemit = np.zeros(len(self.ngd), np.int64)
+
+    def emit(self, tick, nray, per_field=None):
+        pf = int(per_field or 0)
+        cap = pf * len(self.boxsizes) if pf else int(nray)
+        out = np.zeros(max(cap, 1), RAY_DTYPE)
+        n = lib().or_emit_rays(len(self.ngd), p(self.pos), p(self.ngd), p(self.field), p(self.spec),
+                               p(self.lastemit), p(self.off), p(self.eng), p(self.cdf), len(self.boxsizes),
+                               p(self.boxsizes), int(tick), self.tick_time, int(nray), pf, self.seed,
+                               p(out), cap)
+        assert n >= 0
+        return out[:n]

@@ -243,6 +243,48 @@ class Context:
         self._check(self._lib.psr_rays_upload(self.h, ptr(rays), len(rays)), "psr_rays_upload")
         self._nray_pool = len(rays)
 
+    def set_sources(self, emitter):
+        """the sources and spectra of a RayEmitter (and its lastemit ticks)
+        for device emission: psr_sources_upload + psr_spectra_upload"""
+        pos = np.ascontiguousarray(emitter.pos, np.float64)
+        ngd = np.ascontiguousarray(emitter.ngd, np.float64)
+        fld = np.ascontiguousarray(emitter.field, np.int32)
+        spc = np.ascontiguousarray(emitter.spec, np.int32)
+        le = np.ascontiguousarray(emitter.lastemit, np.int64)
+        self._check(self._lib.psr_sources_upload(self.h, len(ngd), ptr(pos), ptr(ngd), ptr(fld), ptr(spc),
+                                                 ptr(le)), "psr_sources_upload")
+        off = np.zeros(len(emitter.spectra) + 1, np.int64)
+        off[1:] = np.cumsum([len(e) for e, _ in emitter.spectra])
+        eng = np.ascontiguousarray(np.concatenate([e for e, _ in emitter.spectra]), np.float64)
+        cdf = np.ascontiguousarray(np.concatenate([c for _, c in emitter.spectra]), np.float64)
+        self._check(self._lib.psr_spectra_upload(self.h, len(emitter.spectra), ptr(off), ptr(eng), ptr(cdf)),
+                    "psr_spectra_upload")
+        self._emit_seed = emitter.seed
+        self._emit_tick_time = emitter.tick_time
+
+    def emit_rays(self, tick, nray, per_field=None, pool_first=0, seed=None, tick_time=None):
+        """psr_emit_rays: the tick's source rays into the pool at pool_first;
+        returns the ray count"""
+        n = C.c_int64(0)
+        seed = self._emit_seed if seed is None else seed
+        tt = self._emit_tick_time if tick_time is None else tick_time
+        self._check(self._lib.psr_emit_rays(self.h, int(tick), float(tt), int(nray), int(per_field or 0),
+                                            int(seed) & 0xFFFFFFFF, int(pool_first), C.byref(n)),
+                    "psr_emit_rays")
+        self._nray_pool = max(getattr(self, "_nray_pool", 0), int(pool_first) + n.value)
+        return n.value
+
+    def sources_lastemit(self, nsrc):
+        le = np.zeros(nsrc, np.int64)
+        self._check(self._lib.psr_sources_lastemit(self.h, ptr(le)), "psr_sources_lastemit")
+        return le
+
+    def rays_download(self, first, n):
+        """n rays of the pool from first (for tests): psr_rays_download"""
+        out = np.zeros(n, RAY_DTYPE)
+        self._check(self._lib.psr_rays_download(self.h, int(first), int(n), ptr(out)), "psr_rays_download")
+        return out
+
     def trace_lists(self, ray_first=0, nray=None):
         """per-ray hit lists sorted by (d, ipar): offsets, ipar, d, b"""
         if nray is None:

@@ -130,6 +130,11 @@ SIGNATURES = {
     "psr_get_tuning": (C.c_int, [_VP, C.c_int32, _VP]),
     "psr_solve_steps": (C.c_int, [_VP, _VP, _I64, _D]),
     "psr_solve_steps_noalloc": (C.c_int, [_VP, _VP, _I64, _D]),
+    "psr_sources_upload": (C.c_int, [_VP, _I64, _VP, _VP, _VP, _VP, _VP]),
+    "psr_spectra_upload": (C.c_int, [_VP, C.c_int32, _VP, _VP, _VP]),
+    "psr_emit_rays": (C.c_int, [_VP, _I64, _D, _I64, _I64, C.c_uint32, _I64, C.POINTER(C.c_int64)]),
+    "psr_sources_lastemit": (C.c_int, [_VP, _VP]),
+    "psr_rays_download": (C.c_int, [_VP, _I64, _I64, _VP]),
     "psr_mlp_features": (C.c_int, [_VP, C.POINTER(Units), _VP]),
     "psr_mlp_train": (C.c_int, [_VP, _VP, _VP, _I64, _I32, _I32, C.c_float, C.c_uint64, _VP]),
     "psr_mlp_forward": (C.c_int, [_VP, _VP, _I64, _VP]),

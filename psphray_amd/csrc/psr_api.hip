@@ -185,6 +185,7 @@ extern "C" void psr_destroy(psr_ctx *c) {
     for (int k = 0; k < 2 * psr_ctx::TUNE_N; k++)
         if (c->gtune_ev[k]) (void)hipEventDestroy(c->gtune_ev[k]);
     psr_graph_reset(c);
+    psr_emit_free(c);
     dfree(c->d_tick);
     if (c->d_scratch) (void)hipFree(c->d_scratch);
     (void)hipStreamDestroy(c->stream);
@@ -484,6 +485,7 @@ extern "C" int psr_tree_build(psr_ctx *c, const float *pos, const float *sml, co
     free_state(c);
     c->npar = npar;
     c->nfield = nfield;
+    c->boxsizes.assign(boxsizes, boxsizes + nfield);
     const int64_t n = npar;
     const int64_t nal = n > 0 ? n : 1;
 
@@ -851,6 +853,15 @@ extern "C" int psr_rays_upload(psr_ctx *c, const psr_ray *rays, int64_t nray) {
     PSR_HIP(c, hipStreamSynchronize(c->stream));
     c->nray_pool = nray;
     c->ray_field_max = maxf;
+    return PSR_OK;
+}
+
+extern "C" int psr_rays_download(psr_ctx *c, int64_t first, int64_t nray, psr_ray *rays) {
+    if (!c || first < 0 || nray < 0 || (nray > 0 && !rays)) return PSR_ERR_ARG;
+    if (first + nray > c->nray_pool) PSR_FAIL(c, PSR_ERR_ARG, "psr_rays_download: range outside the pool");
+    PSR_HIP(c, hipSetDevice(c->device));
+    PSR_HIP(c, hipStreamSynchronize(c->stream));
+    if (nray) PSR_HIP(c, hipMemcpy(rays, c->d_rays + first, sizeof(psr_ray) * nray, hipMemcpyDeviceToHost));
     return PSR_OK;
 }
 

@@ -168,6 +168,7 @@ struct PendingTick {
 #define PSR_RING 256
 
 struct MlpState;
+struct EmitState;
 
 struct psr_ctx {
     int device = 0;
@@ -184,6 +185,7 @@ struct psr_ctx {
     bool have_tree = false;
     int64_t npar = 0;
     int32_t nfield = 0;
+    std::vector<double> boxsizes;  // per field
     int64_t nnode = 0;
     int32_t depth = 0;
     int64_t skipped = 0;
@@ -207,6 +209,7 @@ struct psr_ctx {
     psr_ray *d_rays = nullptr;
     int64_t nray_pool = 0;
     int32_t ray_field_max = -1;   // largest field index in the pool
+    EmitState *emit = nullptr;    // device ray emission: sources, spectra
     int64_t ray_cap = 0;
     // hit slots
     DevHits H{};
@@ -300,6 +303,7 @@ void psr_graph_reset(psr_ctx *c);
 int psr_resolve(psr_ctx *c);
 int psr_solve_steps_dev(psr_ctx *c, psr_step *d_steps, int64_t n, double rtol);
 int psr_noalloc_init(int device);
+void psr_emit_free(psr_ctx *c);
 // stable LSD radix sort of (key, value) pairs by key bits [begin, end) and
 // an exclusive scan (psr_sort.hip); scratch of the size the *_scratch
 // functions give, inputs left unchanged

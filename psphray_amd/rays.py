@@ -5,10 +5,23 @@ streams keyed by (seed, tick, ray), so the GPU path and the CPU oracle consume
 identical rays.  The list of a tick is a function of those keys and of each
 source's lastemit tick (src/sources.c:67-78), which emit() advances like the
 reference: a RayEmitter must emit ticks 1, 2, 3, ... in order, once each
-(a fresh emitter replays the same sequence).  The source choice follows the reference's weights
-Ngamma_dot * (tick - lastemit) * tick_time (src/sources.c:67-78), rays from
-one source share its weight equally (src/loop.c:305-321), directions are
-isotropic and the length is 2 * boxsize (src/loop.c:267).
+(a fresh emitter replays the same sequence).
+
+The steps follow the reference (src/loop.c:163-373):
+- the source of each ray is drawn with weight Ngamma_dot * (tick - lastemit)
+  * tick_time (src/sources.c:67-78) from word 0 of the ray's block 0;
+- rays are ordered by source, stably (the reference's qsort by (type, isrc),
+  src/loop.c:236), and a source's rays share its weight equally
+  (src/loop.c:305-321);
+- the direction is isotropic, drawn as gsl_ran_dir_3d draws it (src/loop.c:250:
+  x, y uniform in [-1, 1] until s = x^2 + y^2 <= 1, then z = -1 + 2 s and
+  (x, y) scaled by 2 sqrt(1 - s); Knop, CACM 13, 326, 1970) from blocks
+  1, 2, ... of the ray's stream, two tries per block;
+- the photon energy is a bin of the source's spectrum drawn from word 3 of
+  block 0 (src/spectra.c:50-60);
+- the length is 2 * boxsize (src/loop.c:267).
+Every operation is a correctly rounded +, *, / or sqrt, so the device and the
+oracle reproduce the list bit for bit.
 """
 import numpy as np
 
@@ -19,6 +32,8 @@ _M1 = np.uint64(0xCD9E8D57)
 _W0 = np.uint32(0x9E3779B9)
 _W1 = np.uint32(0xBB67AE85)
 _MASK = np.uint64(0xFFFFFFFF)
+KEY1 = 0x50535248        # second key word of the emission streams
+DIR_BLOCKS = 32          # blocks of direction tries (64 tries) before the fallback (0, 0, 1)
 
 
 def philox4x32(c0, c1, c2, c3, k0, k1, rounds=10):
@@ -45,15 +60,51 @@ def uniform01(x):
     return (x.astype(np.float64) + 0.5) * (1.0 / 4294967296.0)
 
 
+def isotropic_dirs(ids, tick, seed):
+    """unit vectors (float64 x, y, z) for ray ids: the rejection method of
+    gsl_ran_dir_3d on blocks 1..DIR_BLOCKS of each ray's stream"""
+    n = len(ids)
+    x = np.zeros(n)
+    y = np.zeros(n)
+    z = np.ones(n)
+    done = np.zeros(n, bool)
+    lo = np.full(n, tick & 0xFFFFFFFF, np.uint32)
+    hi = np.full(n, tick >> 32, np.uint32)
+    for blk in range(1, DIR_BLOCKS + 1):
+        todo = np.nonzero(~done)[0]
+        if len(todo) == 0:
+            break
+        w = philox4x32(ids[todo], lo[todo], hi[todo], np.full(len(todo), blk, np.uint32), seed, KEY1)
+        for a, b in ((w[0], w[1]), (w[2], w[3])):
+            xx = -1.0 + 2.0 * uniform01(a)
+            yy = -1.0 + 2.0 * uniform01(b)
+            s = xx * xx + yy * yy
+            ok = (s <= 1.0) & ~done[todo]
+            sel = todo[ok]
+            sc = 2.0 * np.sqrt(1.0 - s[ok])
+            x[sel] = xx[ok] * sc
+            y[sel] = yy[ok] * sc
+            z[sel] = -1.0 + 2.0 * s[ok]
+            done[sel] = True
+    return x, y, z
+
+
+def spectrum_cdf(weights):
+    """cumulative weights normalised by their (sequential) sum"""
+    c = np.cumsum(np.asarray(weights, np.float64))
+    return c / c[-1]
+
+
 class RayEmitter:
     """Emits the ray list of each tick for a set of point sources.
 
     sources: pos (ns,3) float64, ngamma_dot (ns,) photons per internal time,
-    field (ns,) int (batched fields, default 0); spectrum: (energies, weights)
-    in internal energy units (one bin = monochromatic)."""
+    field (ns,) int (batched fields, default 0); spectrum: (energies,
+    weights) in internal energy units (one bin = monochromatic), or a list of
+    such spectra with spec (ns,) naming each source's."""
 
     def __init__(self, seed, pos, ngamma_dot, boxsize, tick_time, spectrum, field=None,
-                 boxsizes=None):
+                 boxsizes=None, spec=None):
         self.seed = int(seed) & 0xFFFFFFFF
         self.pos = np.asarray(pos, np.float64).reshape(-1, 3)
         self.ngd = np.asarray(ngamma_dot, np.float64)
@@ -63,31 +114,37 @@ class RayEmitter:
             boxsizes = np.full(int(self.field.max()) + 1, boxsize, np.float64)
         self.boxsizes = np.asarray(boxsizes, np.float64)
         self.tick_time = float(tick_time)
-        eng, w = spectrum
-        self.eng = np.atleast_1d(np.asarray(eng, np.float64))
-        w = np.atleast_1d(np.asarray(w, np.float64))
-        self.spec_cdf = np.cumsum(w) / w.sum()
+        spectra = spectrum if isinstance(spectrum, list) else [spectrum]
+        self.spectra = [(np.atleast_1d(np.asarray(e, np.float64)), spectrum_cdf(np.atleast_1d(w)))
+                        for e, w in spectra]
+        self.spec = np.zeros(self.ns, np.int32) if spec is None else np.asarray(spec, np.int32)
         self.lastemit = np.zeros(self.ns, np.int64)
+
+    def weights(self, tick):
+        """Ngamma_dot * (tick - lastemit) * tick_time (src/sources.c:67-78)"""
+        return self.ngd * (int(tick) - self.lastemit) * self.tick_time
+
+    def groups(self, nray, per_field=None):
+        """(sources, ray count, first ray id) per emission group: one group of
+        every source, or one per field with per_field rays"""
+        if per_field is None:
+            return [(np.arange(self.ns), int(nray), 0)]
+        nf = len(self.boxsizes)
+        return [(np.nonzero(self.field == f)[0], int(per_field), f * int(per_field)) for f in range(nf)]
 
     def emit(self, tick, nray, per_field=None):
         """rays of one tick.  per_field: rays per field (batched runs), else
         nray rays over all sources."""
         tick = int(tick)
-        if per_field is None:
-            groups = [(np.arange(self.ns), int(nray), 0)]
-        else:
-            nf = len(self.boxsizes)
-            groups = [(np.nonzero(self.field == f)[0], int(per_field), f * int(per_field))
-                      for f in range(nf)]
         out = []
-        for srcs, nr, ray0 in groups:
+        for srcs, nr, ray0 in self.groups(nray, per_field):
             if nr == 0 or len(srcs) == 0:
                 continue
             ids = np.arange(nr, dtype=np.uint32) + np.uint32(ray0)
-            r0, r1, r2, r3 = philox4x32(ids, np.full(nr, tick & 0xFFFFFFFF, np.uint32),
-                                        np.full(nr, tick >> 32, np.uint32),
-                                        np.zeros(nr, np.uint32), self.seed, 0x50535248)
-            wts = self.ngd[srcs] * (tick - self.lastemit[srcs]) * self.tick_time
+            r0, _, _, r3 = philox4x32(ids, np.full(nr, tick & 0xFFFFFFFF, np.uint32),
+                                      np.full(nr, tick >> 32, np.uint32),
+                                      np.zeros(nr, np.uint32), self.seed, KEY1)
+            wts = self.weights(tick)[srcs]
             cdf = np.cumsum(wts)
             if cdf[-1] <= 0:
                 continue
@@ -96,20 +153,25 @@ class RayEmitter:
             isrc = srcs[pick]
             order = np.argsort(isrc, kind="stable")  # qsort by (type, isrc), src/loop.c:236
             isrc = isrc[order]
-            r1, r2, r3 = r1[order], r2[order], r3[order]
+            ids = ids[order]
+            r3 = r3[order]
             cnt = np.bincount(isrc, minlength=self.ns)
             rays = np.zeros(nr, RAY_DTYPE)
-            ct = 2.0 * uniform01(r1) - 1.0
-            phi = 2.0 * np.pi * uniform01(r2)
-            st = np.sqrt(np.maximum(0.0, 1.0 - ct * ct))
-            rays["dir"][:, 0] = (st * np.cos(phi)).astype(np.float32)
-            rays["dir"][:, 1] = (st * np.sin(phi)).astype(np.float32)
-            rays["dir"][:, 2] = ct.astype(np.float32)
+            dx, dy, dz = isotropic_dirs(ids, tick, self.seed)
+            rays["dir"][:, 0] = dx.astype(np.float32)
+            rays["dir"][:, 1] = dy.astype(np.float32)
+            rays["dir"][:, 2] = dz.astype(np.float32)
             rays["s"] = self.pos[isrc].astype(np.float32)
-            w_all = self.ngd * (tick - self.lastemit) * self.tick_time
+            w_all = self.weights(tick)
             rays["Nph"] = w_all[isrc] * (1.0 / cnt[isrc])
-            ib = np.searchsorted(self.spec_cdf, uniform01(r3), side="right")
-            rays["freq"] = self.eng[np.minimum(ib, len(self.eng) - 1)]
+            u3 = uniform01(r3)
+            freq = np.empty(nr)
+            for k in np.unique(self.spec[isrc]):
+                eng, scdf = self.spectra[k]
+                m = self.spec[isrc] == k
+                ib = np.searchsorted(scdf, u3[m], side="right")
+                freq[m] = eng[np.minimum(ib, len(eng) - 1)]
+            rays["freq"] = freq
             fld = self.field[isrc]
             rays["length"] = self.boxsizes[fld] * 2
             rays["type"] = -1
