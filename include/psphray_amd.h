@@ -171,6 +171,74 @@ double psr_tab_ar_get(const psr_tables *t, int col, double logT);
 double psr_tab_xs_get(const psr_tables *t, int id, double eng);
 double psr_tab_sph_depth(const psr_tables *t, double r_h);
 
+/* ---- snapshots (src/reader-psphray.inc, the psphray format) ----
+ * header: the format's 256-byte header record (struct header_t of
+ * src/reader-psphray.inc:1-27, its first 256 bytes in this field order);
+ * N[0] gas, N[4] stars, N[5] sources; flag_Helium etc. switch blocks on */
+typedef struct psr_snap_header {
+    uint32_t N[6];
+    double mass[6];
+    double time, redshift;
+    int32_t flag_sfr, flag_feedback;
+    uint32_t Nparticle_total_low[6];
+    int32_t flag_cool, Nfiles;
+    double boxsize, OmegaM, OmegaL, h;
+    int32_t flag_sft, flag_met;
+    uint32_t Nparticle_total_high[6];
+    int32_t flag_entropy;
+    double OmegaB;
+    uint64_t nrays;
+    int32_t flag_Hmf, flag_Hemf, flag_Helium, flag_gammaHI, flag_cloudy, flag_eos;
+    int32_t unused[4];
+} psr_snap_header;
+/* a block's length (items), item size and file offset under a header;
+ * block names as in the format: "pos", "id", "mass", "ie", "rho", "ye",
+ * "xHI", "sml", "Hmf", "Hemf", "xHeI", "xHeII", "gammaHI", "HIa_cloudy",
+ * "eos", "lasthit", "hits", "sft", "ngammas", "spec" */
+int psr_snapshot_block_info(const psr_snap_header *h, const char *block, int64_t *length,
+                            int64_t *itemsize, int64_t *offset);
+int psr_snapshot_read_header(const char *path, psr_snap_header *h, char *err, size_t errlen);
+/* one block into buf (length * itemsize bytes; nothing when the length is 0) */
+int psr_snapshot_read_block(const char *path, const psr_snap_header *h, const char *block, void *buf,
+                            char *err, size_t errlen);
+/* header and every block the header calls for; names[k] / bufs[k] give the
+ * blocks' data (a block not given is written as zeros) */
+int psr_snapshot_write(const char *path, const psr_snap_header *h, const char *const *names,
+                       const void *const *bufs, int32_t nblocks, char *err, size_t errlen);
+
+/* ---- epochs (psystem_switch_epoch, src/psystem.c:221-298) ----
+ * the next epoch's gas particles, as read from its snapshot */
+typedef struct psr_epoch_snap {
+    int64_t npar;
+    const uint64_t *id;
+    const float *pos; /* [npar][3] */
+    const float *mass, *sml, *rho, *ie, *xHI;
+    const float *ye; /* read by psr_epoch_load only */
+} psr_epoch_snap;
+typedef struct psr_epoch_stats {
+    int64_t matched, added, lost, removed;
+    double mean_shift;
+} psr_epoch_stats;
+/* the first epoch (psystem_read_epoch, src/psystem.c:300-379): s takes the
+ * snapshot's gas (npar <= npar_max; the reference sizes npar_max = 1.05 ngas)
+ * with x_HI and ye giving the ionization state (x_HeII from the electrons
+ * beyond hydrogen's, the rest kept as yeMET); accumulators, lasthit, hits and
+ * flags start at zero */
+int psr_epoch_load(psr_state *s, uint64_t *id, int64_t npar_max, const psr_epoch_snap *snap,
+                   const psr_units *u, int32_t h_only, char *err, size_t errlen);
+/* psystem_match_epoch + compress on the host state s (arrays of capacity
+ * npar_max, s->npar in use, ids id[]): each snapshot particle takes the
+ * nearest current particle (within 500 internal lengths) in its ID bucket
+ * (id & (2^28 - 1)), which keeps its ionization state; an unmatched one is
+ * appended with the snapshot's x_HI; current particles not matched are
+ * removed, the last valid ones filling their places (src/psystem.c:381-504,
+ * :109-150); lasthit is rescaled to the new ticks (:266-272).  Then
+ * psr_tree_build and psr_state_upload start the epoch on the device. */
+int psr_epoch_switch(psr_state *s, uint64_t *id, int64_t npar_max, const psr_epoch_snap *snap,
+                     const psr_units *u, int32_t h_only, int32_t periodic, double boxsize,
+                     double prev_duration, int64_t prev_nticks, double tick_time,
+                     psr_epoch_stats *stats, char *err, size_t errlen);
+
 /* ---- physics set-up ---- */
 int psr_set_physics(psr_ctx *ctx, const psr_units *u, const psr_flags *f,
                     const psr_tables *t);

@@ -88,6 +88,27 @@ STEP_DTYPE = np.dtype([(n, "<f8") for n in
 assert STEP_DTYPE.itemsize == 144
 
 
+class SnapHeader(C.Structure):
+    """psr_snap_header: the psphray snapshot header record (256 bytes)"""
+    _fields_ = [("N", C.c_uint32 * 6), ("mass", C.c_double * 6), ("time", C.c_double),
+                ("redshift", C.c_double), ("flag_sfr", C.c_int32), ("flag_feedback", C.c_int32),
+                ("Nparticle_total_low", C.c_uint32 * 6), ("flag_cool", C.c_int32), ("Nfiles", C.c_int32),
+                ("boxsize", C.c_double), ("OmegaM", C.c_double), ("OmegaL", C.c_double), ("h", C.c_double),
+                ("flag_sft", C.c_int32), ("flag_met", C.c_int32), ("Nparticle_total_high", C.c_uint32 * 6),
+                ("flag_entropy", C.c_int32), ("OmegaB", C.c_double), ("nrays", C.c_uint64),
+                ("flag_Hmf", C.c_int32), ("flag_Hemf", C.c_int32), ("flag_Helium", C.c_int32),
+                ("flag_gammaHI", C.c_int32), ("flag_cloudy", C.c_int32), ("flag_eos", C.c_int32),
+                ("unused", C.c_int32 * 4)]
+
+
+class EpochSnap(C.Structure):
+    _fields_ = [("npar", C.c_int64)] + [(n, C.c_void_p) for n in "id pos mass sml rho ie xHI ye".split()]
+
+
+class EpochStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in "matched added lost removed".split()] + [("mean_shift", C.c_double)]
+
+
 class Stats(C.Structure):
     _fields_ = [(n, _I64) for n in
                 "ticks total_deposit_count saturated_deposit_count active_count solver_steps "
@@ -135,6 +156,18 @@ SIGNATURES = {
     "psr_emit_rays": (C.c_int, [_VP, _I64, _D, _I64, _I64, C.c_uint32, _I64, C.POINTER(C.c_int64)]),
     "psr_sources_lastemit": (C.c_int, [_VP, _VP]),
     "psr_rays_download": (C.c_int, [_VP, _I64, _I64, _VP]),
+    "psr_snapshot_block_info": (C.c_int, [C.POINTER(SnapHeader), C.c_char_p, C.POINTER(C.c_int64),
+                                          C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "psr_snapshot_read_header": (C.c_int, [C.c_char_p, C.POINTER(SnapHeader), C.c_char_p, C.c_size_t]),
+    "psr_snapshot_read_block": (C.c_int, [C.c_char_p, C.POINTER(SnapHeader), C.c_char_p, _VP, C.c_char_p,
+                                          C.c_size_t]),
+    "psr_snapshot_write": (C.c_int, [C.c_char_p, C.POINTER(SnapHeader), C.POINTER(C.c_char_p),
+                                     C.POINTER(C.c_void_p), C.c_int32, C.c_char_p, C.c_size_t]),
+    "psr_epoch_load": (C.c_int, [C.POINTER(State), _VP, _I64, C.POINTER(EpochSnap), C.POINTER(Units),
+                                 C.c_int32, C.c_char_p, C.c_size_t]),
+    "psr_epoch_switch": (C.c_int, [C.POINTER(State), _VP, _I64, C.POINTER(EpochSnap), C.POINTER(Units),
+                                   C.c_int32, C.c_int32, _D, _D, _I64, _D, C.POINTER(EpochStats),
+                                   C.c_char_p, C.c_size_t]),
     "psr_mlp_features": (C.c_int, [_VP, C.POINTER(Units), _VP]),
     "psr_mlp_train": (C.c_int, [_VP, _VP, _VP, _I64, _I32, _I32, C.c_float, C.c_uint64, _VP]),
     "psr_mlp_forward": (C.c_int, [_VP, _VP, _I64, _VP]),

@@ -447,3 +447,353 @@ double psr_tab_sph_depth(const psr_tables *t, double r_h) {
     const int n = (int)(t->kernel_n * r_h);
     return (n < t->kernel_n && n >= 0) ? t->kernel[n] : 0.0;
 }
+
+/* ------------------------------------------------------------ snapshots */
+/* The psphray snapshot format (src/reader-psphray.inc): a 256-byte header
+ * then the blocks in a fixed order, each as <int32 size> data <int32 size>,
+ * present when its length is non-zero.  Lengths count gas (type 0), star
+ * (type 4) and source (type 5) particles as the format's tables give them
+ * (src/reader-psphray.inc:115-330); item sizes :68-90. */
+static const char *const SNAP_BLOCKS[] = {"header", "pos", "id", "mass", "ie", "rho", "ye", "xHI", "sml",
+                                          "Hmf", "Hemf", "xHeI", "xHeII", "gammaHI", "HIa_cloudy", "eos",
+                                          "lasthit", "hits", "sft", "ngammas", "spec"};
+static const int64_t SNAP_ITEM[] = {256, 12, 8, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 8, 4, 4, 8, 8};
+#define SNAP_NBLOCK ((int)(sizeof(SNAP_ITEM) / sizeof(SNAP_ITEM[0])))
+
+static int snap_block_id(const char *b) {
+    for (int i = 0; i < SNAP_NBLOCK; i++)
+        if (!strcmp(b, SNAP_BLOCKS[i])) return i;
+    return -1;
+}
+
+static int64_t snap_length(const psr_snap_header *h, int id) {
+    const int64_t g = h->N[0], st = h->N[4], src = h->N[5];
+    switch (id) {
+    case 0: return 1;
+    case 1: case 2: case 3: return g + st + src;
+    case 4: case 5: case 6: case 7: case 8: case 16: case 17: return g;
+    case 9: return h->flag_Hmf ? g : 0;
+    case 10: return h->flag_Hemf ? g : 0;
+    case 11: case 12: return h->flag_Helium ? g : 0;
+    case 13: return h->flag_gammaHI ? g : 0;
+    case 14: return h->flag_cloudy ? g : 0;
+    case 15: return h->flag_eos ? g : 0;
+    case 18: return st;
+    case 19: case 20: return src;
+    default: return 0;
+    }
+}
+
+int psr_snapshot_block_info(const psr_snap_header *h, const char *block, int64_t *length, int64_t *itemsize,
+                            int64_t *offset) {
+    if (!h || !block) return PSR_ERR_ARG;
+    const int id = snap_block_id(block);
+    if (id < 0) return PSR_ERR_ARG;
+    int64_t off = 0;
+    for (int i = 0; i < id; i++) {
+        const int64_t bytes = snap_length(h, i) * SNAP_ITEM[i];
+        if (bytes > 0) off += bytes + 8;
+    }
+    if (length) *length = snap_length(h, id);
+    if (itemsize) *itemsize = SNAP_ITEM[id];
+    if (offset) *offset = off;
+    return PSR_OK;
+}
+
+static int snap_err(char *err, size_t errlen, const char *msg, const char *what) {
+    if (err && errlen) snprintf(err, errlen, "%s: %s", what, msg);
+    return PSR_ERR_ARG;
+}
+
+int psr_snapshot_read_header(const char *path, psr_snap_header *h, char *err, size_t errlen) {
+    if (!path || !h) return PSR_ERR_ARG;
+    FILE *f = fopen(path, "rb");
+    if (!f) return snap_err(err, errlen, "cannot open", path);
+    int32_t m0 = 0, m1 = 0;
+    memset(h, 0, sizeof(*h));
+    const int ok = fread(&m0, 4, 1, f) == 1 && fread(h, 256, 1, f) == 1 && fread(&m1, 4, 1, f) == 1;
+    fclose(f);
+    if (!ok || m0 != 256 || m1 != 256) return snap_err(err, errlen, "not a psphray snapshot (header record)", path);
+    return PSR_OK;
+}
+
+int psr_snapshot_read_block(const char *path, const psr_snap_header *h, const char *block, void *buf, char *err,
+                            size_t errlen) {
+    int64_t len, item, off;
+    if (!path || !h || !block || psr_snapshot_block_info(h, block, &len, &item, &off))
+        return snap_err(err, errlen, "unknown block", block ? block : "(null)");
+    if (len == 0) return PSR_OK;
+    if (!buf) return PSR_ERR_ARG;
+    FILE *f = fopen(path, "rb");
+    if (!f) return snap_err(err, errlen, "cannot open", path);
+    const int64_t bytes = len * item;
+    int32_t m0 = 0, m1 = 0;
+    int ok = fseeko(f, off, SEEK_SET) == 0 && fread(&m0, 4, 1, f) == 1 && m0 == (int32_t)bytes &&
+             fread(buf, 1, (size_t)bytes, f) == (size_t)bytes && fread(&m1, 4, 1, f) == 1 && m1 == (int32_t)bytes;
+    fclose(f);
+    if (!ok) return snap_err(err, errlen, "block size markers do not match the header", block);
+    return PSR_OK;
+}
+
+int psr_snapshot_write(const char *path, const psr_snap_header *h, const char *const *names, const void *const *bufs,
+                       int32_t nblocks, char *err, size_t errlen) {
+    if (!path || !h || (nblocks > 0 && (!names || !bufs))) return PSR_ERR_ARG;
+    for (int32_t k = 0; k < nblocks; k++)
+        if (snap_block_id(names[k]) <= 0) return snap_err(err, errlen, "unknown block", names[k]);
+    FILE *f = fopen(path, "wb");
+    if (!f) return snap_err(err, errlen, "cannot create", path);
+    int ok = 1;
+    for (int id = 0; id < SNAP_NBLOCK && ok; id++) {
+        const int64_t bytes = snap_length(h, id) * SNAP_ITEM[id];
+        if (bytes == 0) continue;
+        const void *src = id == 0 ? (const void *)h : NULL;
+        for (int32_t k = 0; k < nblocks && id > 0; k++)
+            if (!strcmp(names[k], SNAP_BLOCKS[id])) src = bufs[k];
+        const int32_t m = (int32_t)bytes;
+        ok = fwrite(&m, 4, 1, f) == 1;
+        if (src) {
+            ok = ok && fwrite(src, 1, (size_t)bytes, f) == (size_t)bytes;
+        } else {
+            /* a block the header calls for but the caller did not give */
+            char zero[4096] = {0};
+            for (int64_t left = bytes; left > 0 && ok; left -= (int64_t)sizeof zero)
+                ok = fwrite(zero, 1, (size_t)(left < (int64_t)sizeof zero ? left : (int64_t)sizeof zero), f) > 0;
+        }
+        ok = ok && fwrite(&m, 4, 1, f) == 1;
+    }
+    if (fclose(f) != 0) ok = 0;
+    if (!ok) return snap_err(err, errlen, "write failed", path);
+    return PSR_OK;
+}
+
+/* ------------------------------------------------------------ epochs */
+/* distance of two positions as src/psystem.c:507-518 computes it: float
+ * differences (wrapped when periodic), squares added in double, sqrt */
+static float epoch_dist(const float *a, const float *b, int periodic, double boxsize) {
+    double r = 0.0;
+    for (int d = 0; d < 3; d++) {
+        float dd = fabsf(a[d] - b[d]);
+        if (periodic && dd > 0.5 * boxsize) dd = (float)(boxsize - dd);
+        r += dd * dd;
+    }
+    return (float)sqrt(r);
+}
+
+typedef struct {
+    uint64_t hash;
+    int64_t idx;
+} epoch_key;
+
+/* hash ascending, index descending: the order the reference's hash chains
+ * visit a bucket (the last inserted first, src/psystem.c:49-58) */
+static int epoch_key_cmp(const void *pa, const void *pb) {
+    const epoch_key *a = pa, *b = pb;
+    if (a->hash != b->hash) return a->hash < b->hash ? -1 : 1;
+    return (a->idx < b->idx) - (a->idx > b->idx);
+}
+
+/* a fraction x with complement rest, pinned to 0 or 1 when either is out
+ * of range (psys_set_lambdaH / psys_set_lambdaHe, src/psystem.h:156-169) */
+static double clamp_fraction(double x, double rest) {
+    if (x >= 1.0 || rest <= 0.0) return 1.0;
+    if (x <= 0.0 || rest >= 1.0) return 0.0;
+    return x;
+}
+
+#define PSR_PF_INVALID 1
+#define PSR_PF_NORMAL 0
+#define PSR_IDHASH_MASK ((((uint64_t)1) << 28) - 1)
+
+static void state_swap(psr_state *s, uint64_t *id, int64_t i, int64_t j) {
+#define SW(arr, T)                 \
+    do {                           \
+        const T t_ = (arr)[i];     \
+        (arr)[i] = (arr)[j];       \
+        (arr)[j] = t_;             \
+    } while (0)
+    for (int d = 0; d < 3; d++) {
+        const float t = s->pos[3 * i + d];
+        s->pos[3 * i + d] = s->pos[3 * j + d];
+        s->pos[3 * j + d] = t;
+    }
+    SW(s->lambdaH, double);
+    SW(s->lambdaHeI, double);
+    SW(s->lambdaHeII, double);
+    SW(s->yeMET, float);
+    SW(s->mass, float);
+    SW(s->sml, float);
+    SW(s->rho, float);
+    SW(s->ie, float);
+    SW(s->flag, int8_t);
+    SW(s->yGdepHI, float);
+    SW(s->yGdepHeI, float);
+    SW(s->yGdepHeII, float);
+    SW(s->yGrecHII, float);
+    SW(s->yGrecHeII, float);
+    SW(s->yGrecHeIII, float);
+    SW(s->heat, float);
+    SW(s->lasthit, int64_t);
+    SW(s->hits, int32_t);
+    SW(id, uint64_t);
+#undef SW
+}
+
+int psr_epoch_load(psr_state *s, uint64_t *id, int64_t npar_max, const psr_epoch_snap *n, const psr_units *u,
+                   int32_t h_only, char *err, size_t errlen) {
+    if (!s || !id || !n || !u || n->npar < 0 || !n->ye || !n->xHI)
+        return PSR_ERR_ARG;
+    if (n->npar > npar_max) {
+        if (err && errlen) snprintf(err, errlen, "%lld particles beyond npar_max %lld", (long long)n->npar,
+                                    (long long)npar_max);
+        return PSR_ERR_CAPACITY;
+    }
+    for (int64_t i = 0; i < n->npar; i++) {
+        for (int d = 0; d < 3; d++) s->pos[3 * i + d] = n->pos[3 * i + d];
+        s->mass[i] = n->mass[i];
+        s->sml[i] = n->sml[i];
+        s->rho[i] = n->rho[i];
+        s->ie[i] = n->ie[i];
+        id[i] = n->id[i];
+        const double xHII = 1.0 - n->xHI[i];
+        s->lambdaH[i] = clamp_fraction(n->xHI[i], xHII);
+        /* electrons beyond hydrogen's go to helium first (src/psystem.c:349-358) */
+        double yeMET = fdim(n->ye[i], xHII);
+        if (!h_only) {
+            double xHeII = yeMET * u->C_HMF / (1 - u->C_HMF) * 4;
+            double xHeIII = fdim(xHeII, 1.0);
+            xHeII = fmin(1.0, xHeII);
+            xHeIII = fmin(1.0 - xHeII, xHeIII);
+            const double xHeI = 1.0 - xHeII - xHeIII;
+            s->lambdaHeI[i] = clamp_fraction(xHeI, xHeII + xHeIII);
+            s->lambdaHeII[i] = clamp_fraction(xHeII, xHeI + xHeIII);
+            yeMET = fdim(yeMET, (xHeII + 2 * xHeIII) * (1 - u->C_HMF) / u->C_HMF * 0.25);
+        } else {
+            s->lambdaHeI[i] = s->lambdaHeII[i] = 0.0;
+        }
+        s->yeMET[i] = (float)yeMET;
+        s->yGdepHI[i] = s->yGdepHeI[i] = s->yGdepHeII[i] = 0.0f;
+        s->yGrecHII[i] = s->yGrecHeII[i] = s->yGrecHeIII[i] = 0.0f;
+        s->heat[i] = 0.0f;
+        s->lasthit[i] = 0;
+        s->hits[i] = 0;
+        s->flag[i] = PSR_PF_NORMAL;
+    }
+    s->npar = n->npar;
+    return PSR_OK;
+}
+
+int psr_epoch_switch(psr_state *s, uint64_t *id, int64_t npar_max, const psr_epoch_snap *n, const psr_units *u,
+                     int32_t h_only, int32_t periodic, double boxsize, double prev_duration, int64_t prev_nticks,
+                     double tick_time, psr_epoch_stats *out, char *err, size_t errlen) {
+    if (!s || !id || !n || !u || s->npar < 0 || s->npar > npar_max || n->npar < 0 || prev_nticks <= 0 ||
+        !(tick_time > 0))
+        return PSR_ERR_ARG;
+    const int64_t nold = s->npar;
+    /* the current particles by ID bucket (src/psystem.c:381-382) */
+    epoch_key *keys = malloc(sizeof(epoch_key) * (size_t)(nold > 0 ? nold : 1));
+    for (int64_t i = 0; i < nold; i++) {
+        keys[i].hash = id[i] & PSR_IDHASH_MASK;
+        keys[i].idx = i;
+    }
+    qsort(keys, (size_t)nold, sizeof(epoch_key), epoch_key_cmp);
+    for (int64_t i = 0; i < nold; i++) s->flag[i] |= PSR_PF_INVALID;
+    int64_t npar = nold, added = 0, lost = 0;
+    double distsum = 0.0;
+    for (int64_t ip = 0; ip < n->npar; ip++) {
+        const uint64_t hash = n->id[ip] & PSR_IDHASH_MASK;
+        /* the bucket: keys with this hash */
+        int64_t lo = 0, hi = nold;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) / 2;
+            if (keys[mid].hash < hash) lo = mid + 1;
+            else hi = mid;
+        }
+        int64_t best = -1;
+        float mindist = 500.0f;
+        for (int64_t k = lo; k < nold && keys[k].hash == hash; k++) {
+            const float dst = epoch_dist(&n->pos[3 * ip], &s->pos[3 * keys[k].idx], periodic, boxsize);
+            if (dst < mindist) {
+                mindist = dst;
+                best = keys[k].idx;
+            }
+        }
+        int insert = 0;
+        if (best == -1) {
+            if (npar == npar_max) {
+                lost++;
+                continue;
+            }
+            best = npar++;
+            insert = 1;
+            added++;
+        } else {
+            distsum += mindist;
+        }
+        for (int d = 0; d < 3; d++) s->pos[3 * best + d] = n->pos[3 * ip + d];
+        s->mass[best] = n->mass[ip];
+        s->sml[best] = n->sml[ip];
+        s->rho[best] = n->rho[ip];
+        s->ie[best] = n->ie[ip];
+        s->flag[best] &= ~PSR_PF_INVALID;
+        if (insert) {
+            /* a new particle takes the snapshot's ionization state
+             * (src/psystem.c:455-485) */
+            id[best] = n->id[ip];
+            const double xHI = n->xHI[ip], xHII = 1.0 - n->xHI[ip];
+            s->lambdaH[best] = clamp_fraction(xHI, xHII);
+            double yeMET = fdim(s->yeMET[best], xHII);
+            if (!h_only) {
+                double xHeII = yeMET * u->C_HMF / (1 - u->C_HMF) * 4;
+                double xHeIII = fdim(xHeII, 1.0);
+                xHeIII = fmin(1.0, xHeII);
+                xHeII = fmin(1.0, xHeII);
+                const double xHeI = 1.0 - xHeII - xHeIII;
+                s->lambdaHeI[best] = clamp_fraction(xHeI, xHeII + xHeIII);
+                s->lambdaHeII[best] = clamp_fraction(xHeII, xHeI + xHeIII);
+                yeMET = fdim(yeMET, (xHeII + xHeIII) * (1 - u->C_HMF) / u->C_HMF * 0.25);
+            }
+            s->yeMET[best] = (float)yeMET;
+            s->yGdepHI[best] = s->yGdepHeI[best] = s->yGdepHeII[best] = 0.0f;
+            s->yGrecHII[best] = s->yGrecHeII[best] = s->yGrecHeIII[best] = 0.0f;
+            /* the reference clears heat and hits of particle 3, its loop
+             * counter, not of the new one (src/psystem.c:477-478) */
+            if (npar_max > 3) {
+                s->heat[3] = 0.0f;
+                s->hits[3] = 0;
+            }
+            s->lasthit[best] = 0;
+            s->flag[best] = PSR_PF_NORMAL;
+        }
+    }
+    free(keys);
+    /* lasthit in ticks of the new epoch (src/psystem.c:266-272) */
+    const double last_ticktime = prev_duration / (double)prev_nticks;
+    for (int64_t i = 0; i < npar; i++)
+        s->lasthit[i] = (int64_t)(((double)s->lasthit[i] * last_ticktime - prev_duration) / tick_time);
+    /* compress (src/psystem.c:109-150): the last valid particles fill the
+     * holes from the front */
+    int64_t invalid = 0;
+    for (int64_t i = 0; i < npar; i++) invalid += (s->flag[i] & PSR_PF_INVALID) != 0;
+    int64_t j = npar - 1;
+    while (j > 0 && (s->flag[j] & PSR_PF_INVALID)) j--;
+    for (int64_t i = 0; i < j; i++) {
+        if (s->flag[i] & PSR_PF_INVALID) {
+            state_swap(s, id, i, j);
+            while ((s->flag[j] & PSR_PF_INVALID) && j > i) j--;
+        }
+    }
+    if (j == 0 && npar > 0 && (s->flag[0] & PSR_PF_INVALID)) j = -1;  /* every particle invalid */
+    s->npar = npar > 0 ? j + 1 : 0;
+    if (out) {
+        out->matched = n->npar - added - lost;
+        out->added = added;
+        out->lost = lost;
+        out->removed = invalid;
+        out->mean_shift = n->npar > 0 ? distsum / (double)n->npar : 0.0;
+    }
+    if (lost) {
+        if (err && errlen) snprintf(err, errlen, "%lld unmatched particles beyond npar_max", (long long)lost);
+        return PSR_ERR_CAPACITY;
+    }
+    return PSR_OK;
+}
