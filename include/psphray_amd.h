@@ -171,10 +171,15 @@ double psr_tab_ar_get(const psr_tables *t, int col, double logT);
 double psr_tab_xs_get(const psr_tables *t, int id, double eng);
 double psr_tab_sph_depth(const psr_tables *t, double r_h);
 
-/* ---- snapshots (src/reader-psphray.inc, the psphray format) ----
- * header: the format's 256-byte header record (struct header_t of
- * src/reader-psphray.inc:1-27, its first 256 bytes in this field order);
- * N[0] gas, N[4] stars, N[5] sources; flag_Helium etc. switch blocks on */
+/* ---- snapshots (reader_new, src/reader.c:36-46) ----
+ * format: "psphray" (src/reader-psphray.inc; NULL means it) or
+ * "massiveblack" / "e5" (src/reader-massiveblack.inc).  header: the
+ * 256-byte header record in the psphray field order (struct header_t,
+ * src/reader-psphray.inc:1-28, its first 256 bytes); massiveblack's
+ * header_t (src/reader-massiveblack.inc:1-23) shares every field up to
+ * flag_entropy, which is all its block lengths read.  N[0] gas, N[1] dark
+ * matter (massiveblack), N[4] stars, N[5] sources; the flags switch blocks
+ * on. */
 typedef struct psr_snap_header {
     uint32_t N[6];
     double mass[6];
@@ -191,20 +196,25 @@ typedef struct psr_snap_header {
     int32_t flag_Hmf, flag_Hemf, flag_Helium, flag_gammaHI, flag_cloudy, flag_eos;
     int32_t unused[4];
 } psr_snap_header;
-/* a block's length (items), item size and file offset under a header;
- * block names as in the format: "pos", "id", "mass", "ie", "rho", "ye",
- * "xHI", "sml", "Hmf", "Hemf", "xHeI", "xHeII", "gammaHI", "HIa_cloudy",
- * "eos", "lasthit", "hits", "sft", "ngammas", "spec" */
-int psr_snapshot_block_info(const psr_snap_header *h, const char *block, int64_t *length,
-                            int64_t *itemsize, int64_t *offset);
+/* a block's length (items), item size and file offset under a header
+ * (_length / _itemsize / _offset of each format); psphray blocks: "pos",
+ * "id", "mass", "ie", "rho", "ye", "xHI", "sml", "Hmf", "Hemf", "xHeI",
+ * "xHeII", "gammaHI", "HIa_cloudy", "eos", "lasthit", "hits", "sft",
+ * "ngammas", "spec"; massiveblack: "pos", "vel", "id", "mass", "ie", "rho",
+ * "ye", "xHI", "sml", "sfr", "sft", "met", "bhmass", "bhmdot" */
+int psr_snapshot_block_info(const char *format, const psr_snap_header *h, const char *block,
+                            int64_t *length, int64_t *itemsize, int64_t *offset);
+/* the header record (both formats: 256 bytes framed by 256) */
 int psr_snapshot_read_header(const char *path, psr_snap_header *h, char *err, size_t errlen);
-/* one block into buf (length * itemsize bytes; nothing when the length is 0) */
-int psr_snapshot_read_block(const char *path, const psr_snap_header *h, const char *block, void *buf,
-                            char *err, size_t errlen);
+/* one block into buf (length * itemsize bytes; nothing when the length is
+ * 0); the record's size markers are checked (_read) */
+int psr_snapshot_read_block(const char *format, const char *path, const psr_snap_header *h,
+                            const char *block, void *buf, char *err, size_t errlen);
 /* header and every block the header calls for; names[k] / bufs[k] give the
  * blocks' data (a block not given is written as zeros) */
-int psr_snapshot_write(const char *path, const psr_snap_header *h, const char *const *names,
-                       const void *const *bufs, int32_t nblocks, char *err, size_t errlen);
+int psr_snapshot_write(const char *format, const char *path, const psr_snap_header *h,
+                       const char *const *names, const void *const *bufs, int32_t nblocks, char *err,
+                       size_t errlen);
 
 /* ---- epochs (psystem_switch_epoch, src/psystem.c:221-298) ----
  * the next epoch's gas particles, as read from its snapshot */

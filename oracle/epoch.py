@@ -35,33 +35,37 @@ HEADER_DTYPE = np.dtype({
     "itemsize": 256,
 })
 
-# (name, item bytes, numpy type, components, length rule)
+# (name, item bytes, numpy type, components, particle types counted, flag)
 BLOCKS = [
-    ("pos", 12, "<f4", 3, "all"), ("id", 8, "<u8", 1, "all"), ("mass", 4, "<f4", 1, "all"),
-    ("ie", 4, "<f4", 1, "gas"), ("rho", 4, "<f4", 1, "gas"), ("ye", 4, "<f4", 1, "gas"),
-    ("xHI", 4, "<f4", 1, "gas"), ("sml", 4, "<f4", 1, "gas"), ("Hmf", 4, "<f4", 1, "flag_Hmf"),
-    ("Hemf", 4, "<f4", 1, "flag_Hemf"), ("xHeI", 4, "<f4", 1, "flag_Helium"),
-    ("xHeII", 4, "<f4", 1, "flag_Helium"), ("gammaHI", 4, "<f4", 1, "flag_gammaHI"),
-    ("HIa_cloudy", 4, "<f4", 1, "flag_cloudy"), ("eos", 4, "<f4", 1, "flag_eos"),
-    ("lasthit", 8, "<i8", 1, "gas"), ("hits", 4, "<i4", 1, "gas"), ("sft", 4, "<f4", 1, "star"),
-    ("ngammas", 8, "<f8", 1, "src"), ("spec", 8, "S8", 1, "src"),
+    ("pos", 12, "<f4", 3, (0, 4, 5), None), ("id", 8, "<u8", 1, (0, 4, 5), None),
+    ("mass", 4, "<f4", 1, (0, 4, 5), None), ("ie", 4, "<f4", 1, (0,), None), ("rho", 4, "<f4", 1, (0,), None),
+    ("ye", 4, "<f4", 1, (0,), None), ("xHI", 4, "<f4", 1, (0,), None), ("sml", 4, "<f4", 1, (0,), None),
+    ("Hmf", 4, "<f4", 1, (0,), "flag_Hmf"), ("Hemf", 4, "<f4", 1, (0,), "flag_Hemf"),
+    ("xHeI", 4, "<f4", 1, (0,), "flag_Helium"), ("xHeII", 4, "<f4", 1, (0,), "flag_Helium"),
+    ("gammaHI", 4, "<f4", 1, (0,), "flag_gammaHI"), ("HIa_cloudy", 4, "<f4", 1, (0,), "flag_cloudy"),
+    ("eos", 4, "<f4", 1, (0,), "flag_eos"), ("lasthit", 8, "<i8", 1, (0,), None),
+    ("hits", 4, "<i4", 1, (0,), None), ("sft", 4, "<f4", 1, (4,), None), ("ngammas", 8, "<f8", 1, (5,), None),
+    ("spec", 8, "S8", 1, (5,), None),
 ]
+# src/reader-massiveblack.inc:26-330
+BLOCKS_MASSIVEBLACK = [
+    ("pos", 12, "<f4", 3, (0, 1, 4, 5), None), ("vel", 12, "<f4", 3, (0, 1, 4, 5), None),
+    ("id", 8, "<u8", 1, (0, 1, 4, 5), None), ("mass", 4, "<f4", 1, (0, 4, 5), None),
+    ("ie", 4, "<f4", 1, (0,), None), ("rho", 4, "<f4", 1, (0,), None), ("ye", 4, "<f4", 1, (0,), None),
+    ("xHI", 4, "<f4", 1, (0,), None), ("sml", 4, "<f4", 1, (0,), None), ("sfr", 4, "<f4", 1, (0,), "flag_sfr"),
+    ("sft", 4, "<f4", 1, (4,), "flag_sft"), ("met", 4, "<f4", 1, (0, 4), "flag_met"),
+    ("bhmass", 4, "<f4", 1, (5,), None), ("bhmdot", 4, "<f4", 1, (5,), None),
+]
+FORMATS = {"psphray": BLOCKS, "massiveblack": BLOCKS_MASSIVEBLACK}
 
 
-def block_length(hdr, rule):
-    n = hdr["N"]
-    if rule == "all":
-        return int(n[0]) + int(n[4]) + int(n[5])
-    if rule == "gas":
-        return int(n[0])
-    if rule == "star":
-        return int(n[4])
-    if rule == "src":
-        return int(n[5])
-    return int(n[0]) if hdr[rule] else 0
+def block_length(hdr, types, flag):
+    if flag is not None and not hdr[flag]:
+        return 0
+    return sum(int(hdr["N"][t]) for t in types)
 
 
-def write_snapshot(path, hdr, blocks):
+def write_snapshot(path, hdr, blocks, fmt="psphray"):
     """header record, then each block the header calls for, in order,
     framed by its byte count; absent blocks are zeros"""
     with open(path, "wb") as f:
@@ -70,8 +74,8 @@ def write_snapshot(path, hdr, blocks):
         rec[0] = hdr
         rec.tofile(f)
         np.array([256], "<i4").tofile(f)
-        for name, item, dt, k, rule in BLOCKS:
-            ln = block_length(hdr, rule)
+        for name, item, dt, k, types, flag in FORMATS[fmt]:
+            ln = block_length(hdr, types, flag)
             if ln == 0:
                 continue
             a = np.zeros(ln * k, dt) if name not in blocks else np.asarray(blocks[name], dt).reshape(-1)
@@ -81,13 +85,13 @@ def write_snapshot(path, hdr, blocks):
             np.array([ln * item], "<i4").tofile(f)
 
 
-def read_snapshot(path):
+def read_snapshot(path, fmt="psphray"):
     raw = np.fromfile(path, np.uint8)
     assert raw[:4].view("<i4")[0] == 256 and raw[260:264].view("<i4")[0] == 256
     hdr = raw[4:260].view(HEADER_DTYPE)[0]
     out, off = {}, 264
-    for name, item, dt, k, rule in BLOCKS:
-        ln = block_length(hdr, rule)
+    for name, item, dt, k, types, flag in FORMATS[fmt]:
+        ln = block_length(hdr, types, flag)
         if ln == 0:
             continue
         nb = ln * item

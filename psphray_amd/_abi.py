@@ -156,12 +156,12 @@ SIGNATURES = {
     "psr_emit_rays": (C.c_int, [_VP, _I64, _D, _I64, _I64, C.c_uint32, _I64, C.POINTER(C.c_int64)]),
     "psr_sources_lastemit": (C.c_int, [_VP, _VP]),
     "psr_rays_download": (C.c_int, [_VP, _I64, _I64, _VP]),
-    "psr_snapshot_block_info": (C.c_int, [C.POINTER(SnapHeader), C.c_char_p, C.POINTER(C.c_int64),
+    "psr_snapshot_block_info": (C.c_int, [C.c_char_p, C.POINTER(SnapHeader), C.c_char_p, C.POINTER(C.c_int64),
                                           C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "psr_snapshot_read_header": (C.c_int, [C.c_char_p, C.POINTER(SnapHeader), C.c_char_p, C.c_size_t]),
-    "psr_snapshot_read_block": (C.c_int, [C.c_char_p, C.POINTER(SnapHeader), C.c_char_p, _VP, C.c_char_p,
-                                          C.c_size_t]),
-    "psr_snapshot_write": (C.c_int, [C.c_char_p, C.POINTER(SnapHeader), C.POINTER(C.c_char_p),
+    "psr_snapshot_read_block": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(SnapHeader), C.c_char_p, _VP,
+                                          C.c_char_p, C.c_size_t]),
+    "psr_snapshot_write": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(SnapHeader), C.POINTER(C.c_char_p),
                                      C.POINTER(C.c_void_p), C.c_int32, C.c_char_p, C.c_size_t]),
     "psr_epoch_load": (C.c_int, [C.POINTER(State), _VP, _I64, C.POINTER(EpochSnap), C.POINTER(Units),
                                  C.c_int32, C.c_char_p, C.c_size_t]),

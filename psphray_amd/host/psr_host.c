@@ -9,6 +9,8 @@
  */
 #define _GNU_SOURCE
 #include <math.h>
+#include <stddef.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -449,53 +451,100 @@ double psr_tab_sph_depth(const psr_tables *t, double r_h) {
 }
 
 /* ------------------------------------------------------------ snapshots */
-/* The psphray snapshot format (src/reader-psphray.inc): a 256-byte header
- * then the blocks in a fixed order, each as <int32 size> data <int32 size>,
- * present when its length is non-zero.  Lengths count gas (type 0), star
- * (type 4) and source (type 5) particles as the format's tables give them
- * (src/reader-psphray.inc:115-330); item sizes :68-90. */
-static const char *const SNAP_BLOCKS[] = {"header", "pos", "id", "mass", "ie", "rho", "ye", "xHI", "sml",
-                                          "Hmf", "Hemf", "xHeI", "xHeII", "gammaHI", "HIa_cloudy", "eos",
-                                          "lasthit", "hits", "sft", "ngammas", "spec"};
-static const int64_t SNAP_ITEM[] = {256, 12, 8, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 8, 4, 4, 8, 8};
-#define SNAP_NBLOCK ((int)(sizeof(SNAP_ITEM) / sizeof(SNAP_ITEM[0])))
+/* Gadget-style snapshots: a 256-byte header, then the format's blocks in a
+ * fixed order, each as <int32 size> data <int32 size>, present when its
+ * length is non-zero (_offset, src/reader-psphray.inc:332-344).  Two
+ * formats, as reader_new names them (src/reader.c:36-46):
+ *   psphray       src/reader-psphray.inc: order :55-66, item sizes :68-90,
+ *                 lengths :115-330 (gas type 0, stars 4, sources 5);
+ *   massiveblack  src/reader-massiveblack.inc ("e5" too): order, item sizes
+ *                 and lengths :26-330 (also type 1, dark matter).
+ * The lengths read N[] and the sfr/sft/met/Hmf/.. flags, all in the
+ * header prefix the two formats share. */
+typedef struct {
+    const char *name;
+    int64_t item;
+    /* length = sum of N[t] over the type mask bits, when the flag (a byte
+     * offset into the header, or -1) is set */
+    unsigned types;
+    int flag_off;
+} snap_block;
 
-static int snap_block_id(const char *b) {
-    for (int i = 0; i < SNAP_NBLOCK; i++)
-        if (!strcmp(b, SNAP_BLOCKS[i])) return i;
+#define T0 1u
+#define T1 2u
+#define T4 16u
+#define T5 32u
+#define HOFF(f) ((int)offsetof(psr_snap_header, f))
+
+static const snap_block SNAP_PSPHRAY[] = {
+    {"header", 256, 0, -1}, {"pos", 12, T0 | T4 | T5, -1}, {"id", 8, T0 | T4 | T5, -1},
+    {"mass", 4, T0 | T4 | T5, -1}, {"ie", 4, T0, -1}, {"rho", 4, T0, -1}, {"ye", 4, T0, -1},
+    {"xHI", 4, T0, -1}, {"sml", 4, T0, -1}, {"Hmf", 4, T0, HOFF(flag_Hmf)},
+    {"Hemf", 4, T0, HOFF(flag_Hemf)}, {"xHeI", 4, T0, HOFF(flag_Helium)},
+    {"xHeII", 4, T0, HOFF(flag_Helium)}, {"gammaHI", 4, T0, HOFF(flag_gammaHI)},
+    {"HIa_cloudy", 4, T0, HOFF(flag_cloudy)}, {"eos", 4, T0, HOFF(flag_eos)}, {"lasthit", 8, T0, -1},
+    {"hits", 4, T0, -1}, {"sft", 4, T4, -1}, {"ngammas", 8, T5, -1}, {"spec", 8, T5, -1},
+};
+static const snap_block SNAP_MASSIVEBLACK[] = {
+    {"header", 256, 0, -1}, {"pos", 12, T0 | T1 | T4 | T5, -1}, {"vel", 12, T0 | T1 | T4 | T5, -1},
+    {"id", 8, T0 | T1 | T4 | T5, -1}, {"mass", 4, T0 | T4 | T5, -1}, {"ie", 4, T0, -1},
+    {"rho", 4, T0, -1}, {"ye", 4, T0, -1}, {"xHI", 4, T0, -1}, {"sml", 4, T0, -1},
+    {"sfr", 4, T0, HOFF(flag_sfr)}, {"sft", 4, T4, HOFF(flag_sft)}, {"met", 4, T0 | T4, HOFF(flag_met)},
+    {"bhmass", 4, T5, -1}, {"bhmdot", 4, T5, -1},
+};
+
+typedef struct {
+    const snap_block *b;
+    int n;
+} snap_format;
+
+static int snap_format_of(const char *format, snap_format *f) {
+    if (!format || !strcmp(format, "psphray")) {
+        f->b = SNAP_PSPHRAY;
+        f->n = (int)(sizeof SNAP_PSPHRAY / sizeof SNAP_PSPHRAY[0]);
+        return 0;
+    }
+    if (!strcmp(format, "massiveblack") || !strcmp(format, "e5")) {
+        f->b = SNAP_MASSIVEBLACK;
+        f->n = (int)(sizeof SNAP_MASSIVEBLACK / sizeof SNAP_MASSIVEBLACK[0]);
+        return 0;
+    }
     return -1;
 }
 
-static int64_t snap_length(const psr_snap_header *h, int id) {
-    const int64_t g = h->N[0], st = h->N[4], src = h->N[5];
-    switch (id) {
-    case 0: return 1;
-    case 1: case 2: case 3: return g + st + src;
-    case 4: case 5: case 6: case 7: case 8: case 16: case 17: return g;
-    case 9: return h->flag_Hmf ? g : 0;
-    case 10: return h->flag_Hemf ? g : 0;
-    case 11: case 12: return h->flag_Helium ? g : 0;
-    case 13: return h->flag_gammaHI ? g : 0;
-    case 14: return h->flag_cloudy ? g : 0;
-    case 15: return h->flag_eos ? g : 0;
-    case 18: return st;
-    case 19: case 20: return src;
-    default: return 0;
-    }
+static int snap_block_id(const snap_format *f, const char *b) {
+    for (int i = 0; i < f->n; i++)
+        if (!strcmp(b, f->b[i].name)) return i;
+    return -1;
 }
 
-int psr_snapshot_block_info(const psr_snap_header *h, const char *block, int64_t *length, int64_t *itemsize,
-                            int64_t *offset) {
-    if (!h || !block) return PSR_ERR_ARG;
-    const int id = snap_block_id(block);
+static int64_t snap_length(const snap_format *f, const psr_snap_header *h, int id) {
+    const snap_block *b = &f->b[id];
+    if (id == 0) return 1;
+    if (b->flag_off >= 0) {
+        int32_t flag;
+        memcpy(&flag, (const char *)h + b->flag_off, sizeof flag);
+        if (!flag) return 0;
+    }
+    int64_t n = 0;
+    for (int t = 0; t < 6; t++)
+        if (b->types & (1u << t)) n += h->N[t];
+    return n;
+}
+
+int psr_snapshot_block_info(const char *format, const psr_snap_header *h, const char *block, int64_t *length,
+                            int64_t *itemsize, int64_t *offset) {
+    snap_format f;
+    if (!h || !block || snap_format_of(format, &f)) return PSR_ERR_ARG;
+    const int id = snap_block_id(&f, block);
     if (id < 0) return PSR_ERR_ARG;
     int64_t off = 0;
     for (int i = 0; i < id; i++) {
-        const int64_t bytes = snap_length(h, i) * SNAP_ITEM[i];
+        const int64_t bytes = snap_length(&f, h, i) * f.b[i].item;
         if (bytes > 0) off += bytes + 8;
     }
-    if (length) *length = snap_length(h, id);
-    if (itemsize) *itemsize = SNAP_ITEM[id];
+    if (length) *length = snap_length(&f, h, id);
+    if (itemsize) *itemsize = f.b[id].item;
     if (offset) *offset = off;
     return PSR_OK;
 }
@@ -517,11 +566,11 @@ int psr_snapshot_read_header(const char *path, psr_snap_header *h, char *err, si
     return PSR_OK;
 }
 
-int psr_snapshot_read_block(const char *path, const psr_snap_header *h, const char *block, void *buf, char *err,
-                            size_t errlen) {
+int psr_snapshot_read_block(const char *format, const char *path, const psr_snap_header *h, const char *block,
+                            void *buf, char *err, size_t errlen) {
     int64_t len, item, off;
-    if (!path || !h || !block || psr_snapshot_block_info(h, block, &len, &item, &off))
-        return snap_err(err, errlen, "unknown block", block ? block : "(null)");
+    if (!path || !h || !block || psr_snapshot_block_info(format, h, block, &len, &item, &off))
+        return snap_err(err, errlen, "unknown block or format", block ? block : "(null)");
     if (len == 0) return PSR_OK;
     if (!buf) return PSR_ERR_ARG;
     FILE *f = fopen(path, "rb");
@@ -535,20 +584,26 @@ int psr_snapshot_read_block(const char *path, const psr_snap_header *h, const ch
     return PSR_OK;
 }
 
-int psr_snapshot_write(const char *path, const psr_snap_header *h, const char *const *names, const void *const *bufs,
-                       int32_t nblocks, char *err, size_t errlen) {
+int psr_snapshot_write(const char *format, const char *path, const psr_snap_header *h, const char *const *names,
+                       const void *const *bufs, int32_t nblocks, char *err, size_t errlen) {
+    snap_format fm;
     if (!path || !h || (nblocks > 0 && (!names || !bufs))) return PSR_ERR_ARG;
+    if (snap_format_of(format, &fm)) return snap_err(err, errlen, "unknown format", format);
     for (int32_t k = 0; k < nblocks; k++)
-        if (snap_block_id(names[k]) <= 0) return snap_err(err, errlen, "unknown block", names[k]);
+        if (!names[k] || snap_block_id(&fm, names[k]) <= 0)
+            return snap_err(err, errlen, "unknown block", names[k] ? names[k] : "(null)");
+    for (int id = 1; id < fm.n; id++)
+        if (snap_length(&fm, h, id) * fm.b[id].item > INT32_MAX)
+            return snap_err(err, errlen, "block over 2 GiB (its int32 size marker)", fm.b[id].name);
     FILE *f = fopen(path, "wb");
     if (!f) return snap_err(err, errlen, "cannot create", path);
     int ok = 1;
-    for (int id = 0; id < SNAP_NBLOCK && ok; id++) {
-        const int64_t bytes = snap_length(h, id) * SNAP_ITEM[id];
+    for (int id = 0; id < fm.n && ok; id++) {
+        const int64_t bytes = snap_length(&fm, h, id) * fm.b[id].item;
         if (bytes == 0) continue;
         const void *src = id == 0 ? (const void *)h : NULL;
         for (int32_t k = 0; k < nblocks && id > 0; k++)
-            if (!strcmp(names[k], SNAP_BLOCKS[id])) src = bufs[k];
+            if (!strcmp(names[k], fm.b[id].name)) src = bufs[k];
         const int32_t m = (int32_t)bytes;
         ok = fwrite(&m, 4, 1, f) == 1;
         if (src) {

@@ -1,8 +1,9 @@
-"""psphray-format snapshots and epoch switching (host side).
+"""Snapshots and epoch switching (host side).
 
-read / write: the psphray snapshot format (src/reader-psphray.inc) through
-psr_snapshot_*: a 256-byte header record, then the blocks in the format's
-order, each framed by its byte count.  write_output mirrors the reference's
+read / write: the psphray snapshot format (src/reader-psphray.inc) and the
+massiveblack one (src/reader-massiveblack.inc) through psr_snapshot_*: a
+256-byte header record, then the blocks in the format's order, each framed
+by its byte count.  write_output mirrors the reference's
 psystem_write_output (src/psystem.c:599-673): gas positions, sml, rho, mass,
 x_HI, x_HeI, x_HeII, ye, yGrecHII (as gammaHI), ie, lasthit, hits, ids, then
 the sources as type-5 particles (positions, Ngamma_dot).
@@ -23,16 +24,29 @@ BLOCK_DTYPE = {
     "eos": ("<f4", 1), "lasthit": ("<i8", 1), "hits": ("<i4", 1), "sft": ("<f4", 1),
     "ngammas": ("<f8", 1), "spec": ("S8", 1),
 }
+BLOCK_DTYPE_MASSIVEBLACK = {
+    "pos": ("<f4", 3), "vel": ("<f4", 3), "id": ("<u8", 1), "mass": ("<f4", 1), "ie": ("<f4", 1),
+    "rho": ("<f4", 1), "ye": ("<f4", 1), "xHI": ("<f4", 1), "sml": ("<f4", 1), "sfr": ("<f4", 1),
+    "sft": ("<f4", 1), "met": ("<f4", 1), "bhmass": ("<f4", 1), "bhmdot": ("<f4", 1),
+}
+FORMATS = {"psphray": BLOCK_DTYPE, "massiveblack": BLOCK_DTYPE_MASSIVEBLACK, "e5": BLOCK_DTYPE_MASSIVEBLACK}
+
+
+def _dtypes(fmt):
+    if fmt not in FORMATS:
+        raise ValueError(f"unknown snapshot format {fmt}")
+    return FORMATS[fmt]
 
 
 def _err():
     return C.create_string_buffer(512)
 
 
-def block_info(hdr, name):
+def block_info(hdr, name, fmt="psphray"):
     """(length, item size, offset) of a block under a header"""
     ln, it, off = C.c_int64(), C.c_int64(), C.c_int64()
-    rc = _abi.lib().psr_snapshot_block_info(C.byref(hdr), name.encode(), C.byref(ln), C.byref(it), C.byref(off))
+    rc = _abi.lib().psr_snapshot_block_info(fmt.encode(), C.byref(hdr), name.encode(), C.byref(ln), C.byref(it),
+                                            C.byref(off))
     if rc:
         raise PsrError(rc, f"unknown block {name}")
     return ln.value, it.value, off.value
@@ -60,13 +74,13 @@ def make_header(ngas, nsrc=0, boxsize=0.0, redshift=0.0, time=1.0, h=0.72, heliu
     return hd
 
 
-def write(path, hdr, blocks):
+def write(path, hdr, blocks, fmt="psphray"):
     """header + blocks (name -> array); blocks the header calls for and
     not given are written as zeros"""
     names, bufs, keep = [], [], []
     for name, arr in blocks.items():
-        dt, k = BLOCK_DTYPE[name]
-        ln, _, _ = block_info(hdr, name)
+        dt, k = _dtypes(fmt)[name]
+        ln, _, _ = block_info(hdr, name, fmt)
         a = np.ascontiguousarray(arr, dtype=dt).reshape(-1)
         if a.size != ln * k:
             raise ValueError(f"block {name}: {a.size} values, the header asks for {ln * k}")
@@ -77,12 +91,12 @@ def write(path, hdr, blocks):
     cn = (C.c_char_p * max(n, 1))(*names)
     cb = (C.c_void_p * max(n, 1))(*bufs)
     err = _err()
-    rc = _abi.lib().psr_snapshot_write(str(path).encode(), C.byref(hdr), cn, cb, n, err, 512)
+    rc = _abi.lib().psr_snapshot_write(fmt.encode(), str(path).encode(), C.byref(hdr), cn, cb, n, err, 512)
     if rc:
         raise PsrError(rc, err.value.decode())
 
 
-def read(path, names=None):
+def read(path, names=None, fmt="psphray"):
     """(header, {block: array}) for every present block (or those named)"""
     hdr = _abi.SnapHeader()
     err = _err()
@@ -90,15 +104,15 @@ def read(path, names=None):
     if rc:
         raise PsrError(rc, err.value.decode())
     out = {}
-    for name, (dt, k) in BLOCK_DTYPE.items():
+    for name, (dt, k) in _dtypes(fmt).items():
         if names is not None and name not in names:
             continue
-        ln, _, _ = block_info(hdr, name)
+        ln, _, _ = block_info(hdr, name, fmt)
         if ln == 0:
             continue
         a = np.zeros(ln * k, dtype=dt)
-        rc = _abi.lib().psr_snapshot_read_block(str(path).encode(), C.byref(hdr), name.encode(), a.ctypes.data,
-                                                err, 512)
+        rc = _abi.lib().psr_snapshot_read_block(fmt.encode(), str(path).encode(), C.byref(hdr), name.encode(),
+                                                a.ctypes.data, err, 512)
         if rc:
             raise PsrError(rc, err.value.decode())
         out[name] = a.reshape(ln, 3) if k == 3 else a
@@ -110,6 +124,14 @@ def ye_of(ps, units):
     (lambda_to_ye, src/psystem.h:134-138; yeMET is not part of it)"""
     xHeIII = 1.0 - ps.lambdaHeII - ps.lambdaHeI
     return (1.0 - ps.lambdaH) + units.C_HEMF / units.C_HMF * 0.25 * (ps.lambdaHeII + 2 * xHeIII)
+
+
+def gas_blocks(path, fmt="psphray"):
+    """the gas blocks an epoch reads (src/psystem.c:333-373: pos, mass, sml,
+    rho, ie, ye, xHI, id), type 0's rows of each"""
+    hdr, b = read(path, names=("pos", "id", "mass", "sml", "rho", "ie", "ye", "xHI"), fmt=fmt)
+    n = int(hdr.N[0])
+    return hdr, {k: v[:n] for k, v in b.items()}
 
 
 def write_output(path, ps, ids, units, boxsize, redshift=0.0, time=1.0, src_pos=None, src_ngd=None):

@@ -72,8 +72,8 @@ def test_snapshot_round_trips(tmp_path, ngas, nsrc, flags):
         setattr(hd, k, v)
     oh = _oracle_header(hd)
     blocks = {}
-    for name, item, dt, k, rule in oe.BLOCKS:
-        ln = oe.block_length(oh, rule)
+    for name, item, dt, k, types, flag in oe.BLOCKS:
+        ln = oe.block_length(oh, types, flag)
         if ln:
             a = rng.integers(0, 255, ln * item, dtype=np.uint8).view(dt)
             blocks[name] = a.reshape(ln, 3) if k == 3 else a
@@ -88,6 +88,43 @@ def test_snapshot_round_trips(tmp_path, ngas, nsrc, flags):
     h2, got2 = oe.read_snapshot(tmp_path / "lib")
     for name in blocks:
         assert got2[name].tobytes() == np.ascontiguousarray(blocks[name]).tobytes(), name
+
+
+@pytest.mark.parametrize("fmt", ["massiveblack", "e5"])
+@pytest.mark.parametrize("N,flags", [((30, 50, 0, 0, 7, 2), {}), ((30, 0, 0, 0, 0, 0), dict(flag_met=0)),
+                                     ((5, 9, 0, 0, 4, 3), dict(flag_sfr=0, flag_sft=0))])
+def test_massiveblack_round_trips(tmp_path, fmt, N, flags):
+    """the massiveblack table (dark matter in pos/vel/id only, metals of gas
+    and stars, black holes' mass and accretion): library and oracle write
+    the same bytes and read each other's files; gas_blocks takes type 0's
+    rows"""
+    rng = np.random.default_rng(sum(N))
+    hd = snap.make_header(N[0])
+    for t in range(6):
+        hd.N[t] = N[t]
+    for k, v in flags.items():
+        setattr(hd, k, v)
+    oh = _oracle_header(hd)
+    blocks = {}
+    for name, item, dt, k, types, flag in oe.BLOCKS_MASSIVEBLACK:
+        ln = oe.block_length(oh, types, flag)
+        if ln:
+            a = rng.integers(0, 255, ln * item, dtype=np.uint8).view(dt)
+            blocks[name] = a.reshape(ln, 3) if k == 3 else a
+    assert ("sfr" in blocks) == bool(hd.flag_sfr) and "vel" in blocks
+    snap.write(tmp_path / "lib", hd, blocks, fmt=fmt)
+    oe.write_snapshot(tmp_path / "orc", oh, blocks, fmt="massiveblack")
+    assert (tmp_path / "lib").read_bytes() == (tmp_path / "orc").read_bytes()
+    _, got = snap.read(tmp_path / "orc", fmt=fmt)
+    assert set(got) == set(blocks)
+    for name in blocks:
+        assert got[name].tobytes() == np.ascontiguousarray(blocks[name]).tobytes(), name
+    _, gas = snap.gas_blocks(tmp_path / "lib", fmt=fmt)
+    assert len(gas["id"]) == N[0] and gas["pos"].tobytes() == blocks["pos"][:N[0]].tobytes()
+    with pytest.raises(ValueError):
+        snap.read(tmp_path / "lib", fmt="gadget")
+    with pytest.raises(psr.PsrError):
+        snap.block_info(hd, "lasthit", fmt)           # a psphray block
 
 
 def test_snapshot_missing_blocks_are_zero_and_bad_files_refused(tmp_path):
