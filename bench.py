@@ -34,7 +34,7 @@ BYTES_DEPOSIT_HE = 68
 # passes, scaled by the same counters on a copy of known size (FETCH_SIZE x 2,
 # WRITE_SIZE x 1), the timed-tick dispatch of each workload (the 128^3,
 # 256^3 and batched-field traffic summaries under profiles/)
-TRAFFIC_DEPOSIT = {2: (84968 * 2 + 50806) * 1024, 3: (5984385 * 2 + 4237966) * 1024,
+TRAFFIC_DEPOSIT = {2: (84968 * 2 + 50806) * 1024, 3: (3104723 * 2 + 5163307) * 1024,
                    5: (151006 * 2 + 92589) * 1024}
 WORKLOADS = {
     1: "32^3 jittered lattice, H-only, isothermal 1e4 K, one source",
@@ -388,7 +388,7 @@ def main():
         "graph_launches_timed": graph_launches,
         "surrogate": sur,
         "roofline": {
-            "kernel": "k_deposit",
+            "kernel": "k_deposit_multi" if walk.startswith("four") else "k_deposit",
             "bound": "hbm",
             "achieved": roof_achieved,
             "peak": HBM_PEAK_GBS,
