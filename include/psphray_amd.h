@@ -171,6 +171,21 @@ double psr_tab_ar_get(const psr_tables *t, int col, double logT);
 double psr_tab_xs_get(const psr_tables *t, int id, double eng);
 double psr_tab_sph_depth(const psr_tables *t, double r_h);
 
+/* LTE recombination spectra (lte_init / lte_fill, src/lte.c:28-72), for
+ * the recombination rays: species s (0: HII -> HI, 1: HeII -> HeI, 2:
+ * HeIII -> HeII) has PSR_LTE_NFREQ photon energies E_j = E0 + j (16 E0 -
+ * E0) / PSR_LTE_NFREQ from its threshold E0 (energy[s][j]), and for each
+ * of PSR_LTE_NLOGT rows of log10 T = 3 + i 7/128 the cumulative weights
+ * sigma(E_j) E_j^3 exp(-(E_j - E0) / kT) divided by the row's total
+ * (cdf[s][i][j]); tbound[k] = 10^(3 + (k + 1) 7/128), k < PSR_LTE_NLOGT - 1:
+ * a temperature T takes the row counting the bounds <= T (the reference's
+ * (int)((log10 T - 3) / (7/128)) clamped to the rows) */
+#define PSR_LTE_NFREQ 1024
+#define PSR_LTE_NLOGT 128
+#define PSR_LTE_LOGT_MIN 3.0
+#define PSR_LTE_LOGT_MAX 10.0
+int psr_lte_tables(const psr_tables *t, const psr_units *u, double *energy, double *cdf, double *tbound);
+
 /* ---- snapshots (reader_new, src/reader.c:36-46) ----
  * format: "psphray" (src/reader-psphray.inc; NULL means it) or
  * "massiveblack" / "e5" (src/reader-massiveblack.inc).  header: the
@@ -302,6 +317,30 @@ int psr_spectra_upload(psr_ctx *ctx, int32_t nspec, const int64_t *offsets, cons
  * get lastemit = tick.  Ticks must increase from call to call. */
 int psr_emit_rays(psr_ctx *ctx, int64_t tick, double tick_time, int64_t nray, int64_t per_field,
                   uint32_t seed, int64_t pool_first, int64_t *nray_out);
+/* recombination rays (emit_rays, src/loop.c:197-232, :273-299, :339-368).
+ * Once set with nrec != 0, and when the physics is not on-the-spot,
+ * psr_emit_rays appends them after the source rays from tick 2 on
+ * (*nray_out counts both).  Candidates: the particles hit by the first
+ * nsrc + 1 rays of the last traced tick, nsrc being the source rays
+ * psr_emit_rays wrote for it (every ray of an uploaded list): the
+ * reference's active set up to x_src_length (src/loop.c:626-637).
+ * nrec > 0: nrec rays per species (HII; with helium also HeII, HeIII), each
+ * drawn from the candidates (ascending caller index) with weight
+ * yGrec_s / x_HI (x_HI = 0 counts as 1e-10), the weights summed in chunks of
+ * 1024 candidates, sequentially within a chunk and over the chunk totals;
+ * nrec < 0: one ray from each candidate with yGrec_s > threshold
+ * (psphray.recombineThreshold, 0.1).  Ray k of species s draws from stream
+ * counter (k, tick, (s + 1) << 16 | block): block 0 word 0 the particle,
+ * word 3 the photon energy, blocks 1.. the direction.  A ray starts at its
+ * particle with yGrec_s N_s (N_s: the particle's hydrogen or helium atoms)
+ * shared among the particle's rays of that species, an energy from the LTE
+ * spectrum of the particle's temperature (psr_lte_tables; disable_lte: the
+ * threshold energy), length 0.2 boxsize, type s, src_or_par its caller
+ * index; rays are sorted by (type, particle), a particle's in draw order;
+ * then yGrec_s -= Nph / N_s for each ray in order, clamped at 0.  A species
+ * whose weights are all zero emits nothing.  Single field only
+ * (PSR_ERR_NOTSUP for batched fields). */
+int psr_set_recombination(psr_ctx *ctx, int64_t nrec, double threshold, int32_t disable_lte);
 /* the sources' lastemit ticks */
 int psr_sources_lastemit(psr_ctx *ctx, int64_t *lastemit);
 /* a copy of nray rays of the pool from first (the reference keeps r[] on

@@ -1655,3 +1655,182 @@ int64_t or_emit_rays(int64_t nsrc, const double *pos, const double *ngd, const i
     free(emitted);
     return nout;
 }
+
+
+/* ------------------------------------------------- recombination rays */
+void or_lte_tables(const or_ctx *c, double *eng, double *cdf, double *tb) {
+    const or_units *u = &c->u;
+    const double th[3] = {u->C_HI_ENERGY, u->C_HEI_ENERGY, u->C_HEII_ENERGY};
+    const double logT_step = (10.0 - 3.0) / 128;
+    for (int s = 0; s < 3; s++) {
+        const double freq_step = (16 * th[s] - th[s]) / 1024;
+        for (int j = 0; j < 1024; j++) eng[s * 1024 + j] = th[s] + freq_step * j;
+        for (int i = 0; i < 128; i++) {
+            const double kT = u->C_BOLTZMANN * exp10(3.0 + logT_step * i);
+            double *row = cdf + ((int64_t)s * 128 + i) * 1024;
+            double sum = 0.0;
+            for (int j = 0; j < 1024; j++) {
+                const double freq = eng[s * 1024 + j];
+                sum += or_xs_get(c, s, freq) * freq * freq * freq * exp(-(freq - th[s]) / kT);
+                row[j] = sum;
+            }
+            for (int j = 0; j < 1024; j++) row[j] = sum > 0 ? row[j] / sum : 1.0;
+        }
+    }
+    for (int k = 0; k < 127; k++) tb[k] = exp10(3.0 + logT_step * (k + 1));
+}
+
+typedef struct {
+    int64_t ipar, k;
+} rec_draw;
+static int rec_draw_cmp(const void *a, const void *b) {
+    const rec_draw *x = a, *y = b;
+    if (x->ipar != y->ipar) return x->ipar < y->ipar ? -1 : 1;
+    return (x->k > y->k) - (x->k < y->k);
+}
+
+int64_t or_emit_recomb(const or_ctx *c, const or_tree *t, or_psys *p, const or_ray *prev, int64_t nprev,
+                       int64_t cut, int64_t nrec, double threshold, int32_t disable_lte,
+                       const double *lte_eng, const double *lte_cdf, const double *lte_tb, double boxsize,
+                       int64_t tick, uint32_t seed, or_ray *out, int64_t cap) {
+    if (c->f.on_the_spot || nrec == 0 || tick <= 1) return 0;
+    const uint32_t K1 = 0x50535248u;
+    const uint32_t tlo = (uint32_t)tick, thi = (uint32_t)((uint64_t)tick >> 32);
+    const int64_t np = p->npar;
+    /* merge_pars' active set up to x_src_length: every particle the first
+     * nr rays hit (src/loop.c:620-637) */
+    uint8_t *hit = calloc(np > 0 ? np : 1, 1);
+    const int64_t nr = nprev < cut ? nprev : cut;
+    if (nr > 0) {
+        int64_t *offs = malloc(sizeof(int64_t) * (nr + 1));
+        or_hit *h = trace_all(t, p, prev, nr, offs, 1);
+        for (int64_t k = 0; k < offs[nr]; k++) hit[h[k].ipar] = 1;
+        free(h);
+        free(offs);
+    }
+    int64_t *cand = malloc(sizeof(int64_t) * (np > 0 ? np : 1));
+    int64_t ncand = 0;
+    for (int64_t i = 0; i < np; i++)
+        if (hit[i]) cand[ncand++] = i;
+    free(hit);
+    float *yg[3] = {p->yGrecHII, p->yGrecHeII, p->yGrecHeIII};
+    const double per_mass[3] = {c->u.C_H_PER_MASS, c->u.C_HE_PER_MASS, c->u.C_HE_PER_MASS};
+    double *cum = malloc(sizeof(double) * (ncand > 0 ? ncand : 1));
+    rec_draw *dr = malloc(sizeof(rec_draw) * ((nrec > ncand ? nrec : ncand) + 1));
+    int64_t nout = 0;
+    for (int s = 0; s < (c->f.h_only ? 1 : 3); s++) {
+        int64_t nd = 0;
+        if (nrec > 0) {
+            if (ncand == 0) continue;
+            /* weights yGrec_s / x_HI (src/loop.c:206-211), summed in chunks */
+            double before = 0.0;
+            for (int64_t j0 = 0; j0 < ncand; j0 += 1024) {
+                const int64_t j1 = j0 + 1024 < ncand ? j0 + 1024 : ncand;
+                double in_chunk = 0.0;
+                for (int64_t j = j0; j < j1; j++) {
+                    double x = p->lambdaH[cand[j]];
+                    if (x == 0) x = 1e-10;
+                    in_chunk += (double)yg[s][cand[j]] / x;
+                    cum[j] = in_chunk;
+                }
+                for (int64_t j = j0; j < j1; j++) cum[j] = before + cum[j];
+                before += in_chunk;
+            }
+            const double total = before;
+            if (!(total > 0)) continue;
+            for (int64_t k = 0; k < nrec; k++) {
+                uint32_t ctr[4] = {(uint32_t)k, tlo, thi, (uint32_t)(s + 1) << 16};
+                philox(ctr, seed, K1);
+                const double u = unif(ctr[0]);
+                int64_t lo = 0, hi = ncand;
+                while (lo < hi) {
+                    const int64_t mid = (lo + hi) / 2;
+                    if (cum[mid] / total <= u) lo = mid + 1;
+                    else hi = mid;
+                }
+                dr[nd].ipar = cand[lo < ncand - 1 ? lo : ncand - 1];
+                dr[nd].k = k;
+                nd++;
+            }
+        } else {
+            /* one ray per candidate above the threshold (src/loop.c:222-229) */
+            for (int64_t j = 0; j < ncand; j++)
+                if ((double)yg[s][cand[j]] > threshold) {
+                    dr[nd].ipar = cand[j];
+                    dr[nd].k = nd;
+                    nd++;
+                }
+        }
+        if (nd == 0) continue;
+        if (nout + nd > cap) {
+            nout = -1;
+            break;
+        }
+        /* ordered by (type, particle), src/loop.c:236 */
+        qsort(dr, (size_t)nd, sizeof(rec_draw), rec_draw_cmp);
+        for (int64_t a = 0; a < nd;) {
+            int64_t b = a;
+            while (b < nd && dr[b].ipar == dr[a].ipar) b++;
+            const int64_t ip = dr[a].ipar;
+            const double N = (double)p->mass[ip] * per_mass[s];
+            for (int64_t q = a; q < b; q++) {
+                or_ray *o = out + nout + q;
+                for (int d = 0; d < 3; d++) o->s[d] = p->pos[3 * ip + d];
+                /* src/loop.c:283-293 and the shared weight of :305-321 */
+                o->Nph = ((double)yg[s][ip] * N) * (1.0 / (double)(b - a));
+                uint32_t c0[4] = {(uint32_t)dr[q].k, tlo, thi, (uint32_t)(s + 1) << 16};
+                philox(c0, seed, K1);
+                if (disable_lte) {
+                    o->freq = lte_eng[s * 1024];
+                } else {
+                    /* lte_gen_freq, src/lte.c:34-45 */
+                    const double T = or_ieye2T(c, p->ie[ip], psys_ye(c, p, ip));
+                    int row = 0;
+                    while (row < 127 && lte_tb[row] <= T) row++;
+                    const double *cd = lte_cdf + ((int64_t)s * 128 + row) * 1024;
+                    const double u3 = unif(c0[3]);
+                    int bin = 0;
+                    while (bin < 1024 && cd[bin] <= u3) bin++;
+                    o->freq = lte_eng[s * 1024 + (bin < 1023 ? bin : 1023)];
+                }
+                double x = 0.0, y = 0.0, z = 1.0;
+                int done = 0;
+                for (uint32_t blk = 1; blk <= 32 && !done; blk++) {
+                    uint32_t d4[4] = {(uint32_t)dr[q].k, tlo, thi, ((uint32_t)(s + 1) << 16) | blk};
+                    philox(d4, seed, K1);
+                    for (int h = 0; h < 2 && !done; h++) {
+                        const double xx = -1.0 + 2.0 * unif(d4[2 * h]);
+                        const double yy = -1.0 + 2.0 * unif(d4[2 * h + 1]);
+                        const double ss = xx * xx + yy * yy;
+                        if (ss <= 1.0) {
+                            const double sc = 2.0 * sqrt(1.0 - ss);
+                            x = xx * sc;
+                            y = yy * sc;
+                            z = -1.0 + 2.0 * ss;
+                            done = 1;
+                        }
+                    }
+                }
+                o->dir[0] = (float)x;
+                o->dir[1] = (float)y;
+                o->dir[2] = (float)z;
+                o->length = boxsize * 2 * 0.1; /* src/loop.c:169 */
+                o->type = s;
+                o->field = 0;
+                o->src_or_par = ip;
+            }
+            /* the emitted photons leave the recombination count
+             * (src/loop.c:339-368) */
+            for (int64_t q = a; q < b; q++) {
+                yg[s][ip] = (float)((double)yg[s][ip] - out[nout + q].Nph / N);
+                if (yg[s][ip] < 0) yg[s][ip] = 0;
+            }
+            a = b;
+        }
+        nout += nd;
+    }
+    free(cand);
+    free(cum);
+    free(dr);
+    return nout;
+}

@@ -222,6 +222,29 @@ int64_t or_emit_rays(int64_t nsrc, const double *pos, const double *ngd, const i
                      const double *boxsizes, int64_t tick, double tick_time, int64_t nray,
                      int64_t per_field, uint32_t seed, or_ray *out, int64_t cap);
 
+/* LTE recombination spectra, lte_fill (src/lte.c:47-72): eng[3][1024]
+ * photon energies from each species' threshold E0 to 16 E0; cdf[3][128][1024]
+ * the running sums of sigma(E) E^3 exp(-(E - E0)/kT) over a row of
+ * log10 T = 3 + i 7/128, divided by the row total; tb[127] the rows' lower
+ * bounds in T, 10^(3 + (k+1) 7/128). */
+void or_lte_tables(const or_ctx *c, double *eng, double *cdf, double *tb);
+
+/* recombination rays of one tick (emit_rays, src/loop.c:197-232, :273-299,
+ * :305-321, :339-368): nothing when on-the-spot, nrec == 0 or tick <= 1.
+ * Candidates: the particles the first min(nprev, cut) rays of the previous
+ * tick's list hit (the active set up to x_src_length, src/loop.c:626-637),
+ * in ascending index.  nrec > 0: per species nrec draws with weights
+ * yGrec_s / x_HI (0 -> 1e-10), cumulative weights summed in chunks of 1024
+ * (sequentially within a chunk, then over the chunk totals); nrec < 0: every
+ * candidate with yGrec_s > threshold.  Draw k of species s uses the stream
+ * counter (k, tick lo, tick hi, (s+1) << 16 | block).  Writes the rays
+ * sorted by (species, particle), takes the emitted photons off yGrec_s,
+ * returns the count (-1: cap). */
+int64_t or_emit_recomb(const or_ctx *c, const or_tree *t, or_psys *p, const or_ray *prev, int64_t nprev,
+                       int64_t cut, int64_t nrec, double threshold, int32_t disable_lte,
+                       const double *lte_eng, const double *lte_cdf, const double *lte_tb, double boxsize,
+                       int64_t tick, uint32_t seed, or_ray *out, int64_t cap);
+
 /* units helpers: src/config.h:295-303 */
 double or_ieye2T(const or_ctx *c, double ie, double ye);
 double or_Tye2ie(const or_ctx *c, double T, double ye);

@@ -96,6 +96,9 @@ _SIG = {
     "or_set_flags": (None, [_VP, C.POINTER(OrFlags)]),
     "or_emit_rays": (_I64, [_I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _I32, _VP, _I64, _D, _I64,
                             _I64, C.c_uint32, _VP, _I64]),
+    "or_lte_tables": (None, [_VP, _VP, _VP, _VP]),
+    "or_emit_recomb": (_I64, [_VP, _VP, C.POINTER(OrPsys), _VP, _I64, _I64, _I64, _D, _I32, _VP, _VP, _VP, _D,
+                              _I64, C.c_uint32, _VP, _I64]),
 }
 
 _lib = None
@@ -184,6 +187,30 @@ class Oracle:
     def set_flags(self, f):
         self.f = f
         self.L.or_set_flags(self.h, C.byref(f))
+
+    def lte_tables(self):
+        """(energies [3][1024], cdf [3][128][1024], row bounds in T [127])"""
+        eng = np.zeros((3, 1024))
+        cdf = np.zeros((3, 128, 1024))
+        tb = np.zeros(127)
+        self.L.or_lte_tables(self.h, p(eng), p(cdf), p(tb))
+        return eng, cdf, tb
+
+    def emit_recomb(self, tree, ps, prev_rays, cut, nrec, tick, seed, boxsize, threshold=0.1, disable_lte=False,
+                    lte=None):
+        """the recombination rays of a tick from the previous tick's rays
+        (or_emit_recomb); yGrec of ps is debited in place"""
+        eng, cdf, tb = self.lte_tables() if lte is None else lte
+        prev = np.ascontiguousarray(prev_rays, RAY_DTYPE)
+        cap = max(1, abs(int(nrec)) * 3 if nrec > 0 else 3 * int(ps.npar))
+        out = np.zeros(cap, RAY_DTYPE)
+        s = psys_struct(ps)
+        cut = min(int(cut), 2**62)
+        n = self.L.or_emit_recomb(self.h, tree.h, C.byref(s), p(prev), len(prev), cut, int(nrec), float(threshold),
+                                  int(bool(disable_lte)), p(eng), p(cdf), p(tb), float(boxsize), int(tick),
+                                  int(seed) & 0xFFFFFFFF, p(out), cap)
+        assert n >= 0
+        return out[:n]
 
     # lookups
     def ar_get(self, col, logT):

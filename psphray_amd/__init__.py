@@ -98,6 +98,18 @@ class Physics:
     def sph_depth(self, r_h):
         return _abi.lib().psr_tab_sph_depth(C.byref(self.tables), r_h)
 
+    def lte_tables(self):
+        """LTE recombination spectra (psr_lte_tables): energies [3][1024],
+        cdf [3][128][1024], row bounds in T [127]"""
+        eng = np.zeros((3, _abi.PSR_LTE_NFREQ))
+        cdf = np.zeros((3, _abi.PSR_LTE_NLOGT, _abi.PSR_LTE_NFREQ))
+        tb = np.zeros(_abi.PSR_LTE_NLOGT - 1)
+        rc = _abi.lib().psr_lte_tables(C.byref(self.tables), C.byref(self.units), eng.ctypes.data, cdf.ctypes.data,
+                                       tb.ctypes.data)
+        if rc:
+            raise PsrError(rc, "psr_lte_tables")
+        return eng, cdf, tb
+
     def kernel_table(self):
         n = self.tables.kernel_n
         return np.ctypeslib.as_array(self.tables.kernel, shape=(n,)).copy()
@@ -273,6 +285,11 @@ class Context:
                     "psr_emit_rays")
         self._nray_pool = max(getattr(self, "_nray_pool", 0), int(pool_first) + n.value)
         return n.value
+
+    def set_recombination(self, nrec, threshold=0.1, disable_lte=False):
+        """recombination rays appended by emit_rays (psr_set_recombination)"""
+        self._check(self._lib.psr_set_recombination(self.h, int(nrec), float(threshold), int(bool(disable_lte))),
+                    "psr_set_recombination")
 
     def sources_lastemit(self, nsrc):
         le = np.zeros(nsrc, np.int64)

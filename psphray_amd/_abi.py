@@ -21,6 +21,8 @@ PSR_CHEM_EXACT = 0
 PSR_CHEM_SURROGATE = 1
 PSR_MLP_IN = 22
 PSR_NOALLOC_MAX_STEPS = 32768
+PSR_LTE_NFREQ = 1024
+PSR_LTE_NLOGT = 128
 PSR_MLP_OUT = 4
 PSR_TUNE_DEPOSIT_ACCUM = 1
 PSR_ACCUM_AUTO = 0
@@ -156,6 +158,8 @@ SIGNATURES = {
     "psr_emit_rays": (C.c_int, [_VP, _I64, _D, _I64, _I64, C.c_uint32, _I64, C.POINTER(C.c_int64)]),
     "psr_sources_lastemit": (C.c_int, [_VP, _VP]),
     "psr_rays_download": (C.c_int, [_VP, _I64, _I64, _VP]),
+    "psr_set_recombination": (C.c_int, [_VP, _I64, _D, C.c_int32]),
+    "psr_lte_tables": (C.c_int, [C.POINTER(Tables), C.POINTER(Units), _VP, _VP, _VP]),
     "psr_snapshot_block_info": (C.c_int, [C.c_char_p, C.POINTER(SnapHeader), C.c_char_p, C.POINTER(C.c_int64),
                                           C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "psr_snapshot_read_header": (C.c_int, [C.c_char_p, C.POINTER(SnapHeader), C.c_char_p, C.c_size_t]),

@@ -160,6 +160,7 @@ extern "C" void psr_destroy(psr_ctx *c) {
     free_state(c);
     psr_mlp_free(c);
     dfree(c->d_tables);
+    dfree(c->d_lte);
     dfree(c->d_rays);
     dfree(c->H.d);
     dfree(c->H.b);
@@ -268,6 +269,15 @@ extern "C" int psr_set_physics(psr_ctx *c, const psr_units *u, const psr_flags *
         T.si_tab3 = d;
     }
     c->T = T;
+    // LTE recombination spectra for the recombination rays (src/lte.c:28-72)
+    {
+        const size_t ne = 3 * (size_t)PSR_LTE_NFREQ, nc = ne * PSR_LTE_NLOGT, nb = PSR_LTE_NLOGT - 1;
+        std::vector<double> lte(ne + nc + nb);
+        if (psr_lte_tables(t, u, lte.data(), lte.data() + ne, lte.data() + ne + nc))
+            PSR_FAIL(c, PSR_ERR_ARG, "psr_set_physics: LTE spectra");
+        if (!c->d_lte) PSR_HIP(c, hipMalloc(&c->d_lte, sizeof(double) * lte.size()));
+        PSR_HIP(c, hipMemcpy(c->d_lte, lte.data(), sizeof(double) * lte.size(), hipMemcpyHostToDevice));
+    }
     c->U = *u;
     c->F = *f;
     c->have_physics = true;
@@ -483,6 +493,7 @@ extern "C" int psr_tree_build(psr_ctx *c, const float *pos, const float *sml, co
     hipStream_t st = c->stream;
     free_tree(c);
     free_state(c);
+    c->hits_nray = 0;  // the hit lists name slots of the old tree
     c->npar = npar;
     c->nfield = nfield;
     c->boxsizes.assign(boxsizes, boxsizes + nfield);
@@ -833,6 +844,8 @@ extern "C" int psr_rays_upload(psr_ctx *c, const psr_ray *rays, int64_t nray) {
     PSR_HIP(c, hipSetDevice(c->device));
     // a replay of a pending tick needs the pool it was launched with
     if (int rc = psr_resolve(c)) return rc;
+    // the recombination candidates of the tick tracing these rays: every hit
+    c->rec_cut = INT64_MAX;
     // the trace indexes the field roots with each ray's field: every index is
     // checked before the pool is touched, and a refused upload leaves no pool
     int32_t maxf = -1;

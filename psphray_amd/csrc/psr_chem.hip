@@ -692,15 +692,6 @@ struct Integ {
     }
 };
 
-__device__ __forceinline__ double ieye2T(const DevPhys &P, double ie, double ye) {
-    return P.U_MPROTON_OVER_C_BOLTZMANN * ie / (P.C_HEMF * 0.25 + P.C_HMF + P.C_HMF * ye) * 2.0 / 3.0;
-}
-__device__ __forceinline__ double Tye2ie(const DevPhys &P, double T, double ye) {
-    return T * P.C_BOLTZMANN_OVER_U_MPROTON * (P.C_HEMF * 0.25 + P.C_HMF + P.C_HMF * ye) / 2.0 * 3.0;
-}
-__device__ __forceinline__ double lambda_ye(const DevPhys &P, double lH, double lHeI, double lHeII) {
-    return (1.0 - lH) + P.C_HEMF / P.C_HMF * 0.25 * (lHeII + 2 * (1.0 - lHeII - lHeI));
-}
 
 // Step set-up of one particle, src/loop.c:662-711
 __device__ __forceinline__ void load_step(const DevState &S, uint32_t i, int64_t tick, double tick_time,

@@ -185,6 +185,224 @@ __global__ void k_emit_clear(int64_t nsrc, uint32_t *__restrict__ cnt) {
     if (s < nsrc) cnt[s] = 0;
 }
 
+// ---- recombination rays (src/loop.c:197-232, :273-299, :339-368)
+// The candidates are marked in caller order (marker[caller] = tree slot + 1),
+// compacted in ascending caller index, and each species' draw weights
+// yGrec_s / x_HI summed in chunks of REC_CHUNK candidates: within a chunk
+// sequentially, the chunk totals sequentially, each candidate's cumulative
+// weight = the chunks before it + its running sum in its chunk (the oracle
+// sums in the same order, so both pick the same particles).
+constexpr int REC_CHUNK = 1024;
+
+__global__ void k_rec_mark(DevHits H, int64_t nr, const uint32_t *__restrict__ perm, uint32_t *__restrict__ marker) {
+    for (int64_t r = blockIdx.x; r < nr; r += gridDim.x) {
+        const int n = min(H.count[r], H.cap);
+        const uint32_t *sl = H.slot + r * (int64_t)H.cap;
+        for (int j = threadIdx.x; j < n; j += blockDim.x) {
+            const uint32_t k = sl[j];
+            marker[perm[k]] = k + 1;
+        }
+    }
+}
+
+__global__ void k_rec_flag(int64_t n, const uint32_t *__restrict__ marker, int32_t *__restrict__ flag) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = marker[i] != 0;
+}
+
+__global__ void k_rec_compact(int64_t n, const uint32_t *__restrict__ marker, const int32_t *__restrict__ pos,
+                              uint32_t *__restrict__ cand_slot, uint32_t *__restrict__ cand_ipar) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n || !marker[i]) return;
+    cand_slot[pos[i]] = marker[i] - 1;
+    cand_ipar[pos[i]] = (uint32_t)i;
+}
+
+// running sums of the weights within each chunk; chunk totals into ctot
+__global__ void k_rec_chunk(int64_t ncand, const uint32_t *__restrict__ cand_slot, const float *__restrict__ yg,
+                            const double *__restrict__ lH, double *__restrict__ cdf, double *__restrict__ ctot) {
+    const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t j0 = b * REC_CHUNK;
+    if (j0 >= ncand) return;
+    const int64_t j1 = min(ncand, j0 + REC_CHUNK);
+    double run = 0.0;
+    for (int64_t j = j0; j < j1; j++) {
+        const uint32_t k = cand_slot[j];
+        double x = lH[k];
+        if (x == 0) x = 1e-10;
+        run += (double)yg[k] / x;
+        cdf[j] = run;
+    }
+    ctot[b] = run;
+}
+
+// chunk offsets (exclusive running sum, in place) and the total at [nchunk]
+__global__ void k_rec_offsets(int64_t nchunk, double *__restrict__ ctot) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double run = 0.0;
+    for (int64_t b = 0; b < nchunk; b++) {
+        const double t = ctot[b];
+        ctot[b] = run;
+        run += t;
+    }
+    ctot[nchunk] = run;
+}
+
+__global__ void k_rec_cdf(int64_t ncand, double *__restrict__ cdf, const double *__restrict__ ctot) {
+    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j < ncand) cdf[j] = ctot[j / REC_CHUNK] + cdf[j];
+}
+
+// nrec draws of species s: key (s << 32) | caller index, value the draw
+__global__ void k_rec_draw(int s, int64_t nrec, int64_t ncand, const double *__restrict__ cdf,
+                           const double *__restrict__ ctot, int64_t nchunk, const uint32_t *__restrict__ cand_ipar,
+                           int64_t tick, uint32_t seed, unsigned long long *__restrict__ key,
+                           uint32_t *__restrict__ val) {
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= nrec) return;
+    const double total = ctot[nchunk];
+    const Philox4 b0 = philox4x32_10((uint32_t)q, (uint32_t)tick, (uint32_t)((uint64_t)tick >> 32),
+                                     (uint32_t)(s + 1) << 16, seed, EMIT_KEY1);
+    const double u = uniform01(b0.v[0]);
+    int64_t lo = 0, hi = ncand;  // first candidate with cdf / total > u
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (cdf[mid] / total <= u) lo = mid + 1;
+        else hi = mid;
+    }
+    const int64_t pick = lo < ncand - 1 ? lo : ncand - 1;
+    key[q] = ((unsigned long long)s << 32) | cand_ipar[pick];
+    val[q] = (uint32_t)q;
+}
+
+// threshold mode: flags of the candidates with yGrec_s > threshold
+__global__ void k_rec_above(int64_t ncand, const uint32_t *__restrict__ cand_slot, const float *__restrict__ yg,
+                            double threshold, int32_t *__restrict__ flag) {
+    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j < ncand) flag[j] = (double)yg[cand_slot[j]] > threshold;
+}
+__global__ void k_rec_above_keys(int s, int64_t ncand, const int32_t *__restrict__ flag,
+                                 const int32_t *__restrict__ rank, const uint32_t *__restrict__ cand_ipar,
+                                 unsigned long long *__restrict__ key, uint32_t *__restrict__ val) {
+    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= ncand || !flag[j]) return;
+    key[rank[j]] = ((unsigned long long)s << 32) | cand_ipar[j];
+    val[rank[j]] = (uint32_t)rank[j];
+}
+
+struct RecTabs {
+    const double *eng;    // [3][PSR_LTE_NFREQ]
+    const double *cdf;    // [3][PSR_LTE_NLOGT][PSR_LTE_NFREQ]
+    const double *tb;     // [PSR_LTE_NLOGT - 1]
+};
+
+// the rays, sorted by (species, particle), each group's draws in order
+__global__ void k_rec_write(int64_t n, const unsigned long long *__restrict__ skey, const uint32_t *__restrict__ sval,
+                            const uint32_t *__restrict__ marker, DevState S, DevPhys P, RecTabs L, int disable_lte,
+                            double length, int64_t tick, uint32_t seed, psr_ray *__restrict__ out) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const unsigned long long kk = skey[p];
+    const int s = (int)(kk >> 32);
+    const uint32_t ipar = (uint32_t)kk;
+    const uint32_t k = marker[ipar] - 1;
+    // the particle's rays of this species: [lower, upper) of its key
+    int64_t lo = 0, hi = p;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (skey[mid] < kk) lo = mid + 1;
+        else hi = mid;
+    }
+    const int64_t first = lo;
+    lo = p + 1;
+    hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (skey[mid] <= kk) lo = mid + 1;
+        else hi = mid;
+    }
+    const int64_t cnt = lo - first;
+    const float *yg = s == 0 ? S.yGrecHII : s == 1 ? S.yGrecHeII : S.yGrecHeIII;
+    const double N = (double)S.mass[k] * (s == 0 ? P.C_H_PER_MASS : P.C_HE_PER_MASS);
+    const uint32_t q = sval[p];
+    const uint32_t tlo = (uint32_t)tick, thi = (uint32_t)((uint64_t)tick >> 32);
+    const uint32_t sb = (uint32_t)(s + 1) << 16;
+    const Philox4 b0 = philox4x32_10(q, tlo, thi, sb, seed, EMIT_KEY1);
+    psr_ray r;
+    const float4 ps = S.ps[k];
+    r.s[0] = ps.x;
+    r.s[1] = ps.y;
+    r.s[2] = ps.z;
+    double x = 0.0, y = 0.0, z = 1.0;
+    bool done = false;
+    for (int blk = 1; blk <= DIR_BLOCKS && !done; blk++) {
+        const Philox4 w = philox4x32_10(q, tlo, thi, sb | (uint32_t)blk, seed, EMIT_KEY1);
+        for (int h = 0; h < 2 && !done; h++) {
+            const double xx = -1.0 + 2.0 * uniform01(w.v[2 * h]);
+            const double yy = -1.0 + 2.0 * uniform01(w.v[2 * h + 1]);
+            const double ss = xx * xx + yy * yy;
+            if (ss <= 1.0) {
+                const double sc = 2.0 * sqrt(1.0 - ss);
+                x = xx * sc;
+                y = yy * sc;
+                z = -1.0 + 2.0 * ss;
+                done = true;
+            }
+        }
+    }
+    r.dir[0] = (float)x;
+    r.dir[1] = (float)y;
+    r.dir[2] = (float)z;
+    r.Nph = ((double)yg[k] * N) * (1.0 / (double)cnt);
+    // LTE photon energy at the particle's temperature (lte_gen_freq, src/lte.c:34-45)
+    const double *eng = L.eng + (size_t)s * PSR_LTE_NFREQ;
+    if (disable_lte) {
+        r.freq = eng[0];
+    } else {
+        const double T = ieye2T(P, S.ie[k], lambda_ye(P, S.lH[k], S.lHeI[k], S.lHeII[k]));
+        int row = 0;
+        while (row < PSR_LTE_NLOGT - 1 && L.tb[row] <= T) row++;
+        const double *c = L.cdf + ((size_t)s * PSR_LTE_NLOGT + row) * PSR_LTE_NFREQ;
+        const double u3 = uniform01(b0.v[3]);
+        int lo2 = 0, hi2 = PSR_LTE_NFREQ;
+        while (lo2 < hi2) {
+            const int mid = (lo2 + hi2) >> 1;
+            if (c[mid] <= u3) lo2 = mid + 1;
+            else hi2 = mid;
+        }
+        r.freq = eng[lo2 < PSR_LTE_NFREQ - 1 ? lo2 : PSR_LTE_NFREQ - 1];
+    }
+    r.length = length;
+    r.type = s;
+    r.field = 0;
+    r.src_or_par = ipar;
+    out[p] = r;
+}
+
+// yGrec_s -= Nph / N_s for each ray, in ray order, clamped at 0
+// (src/loop.c:339-368); one thread per particle and species
+__global__ void k_rec_debit(int64_t n, const unsigned long long *__restrict__ skey, const uint32_t *__restrict__ marker,
+                            DevState S, DevPhys P, const psr_ray *__restrict__ rays) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= n || (p > 0 && skey[p - 1] == skey[p])) return;
+    const unsigned long long kk = skey[p];
+    const int s = (int)(kk >> 32);
+    const uint32_t k = marker[(uint32_t)kk] - 1;
+    float *yg = s == 0 ? S.yGrecHII : s == 1 ? S.yGrecHeII : S.yGrecHeIII;
+    const double N = (double)S.mass[k] * (s == 0 ? P.C_H_PER_MASS : P.C_HE_PER_MASS);
+    float v = yg[k];
+    for (int64_t q = p; q < n && skey[q] == kk; q++) {
+        v = (float)((double)v - rays[q].Nph / N);
+        if (v < 0) v = 0;
+    }
+    yg[k] = v;
+}
+
+__global__ void k_fill_u32(int64_t n, uint32_t *__restrict__ a) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
 inline unsigned blocks(int64_t n, int t = 256) { return (unsigned)std::max<int64_t>(1, (n + t - 1) / t); }
 
 }  // namespace
@@ -214,6 +432,16 @@ struct EmitState {
     int64_t grp_cap = 0;
     double *d_box = nullptr;
     int64_t box_n = 0;
+    // recombination rays: per particle (caller order) the candidate marker,
+    // flags and their scan; candidates' tree slots and caller indices and
+    // their cumulative weights; chunk totals; the draws' keys and values
+    int64_t rec_npar = 0;
+    uint32_t *d_marker = nullptr, *d_cslot = nullptr, *d_cipar = nullptr;
+    int32_t *d_flag = nullptr, *d_pos = nullptr;
+    double *d_ccdf = nullptr, *d_ctot = nullptr;
+    int64_t rec_cap = 0;
+    unsigned long long *d_rkey = nullptr, *d_rskey = nullptr;
+    uint32_t *d_rval = nullptr, *d_rsval = nullptr;
 };
 
 template <typename T>
@@ -241,6 +469,17 @@ void psr_emit_free(psr_ctx *c) {
     efree(e->d_fbin);
     efree(e->d_grp);
     efree(e->d_box);
+    efree(e->d_marker);
+    efree(e->d_cslot);
+    efree(e->d_cipar);
+    efree(e->d_flag);
+    efree(e->d_pos);
+    efree(e->d_ccdf);
+    efree(e->d_ctot);
+    efree(e->d_rkey);
+    efree(e->d_rskey);
+    efree(e->d_rval);
+    efree(e->d_rsval);
     delete e;
     c->emit = nullptr;
 }
@@ -339,6 +578,125 @@ extern "C" int psr_sources_lastemit(psr_ctx *c, int64_t *lastemit) {
     return PSR_OK;
 }
 
+extern "C" int psr_set_recombination(psr_ctx *c, int64_t nrec, double threshold, int32_t disable_lte) {
+    if (!c || !(threshold >= 0)) return PSR_ERR_ARG;
+    c->rec_nrec = nrec;
+    c->rec_threshold = threshold;
+    c->rec_disable_lte = disable_lte ? 1 : 0;
+    return PSR_OK;
+}
+
+// the recombination draws of a tick, sorted by (species, particle) into
+// e->d_rskey / d_rsval; *n their count (0 when none are due)
+static int rec_plan(psr_ctx *c, EmitState *e, int64_t tick, uint32_t seed, int64_t cut, int64_t *n) {
+    *n = 0;
+    if (c->rec_nrec == 0 || c->F.on_the_spot || tick <= 1) return PSR_OK;
+    if (!c->have_state) PSR_FAIL(c, PSR_ERR_STATE, "recombination rays need the particle state");
+    if (c->nfield > 1) PSR_FAIL(c, PSR_ERR_NOTSUP, "recombination rays on batched fields");
+    if (!c->d_lte) PSR_FAIL(c, PSR_ERR_STATE, "recombination rays need psr_set_physics");
+    // the hit lists of the last traced tick (an overflowed one replayed first)
+    if (int rc = psr_resolve(c)) return rc;
+    const int64_t nr = std::min(c->hits_nray, cut);
+    const int64_t np = c->npar;
+    if (nr <= 0 || np == 0) return PSR_OK;
+    if (np > e->rec_npar) {
+        efree(e->d_marker);
+        efree(e->d_cslot);
+        efree(e->d_cipar);
+        efree(e->d_flag);
+        efree(e->d_pos);
+        efree(e->d_ccdf);
+        efree(e->d_ctot);
+        PSR_HIP(c, hipMalloc(&e->d_marker, sizeof(uint32_t) * np));
+        PSR_HIP(c, hipMalloc(&e->d_cslot, sizeof(uint32_t) * np));
+        PSR_HIP(c, hipMalloc(&e->d_cipar, sizeof(uint32_t) * np));
+        PSR_HIP(c, hipMalloc(&e->d_flag, sizeof(int32_t) * np));
+        PSR_HIP(c, hipMalloc(&e->d_pos, sizeof(int32_t) * np));
+        PSR_HIP(c, hipMalloc(&e->d_ccdf, sizeof(double) * np));
+        PSR_HIP(c, hipMalloc(&e->d_ctot, sizeof(double) * (np / REC_CHUNK + 2)));
+        e->rec_npar = np;
+    }
+    if (psr_ensure_scratch(c, psr_scan_scratch_i32(np))) return PSR_ERR_HIP;
+    k_fill_u32<<<blocks(np), 256, 0, c->stream>>>(np, e->d_marker);
+    k_rec_mark<<<(unsigned)std::min<int64_t>(nr, 65535 * 4), 256, 0, c->stream>>>(c->H, nr, c->d_perm, e->d_marker);
+    k_rec_flag<<<blocks(np), 256, 0, c->stream>>>(np, e->d_marker, e->d_flag);
+    PSR_HIP(c, hipGetLastError());
+    PSR_HIP(c, psr_exclusive_sum_i32(e->d_flag, e->d_pos, np, c->d_scratch, c->stream));
+    k_rec_compact<<<blocks(np), 256, 0, c->stream>>>(np, e->d_marker, e->d_pos, e->d_cslot, e->d_cipar);
+    PSR_HIP(c, hipGetLastError());
+    int32_t tail[2] = {0, 0};
+    PSR_HIP(c, hipMemcpyAsync(&tail[0], e->d_pos + np - 1, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    PSR_HIP(c, hipMemcpyAsync(&tail[1], e->d_flag + np - 1, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    PSR_HIP(c, hipStreamSynchronize(c->stream));
+    const int64_t ncand = (int64_t)tail[0] + tail[1];
+    if (ncand == 0) return PSR_OK;
+    const int ns = c->F.h_only ? 1 : 3;
+    const DevState S = c->state();
+    const float *yg[3] = {S.yGrecHII, S.yGrecHeII, S.yGrecHeIII};
+    // the draws' counts per species first (threshold mode: a scan of flags)
+    int64_t cnt[3] = {0, 0, 0}, total_n = 0;
+    const int64_t nchunk = (ncand + REC_CHUNK - 1) / REC_CHUNK;
+    for (int sp = 0; sp < ns; sp++) {
+        if (c->rec_nrec > 0) {
+            k_rec_chunk<<<blocks(nchunk, 64), 64, 0, c->stream>>>(ncand, e->d_cslot, yg[sp], S.lH, e->d_ccdf, e->d_ctot);
+            k_rec_offsets<<<1, 1, 0, c->stream>>>(nchunk, e->d_ctot);
+            PSR_HIP(c, hipGetLastError());
+            double tot = 0;
+            PSR_HIP(c, hipMemcpyAsync(&tot, e->d_ctot + nchunk, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            PSR_HIP(c, hipStreamSynchronize(c->stream));
+            // no weight (every yGrec_s zero): no rays of this species
+            cnt[sp] = tot > 0 ? c->rec_nrec : 0;
+        } else {
+            k_rec_above<<<blocks(ncand), 256, 0, c->stream>>>(ncand, e->d_cslot, yg[sp], c->rec_threshold, e->d_flag);
+            PSR_HIP(c, hipGetLastError());
+            PSR_HIP(c, psr_exclusive_sum_i32(e->d_flag, e->d_pos, ncand, c->d_scratch, c->stream));
+            PSR_HIP(c, hipMemcpyAsync(&tail[0], e->d_pos + ncand - 1, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+            PSR_HIP(c, hipMemcpyAsync(&tail[1], e->d_flag + ncand - 1, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+            PSR_HIP(c, hipStreamSynchronize(c->stream));
+            cnt[sp] = (int64_t)tail[0] + tail[1];
+        }
+        total_n += cnt[sp];
+    }
+    if (total_n == 0) return PSR_OK;
+    if (total_n >= (int64_t)INT32_MAX) PSR_FAIL(c, PSR_ERR_ARG, "psr_emit_rays: too many recombination rays");
+    if (total_n > e->rec_cap) {
+        efree(e->d_rkey);
+        efree(e->d_rskey);
+        efree(e->d_rval);
+        efree(e->d_rsval);
+        PSR_HIP(c, hipMalloc(&e->d_rkey, sizeof(unsigned long long) * total_n));
+        PSR_HIP(c, hipMalloc(&e->d_rskey, sizeof(unsigned long long) * total_n));
+        PSR_HIP(c, hipMalloc(&e->d_rval, sizeof(uint32_t) * total_n));
+        PSR_HIP(c, hipMalloc(&e->d_rsval, sizeof(uint32_t) * total_n));
+        e->rec_cap = total_n;
+    }
+    int64_t off = 0;
+    for (int sp = 0; sp < ns; sp++) {
+        if (cnt[sp] == 0) continue;
+        if (c->rec_nrec > 0) {
+            k_rec_chunk<<<blocks(nchunk, 64), 64, 0, c->stream>>>(ncand, e->d_cslot, yg[sp], S.lH, e->d_ccdf, e->d_ctot);
+            k_rec_offsets<<<1, 1, 0, c->stream>>>(nchunk, e->d_ctot);
+            k_rec_cdf<<<blocks(ncand), 256, 0, c->stream>>>(ncand, e->d_ccdf, e->d_ctot);
+            k_rec_draw<<<blocks(cnt[sp]), 256, 0, c->stream>>>(sp, cnt[sp], ncand, e->d_ccdf, e->d_ctot, nchunk,
+                                                              e->d_cipar, tick, seed, e->d_rkey + off,
+                                                              e->d_rval + off);
+        } else {
+            k_rec_above<<<blocks(ncand), 256, 0, c->stream>>>(ncand, e->d_cslot, yg[sp], c->rec_threshold, e->d_flag);
+            PSR_HIP(c, psr_exclusive_sum_i32(e->d_flag, e->d_pos, ncand, c->d_scratch, c->stream));
+            k_rec_above_keys<<<blocks(ncand), 256, 0, c->stream>>>(sp, ncand, e->d_flag, e->d_pos, e->d_cipar,
+                                                                   e->d_rkey + off, e->d_rval + off);
+        }
+        PSR_HIP(c, hipGetLastError());
+        off += cnt[sp];
+    }
+    // (species, particle) keys: 2 + 32 bits, stable in the draw order
+    if (psr_ensure_scratch(c, psr_sort_scratch_u64(total_n))) return PSR_ERR_HIP;
+    PSR_HIP(c, psr_sort_pairs_u64(e->d_rkey, e->d_rval, e->d_rskey, e->d_rsval, total_n, 0, 34, c->d_scratch,
+                                  c->stream));
+    *n = total_n;
+    return PSR_OK;
+}
+
 extern "C" int psr_emit_rays(psr_ctx *c, int64_t tick, double tick_time, int64_t nray, int64_t per_field,
                              uint32_t seed, int64_t pool_first, int64_t *nray_out) {
     if (!c || nray < 0 || per_field < 0 || pool_first < 0) return PSR_ERR_ARG;
@@ -379,12 +737,19 @@ extern "C" int psr_emit_rays(psr_ctx *c, int64_t tick, double tick_time, int64_t
     } else {
         for (int32_t f = 0; f < nf; f++) group(e->gsrc_off[f], e->gsrc_off[f + 1], per_field, f * per_field);
     }
-    if (nray_out) *nray_out = nout;
     if ((int64_t)G.size() >= (int64_t)1 << 31 || nout >= (int64_t)UINT32_MAX)
         PSR_FAIL(c, PSR_ERR_ARG, "psr_emit_rays: too many rays");
+    // recombination rays from the last traced tick's candidates, after the
+    // source rays; the next tick's candidates come from the first nout + 1
+    // rays of this list
+    const int64_t cut = c->rec_cut;
+    int64_t nrec = 0;
+    if (int rc = rec_plan(c, e, tick, seed, cut, &nrec)) return rc;
+    c->rec_cut = nout + 1;
+    if (nray_out) *nray_out = nout + nrec;
     // a pending tick whose rays overlap the range (an overflow replay needs
     // them) is resolved first; so is a pool that must grow
-    const int64_t end = pool_first + nout;
+    const int64_t end = pool_first + nout + nrec;
     bool overlap = end > c->ray_cap;
     for (const PendingTick &t : c->pending)
         if (t.ray_first < end && pool_first < t.ray_first + t.nray) overlap = true;
@@ -401,6 +766,23 @@ extern "C" int psr_emit_rays(psr_ctx *c, int64_t tick, double tick_time, int64_t
         c->ray_cap = std::max<int64_t>(end, 2 * c->ray_cap);
     }
     e->last_tick = tick;
+    if (nrec) {
+        const DevState S = c->state();
+        const DevPhys P = psr_dev_phys(c);
+        RecTabs L;
+        L.eng = c->d_lte;
+        L.cdf = c->d_lte + 3 * PSR_LTE_NFREQ;
+        L.tb = L.cdf + (size_t)3 * PSR_LTE_NLOGT * PSR_LTE_NFREQ;
+        psr_ray *out = c->d_rays + pool_first + nout;
+        // the recombination ray length, src/loop.c:169
+        const double length = c->boxsizes[0] * 2 * 0.1;
+        k_rec_write<<<blocks(nrec), 256, 0, c->stream>>>(nrec, e->d_rskey, e->d_rsval, e->d_marker, S, P, L,
+                                                        c->rec_disable_lte, length, tick, seed, out);
+        k_rec_debit<<<blocks(nrec), 256, 0, c->stream>>>(nrec, e->d_rskey, e->d_marker, S, P, out);
+        PSR_HIP(c, hipGetLastError());
+        c->nray_pool = std::max(c->nray_pool, end);
+        c->ray_field_max = std::max(c->ray_field_max, 0);
+    }
     if (nout == 0) return PSR_OK;
     if (nout > e->cap) {
         efree(e->d_key);

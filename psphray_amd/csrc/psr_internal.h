@@ -59,6 +59,19 @@ struct DevPhys {
     double fake_temperature;
 };
 
+// temperature from internal energy and electrons per hydrogen, and back
+// (ieye2T / Tye2ie, src/psystem.h); electrons per hydrogen from the
+// fractions (lambda_to_ye, src/psystem.h:134-138)
+__device__ __forceinline__ double ieye2T(const DevPhys &P, double ie, double ye) {
+    return P.U_MPROTON_OVER_C_BOLTZMANN * ie / (P.C_HEMF * 0.25 + P.C_HMF + P.C_HMF * ye) * 2.0 / 3.0;
+}
+__device__ __forceinline__ double Tye2ie(const DevPhys &P, double T, double ye) {
+    return T * P.C_BOLTZMANN_OVER_U_MPROTON * (P.C_HEMF * 0.25 + P.C_HMF + P.C_HMF * ye) / 2.0 * 3.0;
+}
+__device__ __forceinline__ double lambda_ye(const DevPhys &P, double lH, double lHeI, double lHeII) {
+    return (1.0 - lH) + P.C_HEMF / P.C_HMF * 0.25 * (lHeII + 2 * (1.0 - lHeII - lHeI));
+}
+
 // particle state pointers (tree order)
 // packed per-particle inputs of the deposit in tree order (sml, mass and the
 // fractions the optical depths need), so that a hit gathers one line instead
@@ -216,6 +229,17 @@ struct psr_ctx {
     int64_t hit_slots = 0;  // number of rays the slot arrays hold
     double *d_ray_len = nullptr, *d_ray_tm = nullptr;
     int64_t last_nray = 0;
+    // recombination rays: LTE spectra on the device (energies [3][1024],
+    // cdf [3][128][1024], row bounds in T [127]), the rays per species
+    // (0: off), threshold and disableLTE, and how many leading rays of the
+    // last emitted list make candidates with their hits (all of an uploaded
+    // list)
+    double *d_lte = nullptr;
+    int64_t rec_nrec = 0;
+    double rec_threshold = 0.1;
+    int32_t rec_disable_lte = 0;
+    int64_t rec_cut = INT64_MAX;
+    int64_t hits_nray = 0;  // rays whose hit lists hold slots of the current tree (0 after a rebuild)
     // deposit records binned by particle range; d_bin_off holds off[nbins + 1],
     // fill[nbins] and claimed[nbins]
     void *d_bin_rec = nullptr;

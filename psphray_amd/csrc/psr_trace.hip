@@ -518,6 +518,7 @@ int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray, bool sync, con
         PSR_HIP(c, hipGetLastError());
         PSR_EV(c, 0, 1);
         c->last_nray = nray;
+        c->hits_nray = nray;
         return PSR_OK;
     }
     for (int attempt = 0; attempt < 8; attempt++) {
@@ -533,6 +534,7 @@ int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray, bool sync, con
         if (fl & PSR_DF_STACK) PSR_FAIL(c, PSR_ERR_CAPACITY, "traversal stack overflow");
         if (!(fl & PSR_DF_HITCAP)) {
             c->last_nray = nray;
+            c->hits_nray = nray;
             return PSR_OK;
         }
         // grow the per-ray hit capacity to the largest list and trace again

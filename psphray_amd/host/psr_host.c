@@ -450,6 +450,37 @@ double psr_tab_sph_depth(const psr_tables *t, double r_h) {
     return (n < t->kernel_n && n >= 0) ? t->kernel[n] : 0.0;
 }
 
+/* ------------------------------------------------------------ LTE spectra */
+/* lte_init / lte_fill (src/lte.c:28-72): per species the photon energies
+ * E_j = E0 + j (16 E0 - E0) / 1024 and, per row i of log10 T = 3 + i 7/128,
+ * the weights sigma(E_j) E_j^3 exp(-(E_j - E0) / kT) (in that order of
+ * operations), here as cumulative sums divided by the row's total; the
+ * rows' lower bounds in T, tbound[k] = 10^(3 + (k + 1) 7/128), pick a row
+ * for a temperature without a logarithm */
+int psr_lte_tables(const psr_tables *t, const psr_units *u, double *energy, double *cdf, double *tbound) {
+    if (!t || !u || !energy || !cdf || !tbound || !t->xs) return PSR_ERR_ARG;
+    const double e0[3] = {u->C_HI_ENERGY, u->C_HEI_ENERGY, u->C_HEII_ENERGY};
+    const double step = (PSR_LTE_LOGT_MAX - PSR_LTE_LOGT_MIN) / PSR_LTE_NLOGT;
+    for (int s = 0; s < 3; s++) {
+        const double fstep = (16 * e0[s] - e0[s]) / PSR_LTE_NFREQ;
+        double *eng = energy + (size_t)s * PSR_LTE_NFREQ;
+        for (int j = 0; j < PSR_LTE_NFREQ; j++) eng[j] = e0[s] + fstep * j;
+        for (int i = 0; i < PSR_LTE_NLOGT; i++) {
+            const double kT = u->C_BOLTZMANN * exp10(PSR_LTE_LOGT_MIN + step * i);
+            double *row = cdf + ((size_t)s * PSR_LTE_NLOGT + i) * PSR_LTE_NFREQ;
+            double run = 0.0;
+            for (int j = 0; j < PSR_LTE_NFREQ; j++) {
+                const double f = eng[j];
+                run += psr_tab_xs_get(t, s, f) * f * f * f * exp(-(f - e0[s]) / kT);
+                row[j] = run;
+            }
+            for (int j = 0; j < PSR_LTE_NFREQ; j++) row[j] = run > 0 ? row[j] / run : 1.0;
+        }
+    }
+    for (int k = 0; k < PSR_LTE_NLOGT - 1; k++) tbound[k] = exp10(PSR_LTE_LOGT_MIN + step * (k + 1));
+    return PSR_OK;
+}
+
 /* ------------------------------------------------------------ snapshots */
 /* Gadget-style snapshots: a 256-byte header, then the format's blocks in a
  * fixed order, each as <int32 size> data <int32 size>, present when its
