@@ -34,8 +34,10 @@ BYTES_DEPOSIT_HE = 68
 # passes, scaled by the same counters on a copy of known size (FETCH_SIZE x 2,
 # WRITE_SIZE x 1), the timed-tick dispatch of each workload (the 128^3,
 # 256^3 and batched-field traffic summaries under profiles/)
-TRAFFIC_DEPOSIT = {2: (84968 * 2 + 50806) * 1024, 3: (3104723 * 2 + 5163307) * 1024,
-                   5: (151006 * 2 + 92589) * 1024}
+TRAFFIC_DEPOSIT = {(2, "k_deposit"): (84968 * 2 + 50806) * 1024,
+                   (3, "k_deposit_multi"): (3104723 * 2 + 5163307) * 1024,
+                   (3, "k_deposit"): (5984385 * 2 + 4237966) * 1024,
+                   (5, "k_deposit"): (151006 * 2 + 92589) * 1024}
 WORKLOADS = {
     1: "32^3 jittered lattice, H-only, isothermal 1e4 K, one source",
     2: "128^3 lognormal, H-only, isothermal 1e4 K, 64 sources",
@@ -352,6 +354,9 @@ def main():
     bytes_dep = BYTES_DEPOSIT_HE if he else BYTES_DEPOSIT_H
     dep_launch_s = tm["deposit"] / 1e3 / K
     roof_achieved = st_t["total_deposit_count"] / K * bytes_dep / dep_launch_s / 1e9
+    # the deposit kernel the autotuner chose; traffic is the counters' figure
+    # for that kernel on this workload, where measured
+    dep_kernel = "k_deposit_multi" if walk.startswith("four") else "k_deposit"
     out = {
         "metric": "ray-particle intersections/s + ticks/s, exact FP64 chemistry (value) vs "
                   "MLP-surrogate chemistry (surrogate) at <=1e-3 x_HII error",
@@ -388,13 +393,13 @@ def main():
         "graph_launches_timed": graph_launches,
         "surrogate": sur,
         "roofline": {
-            "kernel": "k_deposit_multi" if walk.startswith("four") else "k_deposit",
+            "kernel": dep_kernel,
             "bound": "hbm",
             "achieved": roof_achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": roof_achieved / HBM_PEAK_GBS,
-            "traffic": TRAFFIC_DEPOSIT.get(args.config),
+            "traffic": TRAFFIC_DEPOSIT.get((args.config, dep_kernel)),
             "bytes_per_intersection": bytes_dep,
             "launch_us": dep_launch_s * 1e6,
         },

@@ -240,7 +240,7 @@ def match_epoch(ps, snap, C_HMF, h_only, periodic=False, boxsize=0.0):
             ps.yeMET[best] = yeMET
             for f in ("yGdepHI", "yGdepHeI", "yGdepHeII", "yGrecHII", "yGrecHeII", "yGrecHeIII"):
                 getattr(ps, f)[best] = 0.0
-            if ps.npar_max > 3:                # the loop counter i is 3 here (src/psystem.c:477-478)
+            if ps.npar_max > 3:                # the loop counter i is 3 here (src/psystem.c:483-484)
                 ps.heat[3] = 0.0
                 ps.hits[3] = 0
             ps.lasthit[best] = 0

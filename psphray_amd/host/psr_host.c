@@ -842,7 +842,7 @@ int psr_epoch_switch(psr_state *s, uint64_t *id, int64_t npar_max, const psr_epo
             s->yGdepHI[best] = s->yGdepHeI[best] = s->yGdepHeII[best] = 0.0f;
             s->yGrecHII[best] = s->yGrecHeII[best] = s->yGrecHeIII[best] = 0.0f;
             /* the reference clears heat and hits of particle 3, its loop
-             * counter, not of the new one (src/psystem.c:477-478) */
+             * counter, not of the new one (src/psystem.c:483-484) */
             if (npar_max > 3) {
                 s->heat[3] = 0.0f;
                 s->hits[3] = 0;
