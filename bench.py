@@ -35,7 +35,7 @@ BYTES_DEPOSIT_HE = 68
 # WRITE_SIZE x 1), the timed-tick dispatch of each workload (the 128^3,
 # 256^3 and batched-field traffic summaries under profiles/)
 TRAFFIC_DEPOSIT = {(2, "k_deposit"): (84968 * 2 + 50806) * 1024,
-                   (3, "k_deposit_multi"): (3104723 * 2 + 5163307) * 1024,
+                   (3, "k_deposit_multi"): (3104723 * 2 + 5163307) * 1024,   # four rays per workgroup
                    (3, "k_deposit"): (5984385 * 2 + 4237966) * 1024,
                    (5, "k_deposit"): (151006 * 2 + 92589) * 1024}
 WORKLOADS = {
@@ -182,8 +182,9 @@ def main():
                     help="surrogate residual threshold (library default 1e-4)")
     ap.add_argument("--accum", choices=["auto", "atomic", "binned"], default="auto",
                     help="deposit accumulation (auto: timed on the first ticks)")
-    ap.add_argument("--walk", choices=["auto", "single", "multi"], default="auto",
-                    help="deposit: one ray or four rays per workgroup (auto: timed)")
+    ap.add_argument("--walk", choices=["auto", "single", "multi", "multi2", "multi8"], default="auto",
+                    help="deposit: one ray per 3-wave workgroup, or four (two, eight) rays per 4-wave "
+                         "workgroup (auto: timed)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mlp-forward-rows", type=int, default=0,
                     help="only time the surrogate network's forward kernel on this many random "
@@ -233,7 +234,8 @@ def main():
     ctx = psr.Context(0)
     accum_ids = {"auto": psr.PSR_ACCUM_AUTO, "atomic": psr.PSR_ACCUM_ATOMIC, "binned": psr.PSR_ACCUM_BINNED}
     ctx.set_tuning(psr.PSR_TUNE_DEPOSIT_WALK, {"auto": psr.PSR_WALK_AUTO, "single": psr.PSR_WALK_SINGLE,
-                                               "multi": psr.PSR_WALK_MULTI}[args.walk])
+                                               "multi": psr.PSR_WALK_MULTI, "multi2": psr.PSR_WALK_MULTI2,
+                                               "multi8": psr.PSR_WALK_MULTI8}[args.walk])
     ctx.set_tuning(psr.PSR_TUNE_DEPOSIT_ACCUM, accum_ids[args.accum])
     ctx.set_physics(wl.physics)
     t0 = time.perf_counter()
@@ -266,8 +268,9 @@ def main():
         raise SystemExit("autotuner undecided after the warm-up ticks; no line is printed")
     warmup_ticks = args.warmup + extra
     accum = {v: k for k, v in accum_ids.items()}[ctx.tuning()]
-    walk = {psr.PSR_WALK_SINGLE: "one ray per workgroup",
-            psr.PSR_WALK_MULTI: "four rays per workgroup"}[ctx.tuning(psr.PSR_TUNE_DEPOSIT_WALK)]
+    walk = {psr.PSR_WALK_SINGLE: "one ray per workgroup", psr.PSR_WALK_MULTI2: "two rays per workgroup",
+            psr.PSR_WALK_MULTI: "four rays per workgroup",
+            psr.PSR_WALK_MULTI8: "eight rays per workgroup"}[ctx.tuning(psr.PSR_TUNE_DEPOSIT_WALK)]
     graph = {0: "kernel by kernel", 1: "graph"}[ctx.tuning(psr.PSR_TUNE_TICK_GRAPH)]
     log(f"{warmup_ticks} warm-up ticks; deposit: {walk}, {accum} accumulation; ticks: {graph}")
 
@@ -356,7 +359,7 @@ def main():
     roof_achieved = st_t["total_deposit_count"] / K * bytes_dep / dep_launch_s / 1e9
     # the deposit kernel the autotuner chose; traffic is the counters' figure
     # for that kernel on this workload, where measured
-    dep_kernel = "k_deposit_multi" if walk.startswith("four") else "k_deposit"
+    dep_kernel = "k_deposit" if walk == "one ray per workgroup" else "k_deposit_multi"
     out = {
         "metric": "ray-particle intersections/s + ticks/s, exact FP64 chemistry (value) vs "
                   "MLP-surrogate chemistry (surrogate) at <=1e-3 x_HII error",

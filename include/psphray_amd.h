@@ -404,12 +404,16 @@ int psr_get_timers(psr_ctx *ctx, double *ms);
  *   PSR_WALK_MULTI   four rays per 4-wavefront workgroup (32-hit segments;
  *                    the four walks in four lanes, two apply wavefronts),
  *                    rays batched by hit count; f32 atomics only
+ *   PSR_WALK_MULTI2  the same with two rays (64-hit segments)
+ *   PSR_WALK_MULTI8  the same with eight rays (16-hit segments)
  * The values are the same bits either way (deposits within f32 summation
- * order).  PSR_WALK_MULTI with PSR_ACCUM_BINNED is PSR_ERR_NOTSUP. */
+ * order).  The several-ray walks with PSR_ACCUM_BINNED are PSR_ERR_NOTSUP. */
 #define PSR_TUNE_DEPOSIT_WALK 4
 #define PSR_WALK_AUTO 0
 #define PSR_WALK_SINGLE 1
 #define PSR_WALK_MULTI 2
+#define PSR_WALK_MULTI2 3
+#define PSR_WALK_MULTI8 4
 int psr_set_tuning(psr_ctx *ctx, int32_t param, int32_t value);
 int psr_get_tuning(psr_ctx *ctx, int32_t param, int32_t *value);
 

@@ -34,6 +34,8 @@ PSR_TUNE_DEPOSIT_WALK = 4
 PSR_WALK_AUTO = 0
 PSR_WALK_SINGLE = 1
 PSR_WALK_MULTI = 2
+PSR_WALK_MULTI2 = 3
+PSR_WALK_MULTI8 = 4
 
 _D = C.c_double
 _I32 = C.c_int32

@@ -919,12 +919,19 @@ __global__ __launch_bounds__(PLAN_THREADS) void k_bin_plan(uint32_t *__restrict_
 // checkpoint.  Values are those of the kernel above bit for bit (same
 // prep_hit, walk_step and apply_values); the cut is tested after each live
 // hit.  Rays come in batches of similar hit counts (k_ray_order).
+// MR is 2, 4 or 8 (the autotuner times them; one ray with 128-hit segments
+// measured 1.6x slower than two on 256^3: a single walking lane leaves the
+// walk's latency exposed); MSEG = 128 / MR.
 constexpr int ORD_THREADS = 1024;
-constexpr int MR = 4;                 // rays per workgroup
-constexpr int MSEG = 32;              // hits per ray and segment
+constexpr int MHITS = 128;            // hits per iteration: MR rays x MSEG
 constexpr int MW = 4;                 // wavefronts: walk, prep, apply, apply
 constexpr int MTHREADS = 64 * MW;
-static_assert(MR * MSEG == 128, "one prep wavefront: two hits per lane; one hit per apply lane");
+template <int MR>
+struct MShape {
+    static_assert(MR == 2 || MR == 4 || MR == 8, "two, four or eight rays per workgroup");
+    static constexpr int SEG = MHITS / MR;  // hits per ray and segment
+    static constexpr int ROW = SEG + 1;     // padded LDS row (see below)
+};
 
 // rays by descending hit count: buckets of equal bit length (count 0 last)
 __global__ __launch_bounds__(ORD_THREADS) void k_ray_order(const int32_t *__restrict__ count, int64_t nray,
@@ -952,12 +959,11 @@ __global__ __launch_bounds__(ORD_THREADS) void k_ray_order(const int32_t *__rest
 // a ray's row of records (and of TMs) is padded by one entry, so that the
 // MR walking lanes, which read the same record index of different rays, hit
 // different LDS banks
-constexpr int MROW = MSEG + 1;
-
-template <bool HE>
+template <bool HE, int MR>
 struct MLds {
+    static constexpr int MROW = MShape<MR>::ROW;
     WalkRec<HE> rec[3][MR * MROW];
-    HitAux<HE> aux[3][MR * MSEG];
+    HitAux<HE> aux[3][MHITS];
     double tmh[3][MR * MROW];  // TM before each hit (walk -> apply)
     RayConst kc[MR];           // per-ray constants (cross sections, energies)
     int64_t ray[MR];           // ray index, -1 for an empty place in the batch
@@ -969,8 +975,8 @@ struct MLds {
 
 // hit k of a ray's list for the prep wavefront: ray place p, hit index in
 // the list (clamped into it; a ray without hits reads particle 0)
-template <bool HE>
-__device__ __forceinline__ void mr_index(HitGather<HE> &g, const DevHits &H, const MLds<HE> &L, int p, int k) {
+template <bool HE, int MR>
+__device__ __forceinline__ void mr_index(HitGather<HE> &g, const DevHits &H, const MLds<HE, MR> &L, int p, int k) {
     const int n = L.n[p];
     if (n > 0) {
         const int64_t at = L.ray[p] * (int64_t)H.cap + min(k, n - 1);
@@ -982,13 +988,14 @@ __device__ __forceinline__ void mr_index(HitGather<HE> &g, const DevHits &H, con
     }
 }
 
-template <bool HE>
+template <bool HE, int MR>
 __global__ __launch_bounds__(MTHREADS, 3) void k_deposit_multi(const DevTick *__restrict__ tk, int64_t nray,
                                                                DevHits H, const DevCtx *__restrict__ C,
                                                                DevStats *__restrict__ st0,
                                                                double *__restrict__ ray_len,
                                                                double *__restrict__ ray_tm, const int32_t *halt) {
-    __shared__ MLds<HE> L;
+    constexpr int MSEG = MShape<MR>::SEG, MROW = MShape<MR>::ROW;
+    __shared__ MLds<HE, MR> L;
     if (halted(halt)) return;
     const psr_ray *__restrict__ rays = tk->rays;
     const DevState &S = C->S;
@@ -1039,12 +1046,12 @@ __global__ __launch_bounds__(MTHREADS, 3) void k_deposit_multi(const DevTick *__
 #pragma unroll
             for (int h = 0; h < HPL; h++) {
                 const int j = h * 64 + lane, p = j / MSEG, k = j % MSEG;
-                mr_index<HE>(g1[h], H, L, p, k);
+                mr_index<HE, MR>(g1[h], H, L, p, k);
                 gather_hit<HE>(g1[h], S);
-                mr_index<HE>(g2[h], H, L, p, MSEG + k);
+                mr_index<HE, MR>(g2[h], H, L, p, MSEG + k);
                 gather_hit<HE>(g2[h], S);
                 HitGather<HE> t1;
-                mr_index<HE>(t1, H, L, p, 2 * MSEG + k);
+                mr_index<HE, MR>(t1, H, L, p, 2 * MSEG + k);
                 a_slot[h] = t1.slot;
                 a_b[h] = t1.b;
                 sd[h] = sph_depth(T, g1[h].b * (1.0 / (double)g1[h].sml));
@@ -1075,7 +1082,7 @@ __global__ __launch_bounds__(MTHREADS, 3) void k_deposit_multi(const DevTick *__
                     g2[h].b = a_b[h];
                     gather_hit<HE>(g2[h], S);
                     HitGather<HE> t2;
-                    mr_index<HE>(t2, H, L, p, (i + 3) * MSEG + k);
+                    mr_index<HE, MR>(t2, H, L, p, (i + 3) * MSEG + k);
                     a_slot[h] = t2.slot;
                     a_b[h] = t2.b;
                 }
@@ -1286,18 +1293,26 @@ int psr_launch_deposit(psr_ctx *c, int64_t, int64_t nray, int mode) {
     if (rc) return rc;
     const bool he = !c->F.h_only;
     const DevTick *rays = c->d_tick;
-    if (mode == PSR_DEP_MULTI) {
+    if (mode >= PSR_DEP_MULTI) {
+        const int mr = psr_dep_rays(mode);
         const int32_t *halt = &c->d_stats->flags;
         PSR_EV(c, 2, 0);
         k_ray_order<<<1, ORD_THREADS, 0, c->stream>>>(c->H.count, nray, c->H.order, halt);
         PSR_HIP(c, hipGetLastError());
-        const unsigned gm = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nray + MR - 1) / MR, 65535 * 4));
-        if (he)
-            k_deposit_multi<true><<<gm, MTHREADS, 0, c->stream>>>(rays, nray, c->H, c->d_ctx, c->d_stats,
-                                                                 c->d_ray_len, c->d_ray_tm, halt);
-        else
-            k_deposit_multi<false><<<gm, MTHREADS, 0, c->stream>>>(rays, nray, c->H, c->d_ctx, c->d_stats,
-                                                                  c->d_ray_len, c->d_ray_tm, halt);
+        const unsigned gm = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nray + mr - 1) / mr, 65535 * 4));
+#define DEP_M(HE_, MR_)                                                                                 \
+    k_deposit_multi<HE_, MR_><<<gm, MTHREADS, 0, c->stream>>>(rays, nray, c->H, c->d_ctx, c->d_stats, \
+                                                              c->d_ray_len, c->d_ray_tm, halt)
+        if (he) {
+            if (mr == 2) DEP_M(true, 2);
+            else if (mr == 4) DEP_M(true, 4);
+            else DEP_M(true, 8);
+        } else {
+            if (mr == 2) DEP_M(false, 2);
+            else if (mr == 4) DEP_M(false, 4);
+            else DEP_M(false, 8);
+        }
+#undef DEP_M
         PSR_HIP(c, hipGetLastError());
         PSR_EV(c, 2, 1);
         return PSR_OK;

@@ -125,8 +125,13 @@ struct DevHits {
 };
 
 // how a tick's deposit runs: one ray per 3-wavefront workgroup with f32
-// atomics or binned records, or several rays per workgroup (atomics)
-enum { PSR_DEP_WG_ATOMIC = 0, PSR_DEP_WG_BINNED = 1, PSR_DEP_MULTI = 2 };
+// atomics or binned records, or two, four or eight rays per 4-wavefront
+// workgroup (atomics)
+enum { PSR_DEP_WG_ATOMIC = 0, PSR_DEP_WG_BINNED = 1, PSR_DEP_MULTI = 2, PSR_DEP_MULTI2 = 3, PSR_DEP_MULTI8 = 4 };
+// rays per workgroup of a deposit mode
+inline int psr_dep_rays(int mode) {
+    return mode == PSR_DEP_MULTI ? 4 : mode == PSR_DEP_MULTI2 ? 2 : mode == PSR_DEP_MULTI8 ? 8 : 1;
+}
 
 // device counters: PSR_NSTAT copies, one 128-byte line each; a block adds
 // into copy blockIdx.x % PSR_NSTAT and the host folds the copies.  Device-
@@ -296,7 +301,7 @@ struct psr_ctx {
     int32_t accum_req = PSR_ACCUM_AUTO;
     int32_t walk_req = PSR_WALK_AUTO;
     int32_t dep_use = -1;            // the decided PSR_DEP_* mode; -1 while timing
-    static constexpr int DTUNE_N = 6;  // deposit trials: two per candidate mode
+    static constexpr int DTUNE_N = 10;  // deposit trials: two per candidate mode
     hipEvent_t tune_ev[2 * DTUNE_N] = {};
     int32_t tune_slot[DTUNE_N] = {};  // ring slot of the trial's tick; -1: a synchronous replay
     int32_t tune_mode[DTUNE_N] = {};

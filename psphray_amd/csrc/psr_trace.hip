@@ -679,16 +679,22 @@ void psr_tune_reset(psr_ctx *c) {
     c->graph_use = -1;
 }
 
-// the deposit modes the requests leave open, in a fixed order
+// the deposit modes the requests leave open, in a fixed order (at most
+// DEP_NMODE)
+constexpr int DEP_NMODE = 5;
 static int dep_candidates(const psr_ctx *c, int *cand) {
     int n = 0;
-    const bool wg = c->walk_req != PSR_WALK_MULTI;
-    const bool lane = c->walk_req != PSR_WALK_SINGLE && c->accum_req != PSR_ACCUM_BINNED;
+    const int w = c->walk_req;
+    const bool wg = w == PSR_WALK_AUTO || w == PSR_WALK_SINGLE;
+    const bool multi = c->accum_req != PSR_ACCUM_BINNED;
     if (wg && c->accum_req != PSR_ACCUM_BINNED) cand[n++] = PSR_DEP_WG_ATOMIC;
     if (wg && c->accum_req != PSR_ACCUM_ATOMIC) cand[n++] = PSR_DEP_WG_BINNED;
-    if (lane) cand[n++] = PSR_DEP_MULTI;
+    if (multi && (w == PSR_WALK_AUTO || w == PSR_WALK_MULTI)) cand[n++] = PSR_DEP_MULTI;
+    if (multi && (w == PSR_WALK_AUTO || w == PSR_WALK_MULTI2)) cand[n++] = PSR_DEP_MULTI2;
+    if (multi && (w == PSR_WALK_AUTO || w == PSR_WALK_MULTI8)) cand[n++] = PSR_DEP_MULTI8;
     return n;
 }
+static_assert(2 * DEP_NMODE <= psr_ctx::DTUNE_N, "two trials per candidate");
 
 // once every trial has run: keep the mode whose fastest trial deposit was
 // fastest.  A trial on an asynchronous tick that overflowed (its kernels
@@ -696,13 +702,14 @@ static int dep_candidates(const psr_ctx *c, int *cand) {
 // do.  Without a valid trial of each candidate the trials start again.
 // Blocks only when `block`.
 static int tune_decide(psr_ctx *c, bool block) {
-    int cand[3];
+    int cand[DEP_NMODE];
     const int nc = dep_candidates(c, cand);
     if (c->dep_use >= 0 || nc < 2 || c->tune_n < 2 * nc) return PSR_OK;
     const hipEvent_t last = c->tune_ev[2 * (2 * nc) - 1];
     if (block) PSR_HIP(c, hipEventSynchronize(last));
     else if (hipEventQuery(last) != hipSuccess) return PSR_OK;
-    float best[3] = {INFINITY, INFINITY, INFINITY};
+    float best[DEP_NMODE];
+    for (int m = 0; m < DEP_NMODE; m++) best[m] = INFINITY;
     for (int k = 0; k < 2 * nc; k++) {
         if (c->tune_slot[k] >= 0) {
             const int32_t v = ((volatile int32_t *)c->h_ring)[c->tune_slot[k]];
@@ -741,7 +748,7 @@ __global__ void k_fill_u8(uint8_t *p, int64_t n) {
 // decision, the first candidate
 static int tick_mode(const psr_ctx *c, int *trial) {
     *trial = -1;
-    int cand[3];
+    int cand[DEP_NMODE];
     const int nc = dep_candidates(c, cand);
     if (nc == 1) return cand[0];
     if (c->dep_use >= 0) return c->dep_use;
@@ -1037,14 +1044,13 @@ extern "C" int psr_set_tuning(psr_ctx *c, int32_t param, int32_t value) {
     if (param == PSR_TUNE_DEPOSIT_ACCUM) {
         if (value != PSR_ACCUM_AUTO && value != PSR_ACCUM_ATOMIC && value != PSR_ACCUM_BINNED)
             PSR_FAIL(c, PSR_ERR_ARG, "unknown deposit accumulation");
-        if (value == PSR_ACCUM_BINNED && c->walk_req == PSR_WALK_MULTI)
+        if (value == PSR_ACCUM_BINNED && c->walk_req != PSR_WALK_AUTO && c->walk_req != PSR_WALK_SINGLE)
             PSR_FAIL(c, PSR_ERR_NOTSUP, "several rays per workgroup accumulate with atomics only");
         c->accum_req = value;
         psr_tune_reset(c);
     } else if (param == PSR_TUNE_DEPOSIT_WALK) {
-        if (value != PSR_WALK_AUTO && value != PSR_WALK_SINGLE && value != PSR_WALK_MULTI)
-            PSR_FAIL(c, PSR_ERR_ARG, "unknown deposit walk");
-        if (value == PSR_WALK_MULTI && c->accum_req == PSR_ACCUM_BINNED)
+        if (value < PSR_WALK_AUTO || value > PSR_WALK_MULTI8) PSR_FAIL(c, PSR_ERR_ARG, "unknown deposit walk");
+        if (value >= PSR_WALK_MULTI && c->accum_req == PSR_ACCUM_BINNED)
             PSR_FAIL(c, PSR_ERR_NOTSUP, "several rays per workgroup accumulate with atomics only");
         c->walk_req = value;
         psr_tune_reset(c);
@@ -1061,7 +1067,7 @@ extern "C" int psr_set_tuning(psr_ctx *c, int32_t param, int32_t value) {
 
 extern "C" int psr_get_tuning(psr_ctx *c, int32_t param, int32_t *value) {
     if (!c || !value) return PSR_ERR_ARG;
-    int cand[3];
+    int cand[DEP_NMODE];
     const int nc = dep_candidates(c, cand);
     const int mode = nc == 1 ? cand[0] : c->dep_use;
     if (param == PSR_TUNE_DEPOSIT_ACCUM)
@@ -1073,6 +1079,8 @@ extern "C" int psr_get_tuning(psr_ctx *c, int32_t param, int32_t *value) {
         *value = c->walk_req != PSR_WALK_AUTO ? c->walk_req
                  : mode < 0                   ? PSR_WALK_AUTO
                  : mode == PSR_DEP_MULTI       ? PSR_WALK_MULTI
+                 : mode == PSR_DEP_MULTI2      ? PSR_WALK_MULTI2
+                 : mode == PSR_DEP_MULTI8      ? PSR_WALK_MULTI8
                                               : PSR_WALK_SINGLE;
     else if (param == PSR_TUNE_TICK_GRAPH) *value = c->graph_req >= 0 ? c->graph_req : c->graph_use;
     else if (param == PSR_TUNE_GRAPH_LAUNCHES) *value = (int32_t)std::min<int64_t>(c->graph_launches, INT32_MAX);

@@ -75,8 +75,9 @@ CASES = [(1, 16, 64), (1, 32, 32), (2, 24, 256), (3, 20, 256)]
 # deposit modes: one ray per workgroup with atomics or binned records, and
 # four rays per workgroup (atomics)
 ACCUM = [(psr.PSR_ACCUM_ATOMIC, psr.PSR_WALK_SINGLE), (psr.PSR_ACCUM_BINNED, psr.PSR_WALK_SINGLE),
-         (psr.PSR_ACCUM_ATOMIC, psr.PSR_WALK_MULTI)]
-ACCUM_IDS = ["atomic", "binned", "multi"]
+         (psr.PSR_ACCUM_ATOMIC, psr.PSR_WALK_MULTI), (psr.PSR_ACCUM_ATOMIC, psr.PSR_WALK_MULTI2),
+         (psr.PSR_ACCUM_ATOMIC, psr.PSR_WALK_MULTI8)]
+ACCUM_IDS = ["atomic", "binned", "multi", "multi2", "multi8"]
 
 
 @pytest.fixture
@@ -446,9 +447,10 @@ def test_binned_deposit_full_bins(he):
 
 
 def test_deposit_accumulation_is_tuned(ctx):
-    """automatic deposit mode: the first six deposit ticks cycle through
-    atomic and binned sums and four rays per workgroup (each timed); a choice is made
-    once they are done, and every tick's sums are right whichever ran"""
+    """automatic deposit mode: the first ten deposit ticks cycle through
+    one-ray atomic and binned sums and two, four and eight rays per 4-wave
+    workgroup (each timed twice); a choice is made once they are done, and
+    every tick's sums are right whichever ran"""
     print_case("autotune")
     wl = make_workload(2, RATES, n=24, nray=256)
     orc = oracle_for(wl)
@@ -457,12 +459,13 @@ def test_deposit_accumulation_is_tuned(ctx):
     ctx.reset_stats()
     rays = wl.rays(1)
     ctx.rays_upload(rays)
-    nt = 6
+    nt = 10
     for t in range(1, nt + 1):
         ctx.tick(t, wl.tick_time, phases=psr.PSR_PHASE_TRACE | psr.PSR_PHASE_DEPOSIT)
     ctx.sync()
     assert ctx.tuning() in (psr.PSR_ACCUM_ATOMIC, psr.PSR_ACCUM_BINNED)
-    assert ctx.tuning(psr.PSR_TUNE_DEPOSIT_WALK) in (psr.PSR_WALK_SINGLE, psr.PSR_WALK_MULTI)
+    assert ctx.tuning(psr.PSR_TUNE_DEPOSIT_WALK) in (psr.PSR_WALK_SINGLE, psr.PSR_WALK_MULTI,
+                                                     psr.PSR_WALK_MULTI2, psr.PSR_WALK_MULTI8)
     out = wl.ps.copy()
     ctx.state_download(out)
     ref = wl.ps.copy()
