@@ -414,6 +414,18 @@ int psr_get_timers(psr_ctx *ctx, double *ms);
 #define PSR_WALK_MULTI 2
 #define PSR_WALK_MULTI2 3
 #define PSR_WALK_MULTI8 4
+/* the autotuner's decided choices kept between runs (no counterpart in the
+ * reference): psr_tuning_save writes them to a text file, one line per
+ * workload signature (device arch and compute units, particle and field
+ * counts, helium, isothermal, on-the-spot, adiabatic), replacing that
+ * workload's line; psr_tuning_load applies the line matching this
+ * context's workload (PSR_ERR_STATE when there is none), so its ticks skip
+ * the trials.  Call after psr_tree_build and psr_set_physics, which restart
+ * the tuning.  With the environment variable PSR_TUNING_FILE set, every
+ * context loads from that file at its first tick and writes each decision
+ * back. */
+int psr_tuning_save(psr_ctx *ctx, const char *path);
+int psr_tuning_load(psr_ctx *ctx, const char *path);
 int psr_set_tuning(psr_ctx *ctx, int32_t param, int32_t value);
 int psr_get_tuning(psr_ctx *ctx, int32_t param, int32_t *value);
 

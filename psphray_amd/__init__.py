@@ -287,6 +287,14 @@ class Context:
         self._nray_pool = max(getattr(self, "_nray_pool", 0), int(pool_first) + n.value)
         return n.value
 
+    def tuning_save(self, path):
+        """the decided launch choices into a file (psr_tuning_save)"""
+        self._check(self._lib.psr_tuning_save(self.h, str(path).encode()), "psr_tuning_save")
+
+    def tuning_load(self, path):
+        """this workload's saved launch choices (psr_tuning_load)"""
+        self._check(self._lib.psr_tuning_load(self.h, str(path).encode()), "psr_tuning_load")
+
     def set_recombination(self, nrec, threshold=0.1, disable_lte=False):
         """recombination rays appended by emit_rays (psr_set_recombination)"""
         self._check(self._lib.psr_set_recombination(self.h, int(nrec), float(threshold), int(bool(disable_lte))),

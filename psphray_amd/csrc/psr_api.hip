@@ -117,7 +117,10 @@ extern "C" int psr_create(psr_ctx **out, int device) {
         return PSR_ERR_HIP;
     }
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        c->cu_count = prop.multiProcessorCount;
+        c->arch = prop.gcnArchName;
+    }
     if (hipMalloc(&c->d_stats, sizeof(DevStats) * PSR_NSTAT) != hipSuccess ||
         hipMemset(c->d_stats, 0, sizeof(DevStats) * PSR_NSTAT) != hipSuccess ||
         psr_noalloc_init(device) != PSR_OK) {

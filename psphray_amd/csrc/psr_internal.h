@@ -193,6 +193,7 @@ struct psr_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     int cu_count = 256;
+    std::string arch;  // the device's gcnArchName (tuning signatures)
     // physics
     bool have_physics = false;
     psr_units U{};
@@ -306,6 +307,7 @@ struct psr_ctx {
     int32_t tune_slot[DTUNE_N] = {};  // ring slot of the trial's tick; -1: a synchronous replay
     int32_t tune_mode[DTUNE_N] = {};
     int tune_n = 0;
+    bool tune_file_tried = false;  // PSR_TUNING_FILE looked up for this workload
 
     DevState state() const;
 };

@@ -674,6 +674,7 @@ extern "C" int psr_trace_lists(psr_ctx *c, int64_t ray_first, int64_t nray, int6
 
 void psr_tune_reset(psr_ctx *c) {
     c->tune_n = 0;
+    c->tune_file_tried = false;
     c->dep_use = -1;
     c->gtune_n = 0;
     c->graph_use = -1;
@@ -695,6 +696,107 @@ static int dep_candidates(const psr_ctx *c, int *cand) {
     return n;
 }
 static_assert(2 * DEP_NMODE <= psr_ctx::DTUNE_N, "two trials per candidate");
+
+// ---- persistence of the decided choices
+// One line per workload: "psr-tuning 1 <signature> : dep=<mode> graph=<0|1>",
+// the signature naming what the choices depend on (device, particle and
+// field counts, physics); -1 for a choice not made.
+static std::string tune_signature(const psr_ctx *c) {
+    char b[320];
+    snprintf(b, sizeof b, "psr-tuning 1 arch=%s cu=%d npar=%lld nfield=%d he=%d iso=%d ots=%d adi=%d",
+             c->arch.empty() ? "unknown" : c->arch.c_str(), c->cu_count, (long long)c->npar, (int)c->nfield,
+             (int)!c->F.h_only, (int)c->F.isothermal, (int)c->F.on_the_spot, (int)c->F.adiabatic);
+    return b;
+}
+
+static bool read_lines(const char *path, std::vector<std::string> &lines) {
+    FILE *f = fopen(path, "r");
+    if (!f) return false;
+    char buf[1024];
+    while (fgets(buf, sizeof buf, f)) {
+        std::string l(buf);
+        while (!l.empty() && (l.back() == '\n' || l.back() == '\r')) l.pop_back();
+        if (!l.empty()) lines.push_back(l);
+    }
+    fclose(f);
+    return true;
+}
+
+static int tune_save(psr_ctx *c, const char *path) {
+    const std::string sig = tune_signature(c);
+    char tail[64];
+    snprintf(tail, sizeof tail, " : dep=%d graph=%d", (int)c->dep_use, (int)c->graph_use);
+    std::vector<std::string> lines;
+    read_lines(path, lines);
+    bool put = false;
+    for (std::string &l : lines)
+        if (l.compare(0, sig.size() + 3, sig + " : ") == 0) {
+            l = sig + tail;
+            put = true;
+        }
+    if (!put) lines.push_back(sig + tail);
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE *f = fopen(tmp.c_str(), "w");
+    if (!f) PSR_FAIL(c, PSR_ERR_ARG, "psr_tuning_save: cannot write " + tmp);
+    bool ok = true;
+    for (const std::string &l : lines) ok = ok && fprintf(f, "%s\n", l.c_str()) > 0;
+    if (fclose(f) != 0) ok = false;
+    if (!ok || rename(tmp.c_str(), path) != 0) PSR_FAIL(c, PSR_ERR_ARG, std::string("psr_tuning_save: cannot write ") + path);
+    return PSR_OK;
+}
+
+static int tune_load(psr_ctx *c, const char *path) {
+    std::vector<std::string> lines;
+    if (!read_lines(path, lines)) PSR_FAIL(c, PSR_ERR_ARG, std::string("psr_tuning_load: cannot read ") + path);
+    const std::string key = tune_signature(c) + " : ";
+    for (const std::string &l : lines) {
+        if (l.compare(0, key.size(), key) != 0) continue;
+        int dep = -1, graph = -1;
+        if (sscanf(l.c_str() + key.size(), "dep=%d graph=%d", &dep, &graph) != 2)
+            PSR_FAIL(c, PSR_ERR_ARG, "psr_tuning_load: malformed line for this workload");
+        // a choice applies only where the requests still leave it open
+        int cand[8];
+        const int nc = dep_candidates(c, cand);
+        for (int i = 0; i < nc && nc > 1; i++)
+            if (cand[i] == dep) c->dep_use = dep;
+        if (c->graph_req < 0 && (graph == 0 || graph == 1)) c->graph_use = graph;
+        return PSR_OK;
+    }
+    PSR_FAIL(c, PSR_ERR_STATE, std::string("psr_tuning_load: no entry for this workload in ") + path);
+}
+
+// PSR_TUNING_FILE: choices of earlier runs applied at this workload's first
+// tick, and every new decision written back
+static void tune_file_lookup(psr_ctx *c) {
+    if (c->tune_file_tried) return;
+    c->tune_file_tried = true;
+    const char *p = getenv("PSR_TUNING_FILE");
+    if (p && *p) {
+        const std::string keep = c->err;
+        if (tune_load(c, p) != PSR_OK) c->err = keep;
+    }
+}
+static void tune_file_store(psr_ctx *c) {
+    const char *p = getenv("PSR_TUNING_FILE");
+    if (p && *p) {
+        const std::string keep = c->err;
+        if (tune_save(c, p) != PSR_OK) c->err = keep;
+    }
+}
+
+extern "C" int psr_tuning_save(psr_ctx *c, const char *path) {
+    if (!c || !path) return PSR_ERR_ARG;
+    if (!c->have_tree || !c->have_physics) PSR_FAIL(c, PSR_ERR_STATE, "psr_tuning_save needs a tree and physics");
+    return tune_save(c, path);
+}
+
+extern "C" int psr_tuning_load(psr_ctx *c, const char *path) {
+    if (!c || !path) return PSR_ERR_ARG;
+    if (!c->have_tree || !c->have_physics) PSR_FAIL(c, PSR_ERR_STATE, "psr_tuning_load needs a tree and physics");
+    if (int rc = psr_resolve(c)) return rc;
+    c->tune_file_tried = true;
+    return tune_load(c, path);
+}
 
 // once every trial has run: keep the mode whose fastest trial deposit was
 // fastest.  A trial on an asynchronous tick that overflowed (its kernels
@@ -729,6 +831,7 @@ static int tune_decide(psr_ctx *c, bool block) {
         if (pick < 0 || best[cand[i]] < best[pick]) pick = cand[i];
     }
     c->dep_use = pick;
+    tune_file_store(c);
     return PSR_OK;
 }
 
@@ -782,6 +885,7 @@ static int gtune_decide(psr_ctx *c, bool block) {
         return PSR_OK;
     }
     c->graph_use = best[1] < best[0] ? 1 : 0;
+    tune_file_store(c);
     return PSR_OK;
 }
 
@@ -994,6 +1098,7 @@ extern "C" int psr_tick(psr_ctx *c, int64_t ray_first, int64_t nray, int64_t tic
     }
     if (!c->d_tick) PSR_HIP(c, hipMalloc(&c->d_tick, sizeof(DevTick)));
     if ((rc = poll_pending(c))) return rc;
+    tune_file_lookup(c);
     if ((rc = tune_decide(c, false))) return rc;
     if ((rc = gtune_decide(c, false))) return rc;
     if (c->pending.size() >= PSR_RING - 1 && (rc = psr_resolve(c))) return rc;

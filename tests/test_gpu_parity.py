@@ -737,3 +737,64 @@ def test_void_rate_table_steps_on_the_device(ctx):
         assert abs(o["lambdaHeII"][0] - 0.25) < 1e-14
         assert o["ie"][1] == pytest.approx(ie0 * 1.37, rel=1e-14)
         assert o["lambdaH"][1] == 1.0 and o["lambdaHeI"][1] == 1.0
+
+
+def _tick_until_decided(c, wl, rays, limit=40):
+    c.rays_upload(rays)
+    for t in range(1, limit + 1):
+        c.tick(t, wl.tick_time)
+        c.sync()
+        if (c.tuning(psr.PSR_TUNE_DEPOSIT_WALK) != psr.PSR_WALK_AUTO
+                and c.tuning(psr.PSR_TUNE_TICK_GRAPH) in (0, 1)):
+            return t
+    raise AssertionError("the autotuner did not decide")
+
+
+def test_tuning_persistence(tmp_path, monkeypatch):
+    """decided launch choices saved by one context and loaded by another of
+    the same workload (no trial ticks: the first tick already runs the
+    choice); another workload finds no entry; a tree rebuild restarts the
+    tuning; PSR_TUNING_FILE does the same across contexts by itself"""
+    print_case("tuning persistence")
+    wl = make_workload(2, RATES, n=24, nray=256)
+    rays = wl.rays(1)
+    f = tmp_path / "tuning.txt"
+    with psr.Context(0) as a:
+        load(a, wl)
+        _tick_until_decided(a, wl, rays)
+        choice = [a.tuning(p) for p in (psr.PSR_TUNE_DEPOSIT_ACCUM, psr.PSR_TUNE_DEPOSIT_WALK,
+                                        psr.PSR_TUNE_TICK_GRAPH)]
+        a.tuning_save(f)
+        a.tuning_save(f)                       # the workload's line is replaced, not repeated
+    lines = f.read_text().splitlines()
+    assert len(lines) == 1 and "npar=%d" % wl.ps.npar in lines[0] and "dep=" in lines[0]
+    with psr.Context(0) as b:
+        load(b, wl)
+        b.tuning_load(f)
+        assert [b.tuning(p) for p in (psr.PSR_TUNE_DEPOSIT_ACCUM, psr.PSR_TUNE_DEPOSIT_WALK,
+                                      psr.PSR_TUNE_TICK_GRAPH)] == choice
+        b.rays_upload(rays)
+        b.tick(1, wl.tick_time)
+        b.sync()
+        if choice[2] == 1:
+            assert b.tuning(psr.PSR_TUNE_GRAPH_LAUNCHES) == 1
+        load(b, wl)                            # a rebuilt tree restarts the trials
+        assert b.tuning(psr.PSR_TUNE_DEPOSIT_WALK) == psr.PSR_WALK_AUTO
+    other = make_workload(2, RATES, n=16, nray=64)
+    with psr.Context(0) as c:
+        load(c, other)
+        with pytest.raises(psr.PsrError):
+            c.tuning_load(f)
+    env = tmp_path / "env.txt"
+    monkeypatch.setenv("PSR_TUNING_FILE", str(env))
+    with psr.Context(0) as d:
+        load(d, wl)
+        _tick_until_decided(d, wl, rays)
+        want = d.tuning(psr.PSR_TUNE_DEPOSIT_WALK)
+    assert env.exists() and len(env.read_text().splitlines()) == 1
+    with psr.Context(0) as e:
+        load(e, wl)
+        e.rays_upload(rays)
+        e.tick(1, wl.tick_time)
+        e.sync()
+        assert e.tuning(psr.PSR_TUNE_DEPOSIT_WALK) == want
