@@ -347,15 +347,20 @@ __global__ void k_gather_keys(const uint32_t *__restrict__ idx,
     if (i < n) kout[i] = kin[idx[i]];
 }
 
-// tree-order particle records: ps = (x, y, z, sml), orig = index | invalid
+// tree-order particle records: ps = (x, y, z, sml), orig = index | invalid.
+// An invalid particle's sml is stored negated (its sign bit is the flag, so
+// the traversal loads one 16-byte record per particle); w * w and |w| are
+// unchanged
 __global__ void k_particles(const uint32_t *__restrict__ idx, const float *__restrict__ pos,
                             const float *__restrict__ sml, const int8_t *__restrict__ flag,
                             float4 *__restrict__ ps, uint32_t *__restrict__ orig, int64_t n) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t j = idx[i];
-    ps[i] = make_float4(pos[3 * (size_t)j], pos[3 * (size_t)j + 1], pos[3 * (size_t)j + 2], sml[j]);
-    orig[i] = j | ((flag && (flag[j] & 1)) ? PSR_INVALID_BIT : 0u);
+    const bool invalid = flag && (flag[j] & 1);
+    const float h = fabsf(sml[j]);
+    ps[i] = make_float4(pos[3 * (size_t)j], pos[3 * (size_t)j + 1], pos[3 * (size_t)j + 2], invalid ? -h : h);
+    orig[i] = j | (invalid ? PSR_INVALID_BIT : 0u);
 }
 
 // roots: one per field, range = valid particles of the field
@@ -443,8 +448,9 @@ __global__ void k_leaf_aabb(const float4 *__restrict__ ps, int32_t nn,
         for (int k = 0; k < cnt; k++) {
             const float4 p = ps[s + k];
             const float q[3] = {p.x, p.y, p.z};
+            const float h = fabsf(p.w);
             for (int d = 0; d < 3; d++) {
-                const float lo = q[d] - p.w, hi = q[d] + p.w;
+                const float lo = q[d] - h, hi = q[d] + h;
                 if (k == 0 || lo < o.bot[d]) o.bot[d] = lo;
                 if (k == 0 || hi > o.top[d]) o.top[d] = hi;
             }

@@ -186,7 +186,7 @@ __global__ void k_build_dview(DevState S, int64_t n, int he) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         if (he) {
             DepViewHe v;
-            v.sml = S.ps[i].w;
+            v.sml = fabsf(S.ps[i].w);
             v.mass = S.mass[i];
             v.xHI = S.lH[i];
             v.xHeI = S.lHeI[i];
@@ -194,7 +194,7 @@ __global__ void k_build_dview(DevState S, int64_t n, int he) {
             ((DepViewHe *)S.dv)[i] = v;
         } else {
             DepViewH v;
-            v.sml = S.ps[i].w;
+            v.sml = fabsf(S.ps[i].w);
             v.mass = S.mass[i];
             v.xHI = S.lH[i];
             ((DepViewH *)S.dv)[i] = v;

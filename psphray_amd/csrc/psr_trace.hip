@@ -384,10 +384,9 @@ __global__ __launch_bounds__(TR_THREADS, 8) void k_trace(
                 const int2 L = t < nl ? leafq[lq - nl + t] : make_int2(0, 0);
                 const int k = L.x + slot;
                 if (slot < L.y) {
-                    // the flag and the particle are loaded together and the
-                    // test is computed for both (its result is dropped for an
-                    // invalid particle), so one memory round trip serves both
-                    const uint32_t o = orig[k];
+                    // one 16-byte record per particle: an invalid particle's
+                    // sml is stored negated (its sign bit is the flag); the
+                    // test is computed for it too and its result dropped
                     const float4 p = ps[k];
                     // impact test of src/raytrace.c:171-186
                     const float dx = p.x - s[0], dy = p.y - s[1], dz = p.z - s[2];
@@ -402,7 +401,7 @@ __global__ __launch_bounds__(TR_THREADS, 8) void k_trace(
                     const double dist = sqrt(d2);
                     const double r2 = (dist - proj) * (dist + proj);
                     const float hh = p.w * p.w;
-                    const bool valid = !(o & PSR_INVALID_BIT);
+                    const bool valid = !(__float_as_uint(p.w) >> 31);
                     const bool inside = !((double)hh < r2);
                     if (valid & inside) {  // not &&: no branch between the two loads
                         const int h = atomicAdd(&s_nh, 1);
