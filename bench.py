@@ -185,6 +185,9 @@ def main():
     ap.add_argument("--walk", choices=["auto", "single", "multi", "multi2", "multi8"], default="auto",
                     help="deposit: one ray per 3-wave workgroup, or four (two, eight) rays per 4-wave "
                          "workgroup (auto: timed)")
+    ap.add_argument("--surrogate-scope", choices=["all", "stiff"], default="stiff",
+                    help="surrogate chemistry for every active particle, or only for the particles the "
+                         "exact solver's first attempt leaves unfinished")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mlp-forward-rows", type=int, default=0,
                     help="only time the surrogate network's forward kernel on this many random "
@@ -309,6 +312,7 @@ def main():
         wall = time.perf_counter() - t0
         return wall, ctx.stats(), ctx.tuning(psr.PSR_TUNE_GRAPH_LAUNCHES) - g0
 
+    smode = psr.PSR_CHEM_SURROGATE_STIFF if args.surrogate_scope == "stiff" else psr.PSR_CHEM_SURROGATE
     res = {}
     if exact_timed:
         wall, st, gl = timed(psr.PSR_CHEM_EXACT)
@@ -317,12 +321,12 @@ def main():
         res["exact"] = (wall, st, gl)
         log(f"exact: {K} ticks in {wall * 1e3:.2f} ms")
     if surrogate:
-        wall_s, st_s, gl_s = timed(psr.PSR_CHEM_SURROGATE)
+        wall_s, st_s, gl_s = timed(smode)
         sg = wl.ps.copy()
         ctx.state_download(sg)
         res["surrogate"] = (wall_s, st_s, gl_s)
         log(f"surrogate: {K} ticks in {wall_s * 1e3:.2f} ms")
-        sur.update({"ticks_per_s": K / wall_s, "ms_per_step": wall_s / K * 1e3,
+        sur.update({"scope": args.surrogate_scope, "ticks_per_s": K / wall_s, "ms_per_step": wall_s / K * 1e3,
                     "intersections_per_s": st_s["total_deposit_count"] / wall_s,
                     "accepted": int(st_s["surrogate_count"]),
                     "fallbacks": int(st_s["surrogate_fallbacks"]),
@@ -344,7 +348,7 @@ def main():
     ctx.state_upload(snap)
     ctx.reset_stats()
     ctx.set_timing(True)
-    run(K, psr.PSR_CHEM_EXACT if exact_timed else psr.PSR_CHEM_SURROGATE)
+    run(K, psr.PSR_CHEM_EXACT if exact_timed else smode)
     ctx.sync()
     tm = ctx.timers()
     st_t = ctx.stats()

@@ -141,9 +141,14 @@ typedef struct psr_stats {
 #define PSR_PHASE_DEPOSIT 2
 #define PSR_PHASE_UPDATE 4
 #define PSR_PHASE_ALL 7
-/* chemistry modes */
+/* chemistry modes: exact; the network for every active particle (the
+ * exact solver for the predictions it rejects); or the exact solver's first
+ * attempt for every active particle and the network only for the particles
+ * that attempt leaves unfinished (the stiff ones), the exact solver
+ * continuing for the predictions it rejects */
 #define PSR_CHEM_EXACT 0
 #define PSR_CHEM_SURROGATE 1
+#define PSR_CHEM_SURROGATE_STIFF 2
 
 /* ---- context ---- */
 int psr_create(psr_ctx **out, int device);

@@ -19,6 +19,7 @@ PSR_PHASE_UPDATE = 4
 PSR_PHASE_ALL = 7
 PSR_CHEM_EXACT = 0
 PSR_CHEM_SURROGATE = 1
+PSR_CHEM_SURROGATE_STIFF = 2
 PSR_MLP_IN = 22
 PSR_NOALLOC_MAX_STEPS = 32768
 PSR_LTE_NFREQ = 1024

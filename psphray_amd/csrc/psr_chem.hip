@@ -930,7 +930,8 @@ __global__ __launch_bounds__(256) void k_update_rest(DevState S, const uint32_t 
             base = __shfl(base, leader, 64);
             drained = base + (uint32_t)__popcll(idle) >= n;
             const uint32_t k = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1));
-            if (!have && k < n) {
+            // an entry the surrogate has stored since (active 0) is skipped
+            if (!have && k < n && S.active[list[k]] == 2) {
                 i = list[k];
                 double x0[7];
                 StepIn s;
@@ -1011,12 +1012,16 @@ __global__ __launch_bounds__(256) void k_solve(psr_step *__restrict__ steps, int
 //     at a bound the exact step clamps to, src/solve.c:208-213) and 10 tol
 //     relative for ie.
 // Accepted steps are stored as step_evolve's would be (src/loop.c:745-766);
-// the others keep active = 1 for the exact solver that runs next.
-template <bool ISO, bool ADI, bool HONLY, bool BALANCE>
+// the others keep their active mark for the exact solver that runs next.
+// LIST: only the particles the exact solver's first attempt left unfinished
+// (the phase-2 list, active = 2; those it rejects continue from their saved
+// integrator state); else every active particle (active = 1).
+template <bool ISO, bool ADI, bool HONLY, bool BALANCE, bool LIST>
 __global__ __launch_bounds__(256) void k_mlp_update(DevState S, int64_t npar, const DevTick *__restrict__ tk,
                                                     DevTables T, DevPhys P, DevStats *__restrict__ st,
                                                     const float *__restrict__ Pm, const float *__restrict__ norm,
-                                                    double tol) {
+                                                    double tol, const uint32_t *__restrict__ list,
+                                                    const uint32_t *__restrict__ nlist) {
     __shared__ float s_bias[3 * MLP_HID + MLP_OUT];
     __shared__ float s_feat[4][64][MLP_IN + 1];
     __shared__ float s_out[4][64][MLP_OUT];
@@ -1033,9 +1038,11 @@ __global__ __launch_bounds__(256) void k_mlp_update(DevState S, int64_t npar, co
     MlpW w;
     mlp_load_w(Pm, w, lane);
     int nacc = 0, nfb = 0;
-    for (int64_t base = ((int64_t)blockIdx.x * 4 + wv) * 64; base < npar; base += (int64_t)gridDim.x * 256) {
-        const int64_t i = base + lane;
-        const bool act = i < npar && S.active[i] == 1;
+    const int64_t nwork = LIST ? min((int64_t)nlist[0], npar) : npar;
+    for (int64_t base = ((int64_t)blockIdx.x * 4 + wv) * 64; base < nwork; base += (int64_t)gridDim.x * 256) {
+        const int64_t q = base + lane;
+        const int64_t i = LIST ? (q < nwork ? (int64_t)list[q] : npar) : q;
+        const bool act = i < npar && S.active[i] == (LIST ? 2 : 1);
         if (!__ballot(act)) continue;
         double x0[7];
         StepIn s;
@@ -1160,6 +1167,34 @@ int psr_prepare_update(psr_ctx *c) {
     return PSR_OK;
 }
 
+// phase 1 (k_update: the closed form, or one exact attempt per active
+// particle, listing the unfinished) or phase 2 (k_update_rest over the list);
+// no training capture
+int psr_launch_update_phase(psr_ctx *c, int phase) {
+    if (c->npar == 0) return PSR_OK;
+    const DevPhys P = psr_dev_phys(c);
+    const DevState S = c->state();
+    const int64_t n = c->npar;
+    const DevTick *tk = c->d_tick;
+    int rc = psr_prepare_update(c);
+    if (rc) return rc;
+    const bool closed = c->F.isothermal && c->F.on_the_spot && c->F.h_only;
+    unsigned grid;
+    if (phase == 1) {
+        // the list starts empty (the closed form lists nothing)
+        k_zero_u32x2<<<1, 64, 0, c->stream>>>(c->d_q);
+        PSR_HIP(c, hipGetLastError());
+        grid = (unsigned)((n + 255) / 256);
+        CHEM_DISPATCH(k_update, S, n, tk, c->T, P, c->d_stats, c->d_saved, c->d_qlist, c->d_q);
+    } else {
+        if (closed) return PSR_OK;
+        grid = (unsigned)std::max(1, c->cu_count * 8);
+        CHEM_DISPATCH(k_update_rest, S, c->d_qlist, c->d_q, tk, c->T, P, c->d_stats, c->d_saved);
+    }
+    PSR_HIP(c, hipGetLastError());
+    return PSR_OK;
+}
+
 int psr_launch_update(psr_ctx *c, bool timed) {
     if (c->npar == 0) return PSR_OK;
     const DevPhys P = psr_dev_phys(c);
@@ -1204,39 +1239,68 @@ int psr_launch_update(psr_ctx *c, bool timed) {
 // particles whose prediction was rejected (and for every particle when the
 // run is not on-the-spot, whose recombination counters the network does not
 // predict)
-int psr_mlp_update_dev(psr_ctx *c) {
-    if (!psr_mlp_ready(c)) PSR_FAIL(c, PSR_ERR_STATE, "surrogate chemistry: no trained network (psr_mlp_train)");
-    if (c->npar == 0) return PSR_OK;
-    if (!c->F.on_the_spot) return psr_launch_update(c);
+static int mlp_launch(psr_ctx *c, bool list) {
     const DevPhys P = psr_dev_phys(c);
     const DevState S = c->state();
     const MlpDev M = psr_mlp_dev(c);
     const int64_t n = c->npar;
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, c->cu_count * 4));
-    PSR_EV(c, 3, 0);
     const int key = (c->F.isothermal ? 4 : 0) | (c->F.adiabatic ? 2 : 0) | (c->F.h_only ? 1 : 0);
-#define MLP_K(I, A, H)                                                                                      \
+#define MLP_K(I, A, H, L)                                                                                   \
     do {                                                                                                    \
         if (c->mlp_balance)                                                                                 \
-            k_mlp_update<I, A, H, true><<<grid, 256, 0, c->stream>>>(S, n, c->d_tick, c->T, P,             \
-                                                                     c->d_stats, M.params, M.norm,         \
-                                                                     c->mlp_residual);                     \
+            k_mlp_update<I, A, H, true, L><<<grid, 256, 0, c->stream>>>(S, n, c->d_tick, c->T, P,          \
+                                                                        c->d_stats, M.params, M.norm,      \
+                                                                        c->mlp_residual, c->d_qlist, c->d_q); \
         else                                                                                                \
-            k_mlp_update<I, A, H, false><<<grid, 256, 0, c->stream>>>(S, n, c->d_tick, c->T, P,            \
-                                                                      c->d_stats, M.params, M.norm,        \
-                                                                      c->mlp_residual);                    \
+            k_mlp_update<I, A, H, false, L><<<grid, 256, 0, c->stream>>>(S, n, c->d_tick, c->T, P,         \
+                                                                         c->d_stats, M.params, M.norm,     \
+                                                                         c->mlp_residual, c->d_qlist, c->d_q); \
+    } while (0)
+#define MLP_KL(I, A, H)              \
+    do {                             \
+        if (list) MLP_K(I, A, H, true);  \
+        else MLP_K(I, A, H, false);      \
     } while (0)
     switch (key) {
-    case 0: MLP_K(false, false, false); break;
-    case 1: MLP_K(false, false, true); break;
-    case 2: MLP_K(false, true, false); break;
-    case 3: MLP_K(false, true, true); break;
-    case 4: case 6: MLP_K(true, false, false); break;
-    default: MLP_K(true, false, true); break;
+    case 0: MLP_KL(false, false, false); break;
+    case 1: MLP_KL(false, false, true); break;
+    case 2: MLP_KL(false, true, false); break;
+    case 3: MLP_KL(false, true, true); break;
+    case 4: case 6: MLP_KL(true, false, false); break;
+    default: MLP_KL(true, false, true); break;
     }
+#undef MLP_KL
 #undef MLP_K
     PSR_HIP(c, hipGetLastError());
-    int rc = psr_launch_update(c, false);
+    return PSR_OK;
+}
+
+// surrogate chemistry (on-the-spot runs; otherwise the exact solver, since
+// the network does not predict the recombination counters).  stiff: the
+// exact solver's first attempt runs for every active particle, the network
+// for the ones it leaves unfinished, the exact solver again for those the
+// network's checks reject; else the network for every active particle and
+// the exact solver for the rejected
+int psr_mlp_update_dev(psr_ctx *c, bool stiff) {
+    if (!psr_mlp_ready(c)) PSR_FAIL(c, PSR_ERR_STATE, "surrogate chemistry: no trained network (psr_mlp_train)");
+    if (c->npar == 0) return PSR_OK;
+    if (!c->F.on_the_spot) return psr_launch_update(c);
+    PSR_EV(c, 3, 0);
+    int rc;
+    if (stiff) {
+        // the closed form (isothermal hydrogen on the spot) finishes every
+        // particle in phase 1: nothing is left for the network
+        const bool closed = c->F.isothermal && c->F.on_the_spot && c->F.h_only;
+        if ((rc = psr_launch_update_phase(c, 1))) return rc;
+        if (!closed) {
+            if ((rc = mlp_launch(c, true))) return rc;
+            rc = psr_launch_update_phase(c, 2);
+        }
+    } else {
+        if ((rc = mlp_launch(c, false))) return rc;
+        rc = psr_launch_update(c, false);
+    }
     PSR_EV(c, 3, 1);
     return rc;
 }

@@ -325,6 +325,7 @@ void psr_tune_reset(psr_ctx *c);
 // timed: record the update phase's event pair (the surrogate records its own);
 // the tick and its length come from c->d_tick
 int psr_launch_update(psr_ctx *c, bool timed = true);
+int psr_launch_update_phase(psr_ctx *c, int phase);
 // allocations of the deposit / update phases ahead of a graph recording
 int psr_prepare_deposit(psr_ctx *c, int mode);
 int psr_prepare_update(psr_ctx *c);
@@ -352,7 +353,7 @@ hipError_t psr_exclusive_sum_i32(const int32_t *in, int32_t *out, int64_t n, voi
 DevPhys psr_dev_phys(const psr_ctx *c);
 int psr_sync_dev_ctx(psr_ctx *c);
 int psr_ensure_scratch(psr_ctx *c, size_t bytes);
-int psr_mlp_update_dev(psr_ctx *c);
+int psr_mlp_update_dev(psr_ctx *c, bool stiff);
 // the surrogate's device pointers: parameters, normalisation, and the
 // capture buffer of exact steps (cap null when capture is off)
 struct MlpDev {

@@ -924,7 +924,7 @@ static int tick_body(psr_ctx *c, const PendingTick &t, bool sync, bool set = tru
         }
     }
     if (t.phases & PSR_PHASE_UPDATE) {
-        if (t.chem_mode == PSR_CHEM_SURROGATE) rc = psr_mlp_update_dev(c);
+        if (t.chem_mode != PSR_CHEM_EXACT) rc = psr_mlp_update_dev(c, t.chem_mode == PSR_CHEM_SURROGATE_STIFF);
         else rc = psr_launch_update(c);
         if (rc) return rc;
     } else if (c->npar > 0) {
@@ -1088,7 +1088,7 @@ extern "C" int psr_tick(psr_ctx *c, int64_t ray_first, int64_t nray, int64_t tic
     if (rc) return rc;
     if (!c->have_physics) PSR_FAIL(c, PSR_ERR_STATE, "no physics: call psr_set_physics");
     if (!(phases & PSR_PHASE_TRACE)) PSR_FAIL(c, PSR_ERR_ARG, "a tick starts with the trace phase");
-    if (chem_mode != PSR_CHEM_EXACT && chem_mode != PSR_CHEM_SURROGATE)
+    if (chem_mode != PSR_CHEM_EXACT && chem_mode != PSR_CHEM_SURROGATE && chem_mode != PSR_CHEM_SURROGATE_STIFF)
         PSR_FAIL(c, PSR_ERR_ARG, "unknown chemistry mode");
     PSR_HIP(c, hipSetDevice(c->device));
     if (!c->h_ring) {

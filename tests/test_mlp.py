@@ -281,3 +281,63 @@ def test_surrogate_tick_off_the_spot_runs_the_exact_solver():
         va, vb = getattr(a, f).astype(np.float64), getattr(b, f).astype(np.float64)
         d = np.abs(va - vb)
         assert np.median(d) < 1e-9 and (d <= 1e-3 * np.maximum(np.abs(va), np.abs(vb)) + 1e-6).all(), f
+
+
+@pytest.mark.gpu
+def test_surrogate_stiff_scope_against_exact_solver():
+    """PSR_CHEM_SURROGATE_STIFF on an H + He field with temperature: the
+    network sees only the particles the exact solver's first attempt leaves
+    unfinished (its accepted and rejected counts add up to them; a rejected
+    one continues from the saved integrator state), and the ticks agree with
+    exact ticks from the same state within 1e-3 in x_HII; on the closed-form
+    hydrogen field nothing is left for the network"""
+    print_case("surrogate-stiff")
+    wl = make_workload(3, RATES, n=16, nray=256)
+    T, K = 24, 3
+    ticks = [wl.rays(t) for t in range(1, T + K + 1)]
+    counts = np.array([len(r) for r in ticks])
+    firsts = np.concatenate([[0], np.cumsum(counts)])
+    pool = np.concatenate(ticks)
+    with psr.Context(0) as c:
+        c.set_physics(wl.physics)
+        c.load(wl.ps, wl.field_offsets, wl.boxsizes, boxsize=wl.boxsize)
+        c.rays_upload(pool)
+        c.mlp_capture(2_000_000)
+        _ticks(c, wl, firsts, counts, 1, T, psr.PSR_CHEM_EXACT)
+        n, _, _ = c.mlp_captured()
+        assert n > 20000
+        c.mlp_train(n=0, epochs=30, batch=4096, lr=2e-3, seed=5)
+        c.mlp_capture(0)
+        snap = wl.ps.copy()
+        c.state_download(snap)
+        _ticks(c, wl, firsts, counts, T + 1, T + K, psr.PSR_CHEM_EXACT)
+        ex = wl.ps.copy()
+        c.state_download(ex)
+        c.state_upload(snap)
+        c.reset_stats()
+        _ticks(c, wl, firsts, counts, T + 1, T + K, psr.PSR_CHEM_SURROGATE_STIFF)
+        sg = wl.ps.copy()
+        c.state_download(sg)
+        st = c.stats()
+    acc, fb = st["surrogate_count"], st["surrogate_fallbacks"]
+    print("stiff: surrogate", acc, "fallbacks", fb, "active", st["active_count"])
+    assert acc > 0 and acc + fb < st["active_count"]
+    dx = np.abs(sg.lambdaH - ex.lambdaH)
+    assert dx.max() < 1e-3
+    # the closed form finishes everything in phase 1
+    wl2 = make_workload(2, RATES, n=12, nray=64)
+    with psr.Context(0) as c:
+        c.set_physics(wl2.physics)
+        c.load(wl2.ps, wl2.field_offsets, wl2.boxsizes, boxsize=wl2.boxsize)
+        rays = wl2.rays(1)
+        c.rays_upload(rays)
+        c.mlp_capture(100000)
+        c.tick(1, wl2.tick_time)
+        c.sync()
+        c.mlp_train(n=0, epochs=2, batch=256, lr=2e-3, seed=5)
+        c.mlp_capture(0)
+        c.reset_stats()
+        c.tick(2, wl2.tick_time, chem=psr.PSR_CHEM_SURROGATE_STIFF)
+        c.sync()
+        st = c.stats()
+    assert st["surrogate_count"] == 0 and st["surrogate_fallbacks"] == 0 and st["active_count"] > 0
