@@ -297,6 +297,15 @@ int psr_state_download(psr_ctx *ctx, psr_state *s);
  * r[] array, src/loop.c:69) */
 int psr_rays_upload(psr_ctx *ctx, const psr_ray *rays, int64_t nray);
 
+/* the output blocks the ticks change, computed on the device and gathered
+ * into the caller's order (psystem_write_output, src/psystem.c:644-667):
+ * x_HI, x_HeI, x_HeII, ye (electrons per hydrogen from H and He,
+ * lambda_to_ye) and gammaHI (yGrecHII) as the snapshot's floats, ie,
+ * lasthit, hits; npar entries each, NULL skips a block.  32 bytes per
+ * particle cross PCIe instead of the whole state. */
+int psr_output_gather(psr_ctx *ctx, float *xHI, float *xHeI, float *xHeII, float *ye, float *gammaHI, float *ie,
+                      int64_t *lasthit, int32_t *hits);
+
 /* ---- ray emission on the device (emit_rays, src/loop.c:163-373) ----
  * psr_sources_upload: point sources (Source, src/psystem.h:7-24): positions
  * [nsrc][3] (internal length), Ngamma_dot (photons per internal time),

@@ -287,6 +287,18 @@ class Context:
         self._nray_pool = max(getattr(self, "_nray_pool", 0), int(pool_first) + n.value)
         return n.value
 
+    def output_gather(self, npar):
+        """the output blocks the ticks change, gathered on the device
+        (psr_output_gather): dict of xHI, xHeI, xHeII, ye, gammaHI, ie
+        (float32), lasthit (int64), hits (int32)"""
+        out = {k: np.zeros(npar, np.float32) for k in ("xHI", "xHeI", "xHeII", "ye", "gammaHI", "ie")}
+        out["lasthit"] = np.zeros(npar, np.int64)
+        out["hits"] = np.zeros(npar, np.int32)
+        self._check(self._lib.psr_output_gather(self.h, *[out[k].ctypes.data for k in
+                                                          ("xHI", "xHeI", "xHeII", "ye", "gammaHI", "ie",
+                                                           "lasthit", "hits")]), "psr_output_gather")
+        return out
+
     def tuning_save(self, path):
         """the decided launch choices into a file (psr_tuning_save)"""
         self._check(self._lib.psr_tuning_save(self.h, str(path).encode()), "psr_tuning_save")

@@ -162,6 +162,7 @@ SIGNATURES = {
     "psr_sources_lastemit": (C.c_int, [_VP, _VP]),
     "psr_rays_download": (C.c_int, [_VP, _I64, _I64, _VP]),
     "psr_set_recombination": (C.c_int, [_VP, _I64, _D, C.c_int32]),
+    "psr_output_gather": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "psr_tuning_save": (C.c_int, [_VP, C.c_char_p]),
     "psr_tuning_load": (C.c_int, [_VP, C.c_char_p]),
     "psr_lte_tables": (C.c_int, [C.POINTER(Tables), C.POINTER(Units), _VP, _VP, _VP]),

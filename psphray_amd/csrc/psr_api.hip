@@ -847,6 +847,51 @@ extern "C" int psr_state_download(psr_ctx *c, psr_state *s) {
     return PSR_OK;
 }
 
+// the blocks of an output snapshot the ticks change, as
+// psystem_write_output derives them (src/psystem.c:644-667): the fractions
+// and ye (lambda_to_ye) as floats, yGrecHII as gammaHI, ie, lasthit, hits,
+// each written in the caller's order into one of eight columns of out
+// (4-byte columns of n entries, lasthit an 8-byte column)
+__global__ void k_output(DevState S, DevPhys P, const uint32_t *__restrict__ perm, int64_t n, float *__restrict__ f32,
+                         int64_t *__restrict__ lasthit, int32_t *__restrict__ hits) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t j = perm[i];
+    const double lH = S.lH[i], lHeI = S.lHeI[i], lHeII = S.lHeII[i];
+    f32[0 * n + j] = (float)lH;
+    f32[1 * n + j] = (float)lHeI;
+    f32[2 * n + j] = (float)lHeII;
+    f32[3 * n + j] = (float)lambda_ye(P, lH, lHeI, lHeII);
+    f32[4 * n + j] = S.yGrecHII[i];
+    f32[5 * n + j] = S.ie[i];
+    lasthit[j] = S.lasthit[i];
+    hits[j] = S.hits[i];
+}
+
+extern "C" int psr_output_gather(psr_ctx *c, float *xHI, float *xHeI, float *xHeII, float *ye, float *gammaHI,
+                                 float *ie, int64_t *lasthit, int32_t *hits) {
+    if (!c) return PSR_ERR_ARG;
+    if (!c->have_state) PSR_FAIL(c, PSR_ERR_STATE, "no particle state");
+    PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc = psr_resolve(c)) return rc;
+    const int64_t n = c->npar;
+    if (n == 0) return PSR_OK;
+    // six float columns, then lasthit, then hits
+    if (psr_ensure_scratch(c, (sizeof(float) * 6 + sizeof(int64_t) + sizeof(int32_t)) * n + 64)) return PSR_ERR_HIP;
+    float *f32 = (float *)c->d_scratch;
+    int64_t *lh = (int64_t *)(((uintptr_t)(f32 + 6 * n) + 7) & ~(uintptr_t)7);
+    int32_t *ht = (int32_t *)(lh + n);
+    k_output<<<nblk(n), 256, 0, c->stream>>>(c->state(), psr_dev_phys(c), c->d_perm, n, f32, lh, ht);
+    PSR_HIP(c, hipGetLastError());
+    float *dst[6] = {xHI, xHeI, xHeII, ye, gammaHI, ie};
+    for (int k = 0; k < 6; k++)
+        if (dst[k]) PSR_HIP(c, hipMemcpyAsync(dst[k], f32 + k * n, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    if (lasthit) PSR_HIP(c, hipMemcpyAsync(lasthit, lh, sizeof(int64_t) * n, hipMemcpyDeviceToHost, c->stream));
+    if (hits) PSR_HIP(c, hipMemcpyAsync(hits, ht, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    PSR_HIP(c, hipStreamSynchronize(c->stream));
+    return PSR_OK;
+}
+
 // ------------------------------------------------------------ rays
 extern "C" int psr_rays_upload(psr_ctx *c, const psr_ray *rays, int64_t nray) {
     if (!c || nray < 0 || (nray > 0 && !rays)) return PSR_ERR_ARG;

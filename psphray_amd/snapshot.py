@@ -156,6 +156,34 @@ def write_output(path, ps, ids, units, boxsize, redshift=0.0, time=1.0, src_pos=
     write(path, hdr, blocks)
 
 
+def write_output_device(path, ctx, ps, ids, units, boxsize, redshift=0.0, time=1.0, src_pos=None, src_ngd=None):
+    """write_output with the blocks the ticks change gathered on the device
+    (psr_output_gather); positions, sml, rho and mass, which an epoch keeps,
+    from the host arrays ps"""
+    n = int(ps.npar)
+    g = ctx.output_gather(n)
+    view = ParticleSystem(0)
+    view.npar = n
+    for f in ("pos", "sml", "rho", "mass"):
+        setattr(view, f, getattr(ps, f)[:n])
+    ns = 0 if src_pos is None else len(src_ngd)
+    hdr = make_header(n, ns, boxsize=boxsize, redshift=redshift, time=time, h=units.C_H)
+    pos = np.zeros((n + ns, 3), np.float32)
+    pos[:n] = view.pos
+    if ns:
+        pos[n:] = np.asarray(src_pos, np.float64).reshape(-1, 3)
+    idb = np.zeros(n + ns, np.uint64)
+    idb[:n] = ids[:n]
+    mass = np.zeros(n + ns, np.float32)
+    mass[:n] = view.mass
+    blocks = {"pos": pos, "id": idb, "mass": mass, "sml": view.sml, "rho": view.rho, "xHI": g["xHI"],
+              "xHeI": g["xHeI"], "xHeII": g["xHeII"], "ye": g["ye"], "gammaHI": g["gammaHI"], "ie": g["ie"],
+              "lasthit": g["lasthit"], "hits": g["hits"]}
+    if ns:
+        blocks["ngammas"] = np.asarray(src_ngd, np.float64)
+    write(path, hdr, blocks)
+
+
 _SNAP_FIELDS = (("id", np.uint64), ("pos", np.float32), ("mass", np.float32), ("sml", np.float32),
                 ("rho", np.float32), ("ie", np.float32), ("xHI", np.float32), ("ye", np.float32))
 

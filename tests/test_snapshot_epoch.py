@@ -405,3 +405,37 @@ def test_ticks_across_epoch_switch():
     assert np.array_equal(ps.lasthit[:n1], ref.lasthit[:n1])
     assert (ps.lasthit[:n1] <= 2).all() and (ps.hits[:n1] > 0).any()
     assert np.abs(ps.lambdaH[:n1] - ref.lambdaH[:n1]).max() < 1e-6
+
+
+@pytest.mark.gpu
+def test_output_gather_and_device_writer(tmp_path):
+    """psr_output_gather after two H + He ticks: each block is the state's
+    value as psystem_write_output derives it (fractions and ye as floats,
+    yGrecHII as gammaHI, ie, lasthit, hits), and a snapshot written from
+    it is byte for byte the one written from the downloaded state"""
+    print_case("output gather")
+    from psphray_amd.fields import make_workload
+    wl = make_workload(3, RATES, n=16, nray=256)
+    u = wl.physics.units
+    ids = np.arange(wl.ps.npar, dtype=np.uint64) * 5 + 2
+    with psr.Context(0) as ctx:
+        ctx.set_physics(wl.physics)
+        ctx.load(wl.ps, boxsize=wl.boxsize)
+        for t in (1, 2):
+            rays = wl.rays(t)
+            ctx.rays_upload(rays)
+            ctx.tick(t, wl.tick_time, nray=len(rays))
+        g = ctx.output_gather(wl.ps.npar)
+        st = ctx.state_download(wl.ps.copy())
+        snap.write_output_device(tmp_path / "dev", ctx, wl.ps, ids, u, wl.boxsize, redshift=6.0,
+                                 src_pos=wl.emitter.pos, src_ngd=wl.emitter.ngd)
+    assert (st.hits > 0).any()
+    assert np.array_equal(g["xHI"], st.lambdaH.astype(np.float32))
+    assert np.array_equal(g["xHeI"], st.lambdaHeI.astype(np.float32))
+    assert np.array_equal(g["xHeII"], st.lambdaHeII.astype(np.float32))
+    assert np.array_equal(g["ye"], snap.ye_of(st, u).astype(np.float32))
+    assert np.array_equal(g["gammaHI"], st.yGrecHII) and np.array_equal(g["ie"], st.ie)
+    assert np.array_equal(g["lasthit"], st.lasthit) and np.array_equal(g["hits"], st.hits)
+    snap.write_output(tmp_path / "host", st, ids, u, wl.boxsize, redshift=6.0, src_pos=wl.emitter.pos,
+                      src_ngd=wl.emitter.ngd)
+    assert (tmp_path / "dev").read_bytes() == (tmp_path / "host").read_bytes()
