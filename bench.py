@@ -258,7 +258,8 @@ def main():
     def decided():
         return (ctx.tuning(psr.PSR_TUNE_DEPOSIT_ACCUM) != psr.PSR_ACCUM_AUTO
                 and ctx.tuning(psr.PSR_TUNE_DEPOSIT_WALK) != psr.PSR_WALK_AUTO
-                and ctx.tuning(psr.PSR_TUNE_TICK_GRAPH) != -1)
+                and ctx.tuning(psr.PSR_TUNE_TICK_GRAPH) != -1
+                and ctx.tuning(psr.PSR_TUNE_TRACE_THREADS) in (128, 256))
 
     run(args.warmup, psr.PSR_CHEM_EXACT)
     ctx.sync()
@@ -275,7 +276,9 @@ def main():
             psr.PSR_WALK_MULTI: "four rays per workgroup",
             psr.PSR_WALK_MULTI8: "eight rays per workgroup"}[ctx.tuning(psr.PSR_TUNE_DEPOSIT_WALK)]
     graph = {0: "kernel by kernel", 1: "graph"}[ctx.tuning(psr.PSR_TUNE_TICK_GRAPH)]
-    log(f"{warmup_ticks} warm-up ticks; deposit: {walk}, {accum} accumulation; ticks: {graph}")
+    ctx_trace_threads = int(ctx.tuning(psr.PSR_TUNE_TRACE_THREADS))
+    log(f"{warmup_ticks} warm-up ticks; trace: {ctx_trace_threads} threads per ray; deposit: {walk}, "
+        f"{accum} accumulation; ticks: {graph}")
 
     sur = None
     if surrogate:
@@ -396,6 +399,7 @@ def main():
         "solver_steps_per_update": st["solver_steps"] / max(st["active_count"], 1),
         "deposit_accumulation": accum,
         "deposit_walk": walk,
+        "trace_threads_per_ray": ctx_trace_threads,
         "tick_launch": graph,
         "graph_launches_timed": graph_launches,
         "surrogate": sur,

@@ -423,6 +423,10 @@ int psr_get_timers(psr_ctx *ctx, double *ms);
  * The values are the same bits either way (deposits within f32 summation
  * order).  The several-ray walks with PSR_ACCUM_BINNED are PSR_ERR_NOTSUP. */
 #define PSR_TUNE_DEPOSIT_WALK 4
+/* PSR_TUNE_TRACE_THREADS: threads per ray of the traversal, 256 (32 nodes
+ * or leaves per step, a 4096-entry stack) or 128 (16, 2048); -1 (default):
+ * two trace launches of each on the first asynchronous ticks decide */
+#define PSR_TUNE_TRACE_THREADS 5
 #define PSR_WALK_AUTO 0
 #define PSR_WALK_SINGLE 1
 #define PSR_WALK_MULTI 2
@@ -431,7 +435,8 @@ int psr_get_timers(psr_ctx *ctx, double *ms);
 /* the autotuner's decided choices kept between runs (no counterpart in the
  * reference): psr_tuning_save writes them to a text file, one line per
  * workload signature (device arch and compute units, particle and field
- * counts, helium, isothermal, on-the-spot, adiabatic), replacing that
+ * counts, helium, isothermal, on-the-spot, adiabatic): the deposit mode,
+ * the tick launch and the trace's threads per ray; replacing that
  * workload's line; psr_tuning_load applies the line matching this
  * context's workload (PSR_ERR_STATE when there is none), so its ticks skip
  * the trials.  Call after psr_tree_build and psr_set_physics, which restart

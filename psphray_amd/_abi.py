@@ -32,6 +32,7 @@ PSR_ACCUM_BINNED = 2
 PSR_TUNE_TICK_GRAPH = 2
 PSR_TUNE_GRAPH_LAUNCHES = 3
 PSR_TUNE_DEPOSIT_WALK = 4
+PSR_TUNE_TRACE_THREADS = 5
 PSR_WALK_AUTO = 0
 PSR_WALK_SINGLE = 1
 PSR_WALK_MULTI = 2

@@ -188,6 +188,8 @@ extern "C" void psr_destroy(psr_ctx *c) {
         if (c->tune_ev[k]) (void)hipEventDestroy(c->tune_ev[k]);
     for (int k = 0; k < 2 * psr_ctx::TUNE_N; k++)
         if (c->gtune_ev[k]) (void)hipEventDestroy(c->gtune_ev[k]);
+    for (int k = 0; k < 2 * psr_ctx::TTUNE_N; k++)
+        if (c->ttune_ev[k]) (void)hipEventDestroy(c->ttune_ev[k]);
     psr_graph_reset(c);
     psr_emit_free(c);
     dfree(c->d_tick);

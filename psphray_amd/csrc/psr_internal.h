@@ -307,6 +307,15 @@ struct psr_ctx {
     int32_t tune_slot[DTUNE_N] = {};  // ring slot of the trial's tick; -1: a synchronous replay
     int32_t tune_mode[DTUNE_N] = {};
     int tune_n = 0;
+    // traversal threads per ray: requested (-1: timed trials, 128 or 256),
+    // chosen, and the trials (trace launches alternating 256 and 128)
+    int32_t trace_req = -1;
+    int32_t trace_use = -1;
+    static constexpr int TTUNE_N = 4;
+    hipEvent_t ttune_ev[2 * TTUNE_N] = {};
+    int32_t ttune_slot[TTUNE_N] = {};
+    int32_t ttune_nt[TTUNE_N] = {};
+    int ttune_n = 0;
     bool tune_file_tried = false;  // PSR_TUNING_FILE looked up for this workload
 
     DevState state() const;
@@ -315,7 +324,7 @@ struct psr_ctx {
 // ---- launchers implemented in the .hip files ----
 // sync: the trace waits for its capacity check (and retries); otherwise the
 // kernels read the halt flag and the tick is validated later
-int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray, bool sync, const DevTick *tk);
+int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray, bool sync, const DevTick *tk, int nt);
 int psr_launch_sort(psr_ctx *c, int64_t nray);
 // hit slots for nray rays of cap hits each (reallocates after a sync)
 int psr_ensure_hits(psr_ctx *c, int64_t nray, int32_t cap);

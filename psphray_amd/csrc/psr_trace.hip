@@ -21,9 +21,6 @@
 #include "psr_device.h"
 #include "psr_internal.h"
 
-#define TR_THREADS 256
-#define TR_SCAP 4096
-#define TR_LCAP 320
 #define TR_SORT 1024  // lists of up to this many hits are sorted by the trace
 // threads per ray of the k_sort tiers: their LDS allows 3 (4096 hits) and 1
 // (8192) workgroups per CU, so more threads per ray fill the CU (256^3 sort
@@ -329,8 +326,12 @@ __global__ void k_export_lists(int64_t nray, DevHits H, const uint32_t *__restri
 }
 
 
-// 8 waves per SIMD (64 VGPRs): with its sort the trace would take 66 and 7
-__global__ __launch_bounds__(TR_THREADS, 8) void k_trace(
+// NT threads per ray, 256 or 128 (the autotuner times both): a step
+// expands NT/8 nodes or tests NT/8 leaves; the LIFO stack holds 16 NT
+// entries.  256: 8 waves per SIMD (64 VGPRs; with its sort the trace would
+// take 66 and 7); 128: 6 (the sort's distances take 8 registers per thread)
+template <int NT>
+__global__ __launch_bounds__(NT, NT == 256 ? 8 : 6) void k_trace(
     const psr_ray *__restrict__ rays_arg, const DevTick *__restrict__ tk, int64_t nray,
     const Node *__restrict__ nodes, const int32_t *__restrict__ froot, const float4 *__restrict__ ps,
     const uint32_t *__restrict__ orig, DevHits H, uint8_t *__restrict__ active,
@@ -341,10 +342,10 @@ __global__ __launch_bounds__(TR_THREADS, 8) void k_trace(
     // list of up to TR_SORT hits
     union TraceLds {
         struct {
-            int32_t stack[TR_SCAP];
-            int2 leafq[TR_LCAP];  // leaves to test: (first particle, count)
+            int32_t stack[16 * NT];
+            int2 leafq[NT + NT / 4];  // leaves to test: (first particle, count)
         } t;
-        SortLds<TR_SORT, TR_SORT / 4, TR_THREADS> s;
+        SortLds<TR_SORT, TR_SORT / 4, NT> s;
     };
     __shared__ TraceLds U;
     int32_t *const stack = U.t.stack;
@@ -376,8 +377,8 @@ __global__ __launch_bounds__(TR_THREADS, 8) void k_trace(
             __syncthreads();
             const int sp = s_sp, lq = s_lq;
             if (sp == 0 && lq == 0) break;
-            if (lq >= 32 || sp == 0) {
-                const int nl = min(lq, 32);
+            if (lq >= NT / 8 || sp == 0) {
+                const int nl = min(lq, NT / 8);
                 const int t = tid >> 3, slot = tid & 7;
                 // the leaf's particle range was queued with it, so the
                 // particle loads need no node load first
@@ -416,7 +417,7 @@ __global__ __launch_bounds__(TR_THREADS, 8) void k_trace(
                 __syncthreads();
                 if (tid == 0) s_lq = lq - nl;
             } else {
-                const int nn = min(sp, 32);
+                const int nn = min(sp, NT / 8);
                 const int t = tid >> 3, c = tid & 7;
                 const int fc = t < nn ? stack[sp - nn + t] : -1;
                 __syncthreads();
@@ -426,7 +427,7 @@ __global__ __launch_bounds__(TR_THREADS, 8) void k_trace(
                     if (C.b != 0 && box_hit(s, dir, len, C)) {
                         if (C.b < 0) {
                             const int p = sp - nn + atomicAdd(&s_cs, 1);
-                            if (p < TR_SCAP) stack[p] = C.a;
+                            if (p < 16 * NT) stack[p] = C.a;
                             else s_ovf = 1;
                         } else {
                             const int p = lq + atomicAdd(&s_cl, 1);
@@ -436,7 +437,7 @@ __global__ __launch_bounds__(TR_THREADS, 8) void k_trace(
                 }
                 __syncthreads();
                 if (tid == 0) {
-                    s_sp = min(sp - nn + s_cs, TR_SCAP);
+                    s_sp = min(sp - nn + s_cs, 16 * NT);
                     s_lq = lq + s_cl;
                     s_cs = 0;
                     s_cl = 0;
@@ -451,7 +452,7 @@ __global__ __launch_bounds__(TR_THREADS, 8) void k_trace(
         }
         // a short list is sorted here, while it is in the cache, by the
         // threads that found it; longer ones by k_sort's tiers
-        if (nh > 0 && nh <= TR_SORT) sort_ray<TR_SORT, TR_SORT / 4, TR_THREADS>(nh, base, H, orig, U.s);
+        if (nh > 0 && nh <= TR_SORT) sort_ray<TR_SORT, TR_SORT / 4, NT>(nh, base, H, orig, U.s);
         __syncthreads();
     }
 }
@@ -502,7 +503,17 @@ static int clear_flags(psr_ctx *c) {
     return PSR_OK;
 }
 
-int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray, bool sync, const DevTick *tk) {
+static void launch_trace_kernel(psr_ctx *c, int nt, unsigned grid, int64_t ray_first, int64_t nray,
+                                const DevTick *tk, const int32_t *halt) {
+    if (nt == 128)
+        k_trace<128><<<grid, 128, 0, c->stream>>>(c->d_rays + ray_first, tk, nray, c->d_nodes, c->d_field_root,
+                                                  c->d_ps, c->d_orig, c->H, c->d_active, c->d_stats, halt);
+    else
+        k_trace<256><<<grid, 256, 0, c->stream>>>(c->d_rays + ray_first, tk, nray, c->d_nodes, c->d_field_root,
+                                                  c->d_ps, c->d_orig, c->H, c->d_active, c->d_stats, halt);
+}
+
+int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray, bool sync, const DevTick *tk, int nt) {
     int32_t cap0 = c->H.cap > 0 ? c->H.cap : 2048;
     int rc = psr_ensure_hits(c, nray, cap0);
     if (rc) return rc;
@@ -511,9 +522,7 @@ int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray, bool sync, con
         // the flag stays as the previous ticks left it: set only if one of
         // them overflowed, and then this tick is replayed as well
         PSR_EV(c, 0, 0);
-        k_trace<<<grid, TR_THREADS, 0, c->stream>>>(c->d_rays + ray_first, tk, nray, c->d_nodes,
-                                                    c->d_field_root, c->d_ps, c->d_orig, c->H,
-                                                    c->d_active, c->d_stats, &c->d_stats->flags);
+        launch_trace_kernel(c, nt, grid, ray_first, nray, tk, &c->d_stats->flags);
         PSR_HIP(c, hipGetLastError());
         PSR_EV(c, 0, 1);
         c->last_nray = nray;
@@ -523,9 +532,7 @@ int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray, bool sync, con
     for (int attempt = 0; attempt < 8; attempt++) {
         if ((rc = clear_flags(c))) return rc;
         PSR_EV(c, 0, 0);
-        k_trace<<<grid, TR_THREADS, 0, c->stream>>>(c->d_rays + ray_first, tk, nray, c->d_nodes,
-                                                    c->d_field_root, c->d_ps, c->d_orig, c->H,
-                                                    c->d_active, c->d_stats, nullptr);
+        launch_trace_kernel(c, nt, grid, ray_first, nray, tk, nullptr);
         PSR_HIP(c, hipGetLastError());
         PSR_EV(c, 0, 1);
         int32_t fl = 0;
@@ -628,7 +635,8 @@ extern "C" int psr_trace_lists(psr_ctx *c, int64_t ray_first, int64_t nray, int6
     if ((rc = psr_resolve(c))) return rc;
     offsets[0] = 0;
     if (nray == 0) return PSR_OK;
-    if ((rc = psr_launch_trace(c, ray_first, nray, true, nullptr))) return rc;
+    const int nt = c->trace_req > 0 ? c->trace_req : c->trace_use > 0 ? c->trace_use : 256;
+    if ((rc = psr_launch_trace(c, ray_first, nray, true, nullptr, nt))) return rc;
     // parity mode marks particles active like the tick does; clear them
     PSR_HIP(c, hipMemsetAsync(c->d_active, 0, c->npar > 0 ? c->npar : 1, c->stream));
     if ((rc = psr_launch_sort(c, nray))) return rc;
@@ -673,6 +681,8 @@ extern "C" int psr_trace_lists(psr_ctx *c, int64_t ray_first, int64_t nray, int6
 
 void psr_tune_reset(psr_ctx *c) {
     c->tune_n = 0;
+    c->ttune_n = 0;
+    c->trace_use = -1;
     c->tune_file_tried = false;
     c->dep_use = -1;
     c->gtune_n = 0;
@@ -724,7 +734,8 @@ static bool read_lines(const char *path, std::vector<std::string> &lines) {
 static int tune_save(psr_ctx *c, const char *path) {
     const std::string sig = tune_signature(c);
     char tail[64];
-    snprintf(tail, sizeof tail, " : dep=%d graph=%d", (int)c->dep_use, (int)c->graph_use);
+    snprintf(tail, sizeof tail, " : dep=%d graph=%d trace=%d", (int)c->dep_use, (int)c->graph_use,
+             (int)c->trace_use);
     std::vector<std::string> lines;
     read_lines(path, lines);
     bool put = false;
@@ -750,8 +761,8 @@ static int tune_load(psr_ctx *c, const char *path) {
     const std::string key = tune_signature(c) + " : ";
     for (const std::string &l : lines) {
         if (l.compare(0, key.size(), key) != 0) continue;
-        int dep = -1, graph = -1;
-        if (sscanf(l.c_str() + key.size(), "dep=%d graph=%d", &dep, &graph) != 2)
+        int dep = -1, graph = -1, trace = -1;
+        if (sscanf(l.c_str() + key.size(), "dep=%d graph=%d trace=%d", &dep, &graph, &trace) < 2)
             PSR_FAIL(c, PSR_ERR_ARG, "psr_tuning_load: malformed line for this workload");
         // a choice applies only where the requests still leave it open
         int cand[8];
@@ -759,6 +770,7 @@ static int tune_load(psr_ctx *c, const char *path) {
         for (int i = 0; i < nc && nc > 1; i++)
             if (cand[i] == dep) c->dep_use = dep;
         if (c->graph_req < 0 && (graph == 0 || graph == 1)) c->graph_use = graph;
+        if (c->trace_req < 0 && (trace == 128 || trace == 256)) c->trace_use = trace;
         return PSR_OK;
     }
     PSR_FAIL(c, PSR_ERR_STATE, std::string("psr_tuning_load: no entry for this workload in ") + path);
@@ -795,6 +807,47 @@ extern "C" int psr_tuning_load(psr_ctx *c, const char *path) {
     if (int rc = psr_resolve(c)) return rc;
     c->tune_file_tried = true;
     return tune_load(c, path);
+}
+
+// threads per ray of this tick's trace: the requested or decided one, else
+// the next trial (256 and 128 alternating); *trial its index or -1
+static int trace_nt(const psr_ctx *c, int *trial) {
+    *trial = -1;
+    if (c->trace_req > 0) return c->trace_req;
+    if (c->trace_use > 0) return c->trace_use;
+    if (c->ttune_n < psr_ctx::TTUNE_N) {
+        *trial = c->ttune_n;
+        return (c->ttune_n & 1) ? 128 : 256;
+    }
+    return 256;
+}
+
+// the trace trials: the faster shape's best launch wins (an overflowed
+// asynchronous tick's trial does not count)
+static int ttune_decide(psr_ctx *c, bool block) {
+    if (c->trace_req > 0 || c->trace_use > 0 || c->ttune_n < psr_ctx::TTUNE_N) return PSR_OK;
+    const hipEvent_t last = c->ttune_ev[2 * psr_ctx::TTUNE_N - 1];
+    if (block) PSR_HIP(c, hipEventSynchronize(last));
+    else if (hipEventQuery(last) != hipSuccess) return PSR_OK;
+    float best[2] = {INFINITY, INFINITY};
+    for (int k = 0; k < psr_ctx::TTUNE_N; k++) {
+        if (c->ttune_slot[k] >= 0) {
+            const int32_t v = ((volatile int32_t *)c->h_ring)[c->ttune_slot[k]];
+            if (v == 0 && !block) return PSR_OK;
+            if (v == 0 || (v & PSR_DF_HALT)) continue;
+        }
+        float ms = 0;
+        PSR_HIP(c, hipEventElapsedTime(&ms, c->ttune_ev[2 * k], c->ttune_ev[2 * k + 1]));
+        const int b = c->ttune_nt[k] == 128 ? 1 : 0;
+        best[b] = std::min(best[b], ms);
+    }
+    if (!(best[0] < INFINITY && best[1] < INFINITY)) {
+        c->ttune_n = 0;
+        return PSR_OK;
+    }
+    c->trace_use = best[1] < best[0] ? 128 : 256;
+    tune_file_store(c);
+    return PSR_OK;
 }
 
 // once every trial has run: keep the mode whose fastest trial deposit was
@@ -900,7 +953,20 @@ static int tick_body(psr_ctx *c, const PendingTick &t, bool sync, bool set = tru
         PSR_HIP(c, hipGetLastError());
     }
     if (t.nray > 0) {
-        if ((rc = psr_launch_trace(c, t.ray_first, t.nray, sync, c->d_tick))) return rc;
+        int ttrial;
+        const int nt = trace_nt(c, &ttrial);
+        if (ttrial >= 0) {
+            if (!c->ttune_ev[0])
+                for (int k = 0; k < 2 * psr_ctx::TTUNE_N; k++) PSR_HIP(c, hipEventCreate(&c->ttune_ev[k]));
+            PSR_HIP(c, hipEventRecord(c->ttune_ev[2 * ttrial], c->stream));
+        }
+        if ((rc = psr_launch_trace(c, t.ray_first, t.nray, sync, c->d_tick, nt))) return rc;
+        if (ttrial >= 0) {
+            PSR_HIP(c, hipEventRecord(c->ttune_ev[2 * ttrial + 1], c->stream));
+            c->ttune_slot[ttrial] = sync ? -1 : (int32_t)t.slot;
+            c->ttune_nt[ttrial] = nt;
+            c->ttune_n++;
+        }
         if (t.phases & PSR_PHASE_DEPOSIT) {
             if ((rc = psr_launch_sort(c, t.nray))) return rc;
             int trial;
@@ -958,12 +1024,16 @@ static bool graph_usable(const psr_ctx *c, const PendingTick &t) {
     int trial;
     (void)tick_mode(c, &trial);
     if (trial >= 0) return false;
+    (void)trace_nt(c, &trial);
+    if (trial >= 0) return false;
     return c->H.cap > 0 && c->H.cap <= 8192;
 }
 
 static std::vector<int64_t> graph_key(const psr_ctx *c, const PendingTick &t, int mode) {
     const MlpDev M = psr_mlp_dev(c);
-    return {t.nray,
+    int ttrial;
+    return {(int64_t)trace_nt(c, &ttrial),
+            t.nray,
             t.phases,
             t.chem_mode,
             (int64_t)mode,
@@ -1043,11 +1113,13 @@ static int tick_graph(psr_ctx *c, const PendingTick &t, bool *done, int *trial) 
 int psr_resolve(psr_ctx *c) {
     // no tick pending: every trial tick was published, its events are done
     if (c->pending.empty()) {
-        const int rc = tune_decide(c, false);
+        int rc = tune_decide(c, false);
+        if (!rc) rc = ttune_decide(c, false);
         return rc ? rc : gtune_decide(c, false);
     }
     PSR_HIP(c, hipStreamSynchronize(c->stream));
     int rc0 = tune_decide(c, true);
+    if (!rc0) rc0 = ttune_decide(c, true);
     if (!rc0) rc0 = gtune_decide(c, true);
     if (rc0) return rc0;
     std::deque<PendingTick> todo;
@@ -1099,6 +1171,7 @@ extern "C" int psr_tick(psr_ctx *c, int64_t ray_first, int64_t nray, int64_t tic
     if ((rc = poll_pending(c))) return rc;
     tune_file_lookup(c);
     if ((rc = tune_decide(c, false))) return rc;
+    if ((rc = ttune_decide(c, false))) return rc;
     if ((rc = gtune_decide(c, false))) return rc;
     if (c->pending.size() >= PSR_RING - 1 && (rc = psr_resolve(c))) return rc;
     PendingTick t{ray_first, nray, tick, tick_time, phases, chem_mode, c->seq++ % PSR_RING};
@@ -1158,6 +1231,12 @@ extern "C" int psr_set_tuning(psr_ctx *c, int32_t param, int32_t value) {
             PSR_FAIL(c, PSR_ERR_NOTSUP, "several rays per workgroup accumulate with atomics only");
         c->walk_req = value;
         psr_tune_reset(c);
+    } else if (param == PSR_TUNE_TRACE_THREADS) {
+        if (value != -1 && value != 128 && value != 256)
+            PSR_FAIL(c, PSR_ERR_ARG, "trace threads per ray: -1 (timed trials), 128 or 256");
+        c->trace_req = value;
+        c->ttune_n = 0;
+        c->trace_use = -1;
     } else if (param == PSR_TUNE_TICK_GRAPH) {
         if (value < -1 || value > 1) PSR_FAIL(c, PSR_ERR_ARG, "tick graph: -1 (timed trials), 0 or 1");
         c->graph_req = value;
@@ -1187,6 +1266,7 @@ extern "C" int psr_get_tuning(psr_ctx *c, int32_t param, int32_t *value) {
                  : mode == PSR_DEP_MULTI8      ? PSR_WALK_MULTI8
                                               : PSR_WALK_SINGLE;
     else if (param == PSR_TUNE_TICK_GRAPH) *value = c->graph_req >= 0 ? c->graph_req : c->graph_use;
+    else if (param == PSR_TUNE_TRACE_THREADS) *value = c->trace_req > 0 ? c->trace_req : c->trace_use;
     else if (param == PSR_TUNE_GRAPH_LAUNCHES) *value = (int32_t)std::min<int64_t>(c->graph_launches, INT32_MAX);
     else PSR_FAIL(c, PSR_ERR_ARG, "unknown tuning parameter");
     return PSR_OK;

@@ -111,9 +111,17 @@ def test_tree_matches_oracle(ctx, cfg, n, nray):
     assert depth == eo["depth"].max() and skipped == 0
 
 
+@pytest.fixture(params=[256, 128], ids=["nt256", "nt128"])
+def trace_threads(ctx, request):
+    """the traversal's threads per ray, fixed for the test, timed afterwards"""
+    ctx.set_tuning(psr.PSR_TUNE_TRACE_THREADS, request.param)
+    yield request.param
+    ctx.set_tuning(psr.PSR_TUNE_TRACE_THREADS, -1)
+
+
 @pytest.mark.parametrize("cfg,n,nray", CASES)
-def test_trace_lists_bit_exact(ctx, cfg, n, nray):
-    print_case("trace", cfg, n, nray)
+def test_trace_lists_bit_exact(ctx, trace_threads, cfg, n, nray):
+    print_case("trace", cfg, n, nray, trace_threads)
     wl = make_workload(cfg, RATES, n=n, nray=nray)
     load(ctx, wl)
     rays = wl.rays(1)
@@ -175,8 +183,8 @@ def test_lists_longer_than_the_lds_sort(ctx):
     assert np.abs(out.lambdaH - ref.lambdaH).max() < 1e-6
 
 
-def test_trace_batched_fields(ctx):
-    print_case("trace batched")
+def test_trace_batched_fields(ctx, trace_threads):
+    print_case("trace batched", trace_threads)
     wl = make_workload(5, RATES, nfield=6, nray=64)
     load(ctx, wl)
     rays = wl.rays(1)
@@ -547,6 +555,7 @@ def test_tick_graph_matches_kernel_by_kernel(cfg, accum, walk):
             c2.set_tuning(psr.PSR_TUNE_DEPOSIT_ACCUM, accum)
             c2.set_tuning(psr.PSR_TUNE_DEPOSIT_WALK, walk)
             c2.set_tuning(psr.PSR_TUNE_TICK_GRAPH, graph)
+            c2.set_tuning(psr.PSR_TUNE_TRACE_THREADS, 256 if cfg == 2 else 128)
             load(c2, wl)
             c2.rays_upload(pool)
             for t in range(1, 11):
@@ -745,7 +754,8 @@ def _tick_until_decided(c, wl, rays, limit=40):
         c.tick(t, wl.tick_time)
         c.sync()
         if (c.tuning(psr.PSR_TUNE_DEPOSIT_WALK) != psr.PSR_WALK_AUTO
-                and c.tuning(psr.PSR_TUNE_TICK_GRAPH) in (0, 1)):
+                and c.tuning(psr.PSR_TUNE_TICK_GRAPH) in (0, 1)
+                and c.tuning(psr.PSR_TUNE_TRACE_THREADS) in (128, 256)):
             return t
     raise AssertionError("the autotuner did not decide")
 
@@ -763,7 +773,7 @@ def test_tuning_persistence(tmp_path, monkeypatch):
         load(a, wl)
         _tick_until_decided(a, wl, rays)
         choice = [a.tuning(p) for p in (psr.PSR_TUNE_DEPOSIT_ACCUM, psr.PSR_TUNE_DEPOSIT_WALK,
-                                        psr.PSR_TUNE_TICK_GRAPH)]
+                                        psr.PSR_TUNE_TICK_GRAPH, psr.PSR_TUNE_TRACE_THREADS)]
         a.tuning_save(f)
         a.tuning_save(f)                       # the workload's line is replaced, not repeated
     lines = f.read_text().splitlines()
@@ -772,7 +782,7 @@ def test_tuning_persistence(tmp_path, monkeypatch):
         load(b, wl)
         b.tuning_load(f)
         assert [b.tuning(p) for p in (psr.PSR_TUNE_DEPOSIT_ACCUM, psr.PSR_TUNE_DEPOSIT_WALK,
-                                      psr.PSR_TUNE_TICK_GRAPH)] == choice
+                                      psr.PSR_TUNE_TICK_GRAPH, psr.PSR_TUNE_TRACE_THREADS)] == choice
         b.rays_upload(rays)
         b.tick(1, wl.tick_time)
         b.sync()
