@@ -101,23 +101,17 @@ struct StepIn {
     double yGdepHI, yGdepHeI, yGdepHeII, heat, time, nH, T, yeMET;
 };
 
-// RHS of src/solve.c:76-141, scaled by the step time
+// f_i / time of src/solve.c:76-141 from the fractions, ye and the rates, in
+// the reference's expression order (rhs and rhs_jac share it, so their f
+// values are the same bits)
 template <bool ISO, bool ADI, bool OTS, bool HONLY>
-__device__ void rhs(const double x[7], double f[7], const StepIn &s, const DevTables &T,
-                    const DevPhys &P, const Rates &Riso) {
+__device__ __forceinline__ void rhs_rows(const double x[7], double ye, const Rates &R, const StepIn &s,
+                                         const DevPhys &P, double f[7]) {
     const double xHI = x[0];
     const double xHII = 1.0 - x[0];
     const double xHeI = x[1];
     const double xHeII = x[2];
     const double xHeIII = 1.0 - x[2] - x[1];
-    const double ye = (1.0 - x[0]) + P.C_HEMF / P.C_HMF * 0.25 * (x[2] + 2 * (1.0 - x[2] - x[1])) + s.yeMET;
-    Rates R;
-    if (ISO) {
-        R = Riso;
-    } else {
-        const double Tk = P.U_MPROTON_OVER_C_BOLTZMANN * x[6] / (P.C_HEMF * 0.25 + P.C_HMF + P.C_HMF * ye) * 2.0 / 3.0;
-        rates_at<OTS>(T, log10(Tk), s.nH, R);
-    }
     if (!OTS) {
         f[0] = -s.yGdepHI / s.time + (-R.gamma_HI * ye * xHI + R.alpha_HII_A * ye * xHII);
         f[3] = R.alpha_HII_AB * ye * xHII;
@@ -149,109 +143,27 @@ __device__ void rhs(const double x[7], double f[7], const StepIn &s, const DevTa
                                            R.beta * ye * xHII + R.chi * ye);
         f[6] = s.heat / s.time - L;
     }
-#pragma unroll
-    for (int i = 0; i < 7; i++) f[i] *= s.time;
 }
 
-// ---- right-hand side with its Jacobian, by forward-mode dual numbers ----
-// Every quantity carries its value and its derivatives with respect to the
-// N evolving variables.  The rates are the table's piecewise-linear
-// interpolation in log10 T, so d rate / d log10 T is the segment's slope,
-// and log10 T depends on ie and (through ye) on the fractions.  The value
-// parts are computed with the operations and order of rhs(), so f is the
-// same; the Jacobian replaces N finite-difference evaluations (each one a
-// full pass over the rate table) with one.
-template <int N>
-struct Dn {
-    double v;
-    double d[N];
-};
-template <int N>
-__device__ __forceinline__ Dn<N> dn_const(double v) {
-    Dn<N> r;
-    r.v = v;
+__device__ __forceinline__ double ye_of(const double x[7], const StepIn &s, const DevPhys &P) {
+    return (1.0 - x[0]) + P.C_HEMF / P.C_HMF * 0.25 * (x[2] + 2 * (1.0 - x[2] - x[1])) + s.yeMET;
+}
+
+// RHS of src/solve.c:76-141, scaled by the step time
+template <bool ISO, bool ADI, bool OTS, bool HONLY>
+__device__ void rhs(const double x[7], double f[7], const StepIn &s, const DevTables &T,
+                    const DevPhys &P, const Rates &Riso) {
+    const double ye = ye_of(x, s, P);
+    Rates R;
+    if (ISO) {
+        R = Riso;
+    } else {
+        const double Tk = P.U_MPROTON_OVER_C_BOLTZMANN * x[6] / (P.C_HEMF * 0.25 + P.C_HMF + P.C_HMF * ye) * 2.0 / 3.0;
+        rates_at<OTS>(T, log10(Tk), s.nH, R);
+    }
+    rhs_rows<ISO, ADI, OTS, HONLY>(x, ye, R, s, P, f);
 #pragma unroll
-    for (int k = 0; k < N; k++) r.d[k] = 0.0;
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dn<N> operator+(const Dn<N> &a, const Dn<N> &b) {
-    Dn<N> r;
-    r.v = a.v + b.v;
-#pragma unroll
-    for (int k = 0; k < N; k++) r.d[k] = a.d[k] + b.d[k];
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dn<N> operator-(const Dn<N> &a, const Dn<N> &b) {
-    Dn<N> r;
-    r.v = a.v - b.v;
-#pragma unroll
-    for (int k = 0; k < N; k++) r.d[k] = a.d[k] - b.d[k];
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dn<N> operator*(const Dn<N> &a, const Dn<N> &b) {
-    Dn<N> r;
-    r.v = a.v * b.v;
-#pragma unroll
-    for (int k = 0; k < N; k++) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dn<N> operator-(const Dn<N> &a) {
-    Dn<N> r;
-    r.v = -a.v;
-#pragma unroll
-    for (int k = 0; k < N; k++) r.d[k] = -a.d[k];
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dn<N> operator+(double a, const Dn<N> &b) {
-    Dn<N> r = b;
-    r.v = a + b.v;
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dn<N> operator+(const Dn<N> &a, double b) {
-    Dn<N> r = a;
-    r.v = a.v + b;
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dn<N> operator-(double a, const Dn<N> &b) {
-    Dn<N> r;
-    r.v = a - b.v;
-#pragma unroll
-    for (int k = 0; k < N; k++) r.d[k] = -b.d[k];
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dn<N> operator-(const Dn<N> &a, double b) {
-    Dn<N> r = a;
-    r.v = a.v - b;
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dn<N> operator*(double a, const Dn<N> &b) {
-    Dn<N> r;
-    r.v = a * b.v;
-#pragma unroll
-    for (int k = 0; k < N; k++) r.d[k] = a * b.d[k];
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dn<N> operator*(const Dn<N> &a, double b) {
-    Dn<N> r;
-    r.v = a.v * b;
-#pragma unroll
-    for (int k = 0; k < N; k++) r.d[k] = a.d[k] * b;
-    return r;
-}
-// fdim(a, b) = a - b where a > b, else 0 (and so its derivatives)
-template <int N>
-__device__ __forceinline__ Dn<N> dn_fdim(const Dn<N> &a, const Dn<N> &b) {
-    return a.v > b.v ? a - b : dn_const<N>(fdim(a.v, b.v));
+    for (int i = 0; i < 7; i++) f[i] *= s.time;
 }
 
 // the rates at log10 T (as rates_at) and d rate / d log10 T
@@ -311,6 +223,95 @@ __device__ __forceinline__ void rates_slopes_at(const DevTables &T, double logT,
     S.beta = sl * nH;
     R.chi = col(AR_E_COMPC, sl);
     S.chi = sl;
+    // fdim(a, b) = a - b where a > b, else 0, and so its slope
+    R.alpha_HII_AB = fdim(R.alpha_HII_A, R.alpha_HII_B);
+    S.alpha_HII_AB = R.alpha_HII_A > R.alpha_HII_B ? S.alpha_HII_A - S.alpha_HII_B : 0.0;
+    R.alpha_HeII_AB = fdim(R.alpha_HeII_A, R.alpha_HeII_B);
+    S.alpha_HeII_AB = R.alpha_HeII_A > R.alpha_HeII_B ? S.alpha_HeII_A - S.alpha_HeII_B : 0.0;
+    R.alpha_HeIII_AB = fdim(R.alpha_HeIII_A, R.alpha_HeIII_B);
+    S.alpha_HeIII_AB = R.alpha_HeIII_A > R.alpha_HeIII_B ? S.alpha_HeIII_A - S.alpha_HeIII_B : 0.0;
+}
+
+// ---- the Jacobian of the right-hand side ----
+// Every term of row i carries one factor ye: f_i = c_i + ye p_i(x, r(log10 T)),
+// with the rates r piecewise linear in log10 T (slope s_r on the segment).
+// Only x_HI, x_HeI, x_HeII appear in ye and in the p_i, and log10 T depends
+// on ye and ie, so
+//     d f_i / d x_q = (d ye / d x_q) p_i
+//                     + ye (dp_i/d log10 T · d log10 T / d x_q + dp_i/dx_q |_T).
+// Per row that is the value p_i, its slope in log10 T and three partials at
+// fixed T: a few products instead of carrying every intermediate value's
+// derivatives through the right-hand side.
+struct RowD {
+    double p, ps, d0, d1, d2;  // p_i, d p_i / d log10 T, d p_i / d x_{0,1,2}
+};
+
+template <bool ISO, bool OTS>
+__device__ __forceinline__ RowD row_d(int i, const double x[7], const Rates &R, const Rates &S, double C) {
+    const double xHI = x[0], xHII = 1.0 - x[0], xHeI = x[1], xHeII = x[2], xHeIII = 1.0 - x[2] - x[1];
+    RowD r{0.0, 0.0, 0.0, 0.0, 0.0};
+    switch (i) {
+    case 0:
+        if (OTS) {
+            r.p = -R.gamma_HI * xHI + R.alpha_HII_B * xHII;
+            if (!ISO) r.ps = -S.gamma_HI * xHI + S.alpha_HII_B * xHII;
+            r.d0 = -R.gamma_HI - R.alpha_HII_B;
+        } else {
+            r.p = -R.gamma_HI * xHI + R.alpha_HII_A * xHII;
+            if (!ISO) r.ps = -S.gamma_HI * xHI + S.alpha_HII_A * xHII;
+            r.d0 = -R.gamma_HI - R.alpha_HII_A;
+        }
+        break;
+    case 1:
+        if (OTS) {
+            r.p = -R.gamma_HeI * xHI + R.alpha_HeII_B * xHeII;
+            if (!ISO) r.ps = -S.gamma_HeI * xHI + S.alpha_HeII_B * xHeII;
+            r.d0 = -R.gamma_HeI;
+            r.d2 = R.alpha_HeII_B;
+        } else {
+            r.p = -R.gamma_HeI * xHeI + R.alpha_HeII_A * xHeII;
+            if (!ISO) r.ps = -S.gamma_HeI * xHeI + S.alpha_HeII_A * xHeII;
+            r.d1 = -R.gamma_HeI;
+            r.d2 = R.alpha_HeII_A;
+        }
+        break;
+    case 2:
+        if (OTS) {
+            r.p = -R.gamma_HeII * xHI - R.alpha_HeII_B * xHeII + R.alpha_HeIII_B * xHeIII;
+            if (!ISO) r.ps = -S.gamma_HeII * xHI - S.alpha_HeII_B * xHeII + S.alpha_HeIII_B * xHeIII;
+            r.d0 = -R.gamma_HeII;
+            r.d1 = -R.alpha_HeIII_B;
+            r.d2 = -R.alpha_HeII_B - R.alpha_HeIII_B;
+        } else {
+            r.p = -R.gamma_HeII * xHeII - R.alpha_HeII_A * xHeII + R.alpha_HeIII_A * xHeIII;
+            if (!ISO) r.ps = -S.gamma_HeII * xHeII - S.alpha_HeII_A * xHeII + S.alpha_HeIII_A * xHeIII;
+            r.d1 = -R.alpha_HeIII_A;
+            r.d2 = -R.gamma_HeII - R.alpha_HeII_A - R.alpha_HeIII_A;
+        }
+        break;
+    case 3:
+        r.p = R.alpha_HII_AB * xHII;
+        if (!ISO) r.ps = S.alpha_HII_AB * xHII;
+        r.d0 = -R.alpha_HII_AB;
+        break;
+    case 4:
+        r.p = R.alpha_HeII_AB * xHeII;
+        if (!ISO) r.ps = S.alpha_HeII_AB * xHeII;
+        r.d2 = R.alpha_HeII_AB;
+        break;
+    case 5:
+        r.p = R.alpha_HeIII_AB * xHeIII;
+        if (!ISO) r.ps = S.alpha_HeIII_AB * xHeIII;
+        r.d1 = -R.alpha_HeIII_AB;
+        r.d2 = -R.alpha_HeIII_AB;
+        break;
+    default:  // 6: the cooling, f_6 = heat / time - C ye (...)
+        r.p = -C * ((R.zeta_HI + R.psi_HI) * xHI + (R.eta_HII + R.beta) * xHII + R.chi);
+        if (!ISO) r.ps = -C * ((S.zeta_HI + S.psi_HI) * xHI + (S.eta_HII + S.beta) * xHII + S.chi);
+        r.d0 = -C * ((R.zeta_HI + R.psi_HI) - (R.eta_HII + R.beta));
+        break;
+    }
+    return r;
 }
 
 // f = rhs(x) and J[i][j] = d f_act(i) / d x_act(j) over the evolving variables
@@ -319,87 +320,44 @@ __device__ void rhs_jac(const double x[7], double f[7], double *J, const StepIn 
                         const DevPhys &P, const Rates &Riso) {
     using SY = Sys<ISO, ADI, OTS, HONLY>;
     constexpr int N = SY::NV;
-    using D = Dn<N>;
-    D X[7];
-#pragma unroll
-    for (int q = 0; q < 7; q++) X[q] = dn_const<N>(x[q]);
-#pragma unroll
-    for (int jj = 0; jj < N; jj++) X[SY::act(jj)].d[jj] = 1.0;
-    const D xHI = X[0];
-    const D xHII = 1.0 - X[0];
-    const D xHeI = X[1];
-    const D xHeII = X[2];
-    const D xHeIII = 1.0 - X[2] - X[1];
-    const D ye = (1.0 - X[0]) + P.C_HEMF / P.C_HMF * 0.25 * (X[2] + 2 * (1.0 - X[2] - X[1])) + s.yeMET;
+    const double cHe = P.C_HEMF / P.C_HMF * 0.25;
+    const double ye = ye_of(x, s, P);
+    // d ye / d x_q for q = 0, 1, 2
+    const double dye[3] = {-1.0, -2.0 * cHe, -cHe};
     Rates R, Sl;
-    D L = dn_const<N>(0.0);  // log10 T
+    double dL[N];  // d log10 T / d x_act(j)
+#pragma unroll
+    for (int jj = 0; jj < N; jj++) dL[jj] = 0.0;
     if (ISO) {
         R = Riso;
     } else {
-        const double den = P.C_HEMF * 0.25 + P.C_HMF + P.C_HMF * ye.v;
+        const double den = P.C_HEMF * 0.25 + P.C_HMF + P.C_HMF * ye;
         const double Tk = P.U_MPROTON_OVER_C_BOLTZMANN * x[6] / den * 2.0 / 3.0;
-        L.v = log10(Tk);
-        rates_slopes_at<OTS>(T, L.v, s.nH, R, Sl);
+        rates_slopes_at<OTS>(T, log10(Tk), s.nH, R, Sl);
         // log10 T = log10 ie - log10 den + const
-        const double dye = -P.C_HMF * 0.43429448190325182765 / den;
+        const double dyefac = -P.C_HMF * 0.43429448190325182765 / den;
 #pragma unroll
         for (int jj = 0; jj < N; jj++) {
-            L.d[jj] = dye * ye.d[jj];
-            if (SY::act(jj) == 6) L.d[jj] += 0.43429448190325182765 / x[6];
+            const int q = SY::act(jj);
+            if (q < 3) dL[jj] = dyefac * dye[q];
+            if (q == 6) dL[jj] = 0.43429448190325182765 / x[6];
         }
     }
-    auto rate = [&](double r, double sl) -> D {
-        D o;
-        o.v = r;
+    rhs_rows<ISO, ADI, OTS, HONLY>(x, ye, R, s, P, f);
 #pragma unroll
-        for (int jj = 0; jj < N; jj++) o.d[jj] = ISO ? 0.0 : sl * L.d[jj];
-        return o;
-    };
-    const D gamma_HI = rate(R.gamma_HI, Sl.gamma_HI);
-    const D gamma_HeI = rate(R.gamma_HeI, Sl.gamma_HeI);
-    const D gamma_HeII = rate(R.gamma_HeII, Sl.gamma_HeII);
-    D g[7];
-    if (!OTS) {
-        const D aHA = rate(R.alpha_HII_A, Sl.alpha_HII_A), aHB = rate(R.alpha_HII_B, Sl.alpha_HII_B);
-        g[0] = -s.yGdepHI / s.time + (-gamma_HI * ye * xHI + aHA * ye * xHII);
-        g[3] = dn_fdim(aHA, aHB) * ye * xHII;
-        if (HONLY) {
-            g[1] = g[2] = g[4] = g[5] = dn_const<N>(0.0);
-        } else {
-            const D a2A = rate(R.alpha_HeII_A, Sl.alpha_HeII_A), a2B = rate(R.alpha_HeII_B, Sl.alpha_HeII_B);
-            const D a3A = rate(R.alpha_HeIII_A, Sl.alpha_HeIII_A), a3B = rate(R.alpha_HeIII_B, Sl.alpha_HeIII_B);
-            g[1] = -s.yGdepHeI / s.time + (-gamma_HeI * ye * xHeI + a2A * ye * xHeII);
-            g[2] = -s.yGdepHeII / s.time + (-gamma_HeII * ye * xHeII - a2A * ye * xHeII + a3A * ye * xHeIII);
-            g[4] = dn_fdim(a2A, a2B) * ye * xHeII;
-            g[5] = dn_fdim(a3A, a3B) * ye * xHeIII;
+    for (int i = 0; i < 7; i++) f[i] *= s.time;
+#pragma unroll
+    for (int ii = 0; ii < N; ii++) {
+        const RowD r = row_d<ISO, OTS>(SY::act(ii), x, R, Sl, P.C_H_PER_MASS);
+        const double yt = ye * s.time, pt = r.p * s.time, pst = r.ps * yt;
+#pragma unroll
+        for (int jj = 0; jj < N; jj++) {
+            const int q = SY::act(jj);
+            double v = ISO ? 0.0 : pst * dL[jj];
+            if (q < 3) v += dye[q] * pt + yt * (q == 0 ? r.d0 : q == 1 ? r.d1 : r.d2);
+            J[ii * N + jj] = v;
         }
-    } else {
-        const D aHB = rate(R.alpha_HII_B, Sl.alpha_HII_B);
-        g[0] = -s.yGdepHI / s.time + (-gamma_HI * ye * xHI + aHB * ye * xHII);
-        if (HONLY) {
-            g[1] = g[2] = dn_const<N>(0.0);
-        } else {
-            const D a2B = rate(R.alpha_HeII_B, Sl.alpha_HeII_B), a3B = rate(R.alpha_HeIII_B, Sl.alpha_HeIII_B);
-            // x_HI in the He lines, as src/solve.c:103-104
-            g[1] = -s.yGdepHeI / s.time + (-gamma_HeI * ye * xHI + a2B * ye * xHeII);
-            g[2] = -s.yGdepHeII / s.time + (-gamma_HeII * ye * xHI - a2B * ye * xHeII + a3B * ye * xHeIII);
-        }
-        g[3] = g[4] = g[5] = dn_const<N>(0.0);
     }
-    if (ADI || ISO) {
-        g[6] = dn_const<N>(0.0);
-    } else {
-        const D zeta = rate(R.zeta_HI, Sl.zeta_HI), psi = rate(R.psi_HI, Sl.psi_HI);
-        const D eta = rate(R.eta_HII, Sl.eta_HII), beta = rate(R.beta, Sl.beta), chi = rate(R.chi, Sl.chi);
-        const D Lc = P.C_H_PER_MASS * ((zeta + psi) * ye * xHI + (eta * ye * xHII) + beta * ye * xHII + chi * ye);
-        g[6] = s.heat / s.time - Lc;
-    }
-#pragma unroll
-    for (int i = 0; i < 7; i++) f[i] = g[i].v * s.time;
-#pragma unroll
-    for (int ii = 0; ii < N; ii++)
-#pragma unroll
-        for (int jj = 0; jj < N; jj++) J[ii * N + jj] = g[SY::act(ii)].d[jj] * s.time;
 }
 
 // ROS3 of Sandu, Verwer, Blom, Spee, Carmichael & Potra, Atmos. Environ. 31
@@ -661,22 +619,25 @@ struct Integ {
         return false;
     }
 
-    // state after an accepted step: the evolving variables, t, h, steps
+    // state between attempts: the evolving variables, t, h, the attempt
+    // count (negated after a rejected attempt: the next accepted step may
+    // then not grow h).  f and J are recomputed at x on restore, the same
+    // bits as the values the uninterrupted integration keeps.
     static constexpr int SAVE_WORDS = N + 3;
     __device__ void save(double *w) const {
 #pragma unroll
         for (int jj = 0; jj < N; jj++) w[jj] = x[S::act(jj)];
         w[N] = t;
         w[N + 1] = h;
-        w[N + 2] = (double)nsteps;
+        w[N + 2] = rejected ? -(double)nsteps : (double)nsteps;
     }
     __device__ void restore(const double *w) {
 #pragma unroll
         for (int jj = 0; jj < N; jj++) x[S::act(jj)] = w[jj];
         t = w[N];
         h = w[N + 1];
-        nsteps = (int)w[N + 2];
-        rejected = false;
+        nsteps = (int)fabs(w[N + 2]);
+        rejected = w[N + 2] < 0.0;
         need_fj = true;
         dead = false;
     }
@@ -830,16 +791,15 @@ __global__ __launch_bounds__(256) void k_cap_post(DevState S, int64_t npar, DevP
     r[MLP_IN + 3] = iso ? r[4] : mlp_lg(T1, 1e-30);
 }
 
-// one attempt in phase 1 (measured on 256^3 H + He with 1, 2, 4, 8: 6.49,
-// 6.51, 6.93, 9.37 ms per update phase): particles that need more go to the
-// lane-persistent phase 2, whose waves do not wait for their slowest lane
-#define PHASE1_ATTEMPTS 1
-
 // Chemistry of the active set, phase 1: one thread per particle in tree
-// order, at most PHASE1_ATTEMPTS step attempts (then up to the next accepted
-// step).  Most particles finish here in one or two steps; the others save
-// their integrator state and are marked active = 2 for phase 2.  The split
-// bounds the wavefront divergence: step counts range from 1 to over 100.
+// order, exactly one step attempt (the whole step, h = 1).  Most particles
+// finish with it; the others (attempt rejected, or accepted short of t = 1)
+// save their integrator state and are marked active = 2 for phase 2, whose
+// lane-persistent waves do not wait for their slowest lane.  With a single
+// attempt no lane of a phase-1 wavefront waits for another: attempt counts
+// range from 1 to over 100, and continuing to the first accepted step here
+// made each wavefront run as many attempts as its slowest lane (256^3 H + He:
+// phase 1 4.28 ms that way).
 template <bool ISO, bool ADI, bool OTS, bool HONLY>
 __global__ __launch_bounds__(256) void k_update(DevState S, int64_t npar, const DevTick *__restrict__ tk,
                                                 DevTables T, DevPhys P, DevStats *__restrict__ st,
@@ -867,12 +827,7 @@ __global__ __launch_bounds__(256) void k_update(DevState S, int64_t npar, const 
         load_step(S, (uint32_t)i, tick, tick_time, P, x0, s);
         G g;
         g.init(x0, s, T);
-        int att = 0;
-        bool fin;
-        while (!(fin = g.attempt(T, P, 1e-4))) {
-            if (++att >= PHASE1_ATTEMPTS && g.need_fj) break;
-        }
-        if (fin) {
+        if (g.attempt(T, P, 1e-4)) {
             g.clamp();
             steps = g.nsteps;
             fail = g.t < 1.0;
