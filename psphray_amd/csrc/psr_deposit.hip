@@ -82,7 +82,8 @@ struct __attribute__((aligned(16))) WalkRec {
 };
 template <>
 struct __attribute__((aligned(16))) WalkRec<true> {
-    double em, nhi, fhi, fhei, fheii, nhei, nheii, pad;
+    double em, nhi, fhi, fhei, fheii, nhei, nheii;
+    double tm;  // the several-ray walk stores the TM the hit saw here
 };
 
 // what apply needs besides the walk record
@@ -278,22 +279,27 @@ struct SecVals {
     double vH[3], vHe[3];
 };
 
+// x = ye NH / (NH + NHe), capped at 1 (0 without secondary ionization)
 template <bool HE>
-__device__ __forceinline__ SecVals hit_secion(double xHI, double xHeI, double xHeII, float massf,
-                                              const DevTables &T, const DevPhys &P, const RayConst &k) {
+__device__ __forceinline__ double secion_x(double xHI, double xHeI, double xHeII, float massf, const DevPhys &P) {
+    if (!P.secondary_ionization) return 0.0;
+    const double mass = (double)massf;
+    const double NH = mass * P.C_H_PER_MASS;
+    const double NHe = mass * P.C_HE_PER_MASS;
+    if (!HE) xHeI = xHeII = 0.0;
+    const double ye = (1.0 - xHI) + P.C_HEMF / P.C_HMF * 0.25 * (xHeII + 2 * (1.0 - xHeII - xHeI));
+    double x = ye * NH / (NH + NHe);
+    if (x > 1.0) x = 1.0;
+    return x;
+}
+
+template <bool HE>
+__device__ __forceinline__ SecVals secion_at(double x, const DevTables &T, const DevPhys &P, const RayConst &k) {
     SecVals sv;
-    sv.x = 0.0;
+    sv.x = x;
 #pragma unroll
     for (int t = 0; t < 3; t++) sv.vH[t] = sv.vHe[t] = 0.0;
     if (P.secondary_ionization) {
-        const double mass = (double)massf;
-        const double NH = mass * P.C_H_PER_MASS;
-        const double NHe = mass * P.C_HE_PER_MASS;
-        if (!HE) xHeI = xHeII = 0.0;
-        const double ye = (1.0 - xHI) + P.C_HEMF / P.C_HMF * 0.25 * (xHeII + 2 * (1.0 - xHeII - xHeI));
-        double x = ye * NH / (NH + NHe);
-        if (x > 1.0) x = 1.0;
-        sv.x = x;
         if (x > 0.0 && k.secion_terms) {
             const double log10x = log10(x);
             // phi_HI, phi_HeI, E_h at (dE_HI, log10 x) and at (dE_HeI, log10 x);
@@ -303,6 +309,12 @@ __device__ __forceinline__ SecVals hit_secion(double xHI, double xHeI, double xH
         }
     }
     return sv;
+}
+
+template <bool HE>
+__device__ __forceinline__ SecVals hit_secion(double xHI, double xHeI, double xHeII, float massf,
+                                              const DevTables &T, const DevPhys &P, const RayConst &k) {
+    return secion_at<HE>(secion_x<HE>(xHI, xHeI, xHeII, massf, P), T, P, k);
 }
 
 // src/loop.c:479-573 for one live hit that saw TM = tm, with the state-only
@@ -956,15 +968,32 @@ __global__ __launch_bounds__(ORD_THREADS) void k_ray_order(const int32_t *__rest
     }
 }
 
+// what the apply wavefronts read besides the walk record: for H + He the
+// secondary-ionization x (computed by the prep from the fractions, so the
+// record is 16 bytes instead of 32), for hydrogen x_HI
+template <bool HE>
+struct MAux {
+    uint32_t slot;
+    float mass;
+    double xHI;
+};
+template <>
+struct MAux<true> {
+    uint32_t slot;
+    float mass;
+    double x;
+};
+
 // a ray's row of records (and of TMs) is padded by one entry, so that the
 // MR walking lanes, which read the same record index of different rays, hit
-// different LDS banks
+// different LDS banks.  The H + He record carries the TM its hit saw (the
+// walk writes it); hydrogen's 16-byte record keeps a separate TM row.
 template <bool HE, int MR>
 struct MLds {
     static constexpr int MROW = MShape<MR>::ROW;
     WalkRec<HE> rec[3][MR * MROW];
-    HitAux<HE> aux[3][MHITS];
-    double tmh[3][MR * MROW];  // TM before each hit (walk -> apply)
+    MAux<HE> aux[3][MHITS];
+    double tmh[3][HE ? 1 : MR * MROW];  // TM before each hit (walk -> apply), hydrogen
     RayConst kc[MR];           // per-ray constants (cross sections, energies)
     int64_t ray[MR];           // ray index, -1 for an empty place in the batch
     int32_t n[MR];             // hits of the ray (0: nothing to deposit)
@@ -989,7 +1018,7 @@ __device__ __forceinline__ void mr_index(HitGather<HE> &g, const DevHits &H, con
 }
 
 template <bool HE, int MR>
-__global__ __launch_bounds__(MTHREADS, 3) void k_deposit_multi(const DevTick *__restrict__ tk, int64_t nray,
+__global__ __launch_bounds__(MTHREADS, 4) void k_deposit_multi(const DevTick *__restrict__ tk, int64_t nray,
                                                                DevHits H, const DevCtx *__restrict__ C,
                                                                DevStats *__restrict__ st0,
                                                                double *__restrict__ ray_len,
@@ -1071,7 +1100,12 @@ __global__ __launch_bounds__(MTHREADS, 3) void k_deposit_multi(const DevTick *__
                     prep_hit<HE>(g1[h], sd[h], P, kc, rr, au);
                     if (i * MSEG + k >= L.n[p]) noop_rec<HE>(rr, au);
                     L.rec[ring][p * MROW + k] = rr;
-                    L.aux[ring][j] = au;
+                    MAux<HE> ma;
+                    ma.slot = au.slot;
+                    ma.mass = au.mass;
+                    if constexpr (HE) ma.x = secion_x<HE>(au.xHI, au.xHeI, au.xHeII, au.mass, P);
+                    else ma.xHI = au.xHI;
+                    L.aux[ring][j] = ma;
                 }
 #pragma unroll
                 for (int h = 0; h < HPL; h++) {
@@ -1116,8 +1150,8 @@ __global__ __launch_bounds__(MTHREADS, 3) void k_deposit_multi(const DevTick *__
                     constexpr int WG = 4;
                     const int ring = s % 3;
                     const int kn = min(MSEG, L.n[p] - s * MSEG);
-                    const WalkRec<HE> *R = &L.rec[ring][p * MROW];
-                    double *TMH = &L.tmh[ring][p * MROW];
+                    WalkRec<HE> *R = &L.rec[ring][p * MROW];
+                    double *TMH = &L.tmh[ring][HE ? 0 : p * MROW];
                     WalkRec<HE> A[WG];
 #pragma unroll
                     for (int q = 0; q < WG; q++) A[q] = R[q];
@@ -1127,7 +1161,8 @@ __global__ __launch_bounds__(MTHREADS, 3) void k_deposit_multi(const DevTick *__
                         const int g1 = min(g + 1, MSEG / WG - 1);
 #pragma unroll
                         for (int q = 0; q < WG; q++) {
-                            TMH[g * WG + q] = t;
+                            if constexpr (HE) R[g * WG + q].tm = t;
+                            else TMH[g * WG + q] = t;
                             lv[q] = is_live(A[q].em);
                             t = walk_step<HE>(t, A[q]);
                             post[q] = t;
@@ -1171,22 +1206,22 @@ __global__ __launch_bounds__(MTHREADS, 3) void k_deposit_multi(const DevTick *__
                 const SecVals cur = nxt;
                 const int s1 = i - 1;
                 if (s1 >= 0 && L.n[p] > 0) {
-                    const HitAux<HE> &au = L.aux[s1 % 3][j];
-                    double xHeI = 0.0, xHeII = 0.0;
-                    if constexpr (HE) {
-                        xHeI = au.xHeI;
-                        xHeII = au.xHeII;
-                    }
-                    nxt = hit_secion<HE>(au.xHI, xHeI, xHeII, au.mass, T, P, kc);
+                    const MAux<HE> &au = L.aux[s1 % 3][j];
+                    double x;
+                    if constexpr (HE) x = au.x;
+                    else x = secion_x<HE>(au.xHI, 0.0, 0.0, au.mass, P);
+                    nxt = secion_at<HE>(x, T, P, kc);
                 }
                 const int s = i - 2;
                 if (s >= 0 && s <= L.endseg[p] && s * MSEG + k <= L.last[p]) {
                     const int ring = s % 3;
                     const WalkRec<HE> v = L.rec[ring][p * MROW + k];
                     if (is_live(v.em)) {
-                        const HitAux<HE> &au = L.aux[ring][j];
-                        const DepRec<HE> rr =
-                            apply_core<HE>(L.tmh[ring][p * MROW + k], v, au.slot, au.mass, cur, P, kc, acc);
+                        const MAux<HE> &au = L.aux[ring][j];
+                        double tm;
+                        if constexpr (HE) tm = v.tm;
+                        else tm = L.tmh[ring][p * MROW + k];
+                        const DepRec<HE> rr = apply_core<HE>(tm, v, au.slot, au.mass, cur, P, kc, acc);
                         rec_atomic<HE>(S, rr);
                     }
                 }
