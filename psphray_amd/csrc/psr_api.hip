@@ -139,6 +139,7 @@ static void free_tree(psr_ctx *c) {
     dfree(c->d_field_root);
     dfree(c->d_perm);
     dfree(c->d_ps);
+    dfree(c->d_psl);
     dfree(c->d_orig);
     c->have_tree = false;
 }
@@ -484,6 +485,19 @@ __global__ void k_internal_aabb(int32_t ls, int32_t n, const int32_t *__restrict
     nodes[node] = o;
 }
 
+// the trace's particle lines: slot s of leaf j is its particle s, or an
+// invalid record (sml -0: sign bit set) past its count
+__global__ void k_leaf_lines(const float4 *__restrict__ ps, const Node *__restrict__ nodes, int32_t nn,
+                             float4 *__restrict__ psl) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t j = t >> 3;
+    const int s = (int)(t & 7);
+    if (j >= nn) return;
+    const Node nd = nodes[j];
+    if (nd.b <= 0) return;
+    psl[t] = s < nd.b ? ps[nd.a + s] : make_float4(0.0f, 0.0f, 0.0f, -0.0f);
+}
+
 static inline unsigned nblk(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
 extern "C" int psr_tree_build(psr_ctx *c, const float *pos, const float *sml, const int8_t *flag,
@@ -686,6 +700,9 @@ extern "C" int psr_tree_build(psr_ctx *c, const float *pos, const float *sml, co
         k_internal_aabb<<<nblk(b - a), 256, 0, st>>>(a, b - a, nd_child, c->d_nodes);
         TB_HIP(hipGetLastError());
     }
+    TB_HIP(hipMalloc(&c->d_psl, sizeof(float4) * 8 * (size_t)nn));
+    k_leaf_lines<<<nblk(8 * (int64_t)nn), 256, 0, st>>>(c->d_ps, c->d_nodes, nn, c->d_psl);
+    TB_HIP(hipGetLastError());
     // skipped particles: invalid keys
     std::vector<int32_t> rootcnt(nfield);
     TB_HIP(hipMemcpyAsync(rootcnt.data(), nd_count, sizeof(int32_t) * nfield, hipMemcpyDeviceToHost, st));

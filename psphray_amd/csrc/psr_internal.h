@@ -215,6 +215,10 @@ struct psr_ctx {
     // particles
     bool have_state = false;
     float4 *d_ps = nullptr;
+    // the trace's copy of d_ps by leaf: leaf node j's particles at
+    // [8 j, 8 j + count), its other slots invalid, so a leaf is one aligned
+    // 128-byte line (slots of internal and empty nodes are never read)
+    float4 *d_psl = nullptr;
     uint32_t *d_orig = nullptr;
     float *d_f32 = nullptr;   // mass rho ie yeMET yGdep*3 yGrec*3 heat : 11 arrays
     double *d_f64 = nullptr;  // lambdaH lambdaHeI lambdaHeII

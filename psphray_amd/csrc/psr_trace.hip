@@ -333,7 +333,7 @@ __global__ void k_export_lists(int64_t nray, DevHits H, const uint32_t *__restri
 template <int NT>
 __global__ __launch_bounds__(NT, NT == 256 ? 8 : 6) void k_trace(
     const psr_ray *__restrict__ rays_arg, const DevTick *__restrict__ tk, int64_t nray,
-    const Node *__restrict__ nodes, const int32_t *__restrict__ froot, const float4 *__restrict__ ps,
+    const Node *__restrict__ nodes, const int32_t *__restrict__ froot, const float4 *__restrict__ psl,
     const uint32_t *__restrict__ orig, DevHits H, uint8_t *__restrict__ active,
     DevStats *__restrict__ st, const int32_t *halt) {
     // a tick's rays come through its tick record, parity tracing passes them
@@ -343,7 +343,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 8 : 6) void k_trace(
     union TraceLds {
         struct {
             int32_t stack[16 * NT];
-            int2 leafq[NT + NT / 4];  // leaves to test: (first particle, count)
+            int2 leafq[NT + NT / 4];  // leaves to test: (node, first particle)
         } t;
         SortLds<TR_SORT, TR_SORT / 4, NT> s;
     };
@@ -370,7 +370,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 8 : 6) void k_trace(
             const Node nd = nodes[root];
             if (nd.b != 0 && box_hit(s, dir, len, nd)) {
                 if (nd.b < 0) stack[s_sp++] = nd.a;
-                else leafq[s_lq++] = make_int2(nd.a, nd.b);
+                else leafq[s_lq++] = make_int2(root, nd.a);
             }
         }
         for (;;) {
@@ -382,13 +382,14 @@ __global__ __launch_bounds__(NT, NT == 256 ? 8 : 6) void k_trace(
                 const int t = tid >> 3, slot = tid & 7;
                 // the leaf's particle range was queued with it, so the
                 // particle loads need no node load first
-                const int2 L = t < nl ? leafq[lq - nl + t] : make_int2(0, 0);
-                const int k = L.x + slot;
-                if (slot < L.y) {
-                    // one 16-byte record per particle: an invalid particle's
-                    // sml is stored negated (its sign bit is the flag); the
-                    // test is computed for it too and its result dropped
-                    const float4 p = ps[k];
+                const int2 L = t < nl ? leafq[lq - nl + t] : make_int2(-1, 0);
+                const int k = L.y + slot;
+                if (L.x >= 0) {
+                    // the leaf's aligned line of eight 16-byte records: an
+                    // invalid particle's sml is stored negated (its sign bit
+                    // is the flag; slots past the leaf's count are invalid);
+                    // the test is computed for it too and its result dropped
+                    const float4 p = psl[8 * (int64_t)L.x + slot];
                     // impact test of src/raytrace.c:171-186
                     const float dx = p.x - s[0], dy = p.y - s[1], dz = p.z - s[2];
                     const float px = dx * dir[0], py = dy * dir[1], pz = dz * dir[2];
@@ -431,7 +432,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 8 : 6) void k_trace(
                             else s_ovf = 1;
                         } else {
                             const int p = lq + atomicAdd(&s_cl, 1);
-                            leafq[p] = make_int2(C.a, C.b);
+                            leafq[p] = make_int2(ch, C.a);
                         }
                     }
                 }
@@ -507,10 +508,10 @@ static void launch_trace_kernel(psr_ctx *c, int nt, unsigned grid, int64_t ray_f
                                 const DevTick *tk, const int32_t *halt) {
     if (nt == 128)
         k_trace<128><<<grid, 128, 0, c->stream>>>(c->d_rays + ray_first, tk, nray, c->d_nodes, c->d_field_root,
-                                                  c->d_ps, c->d_orig, c->H, c->d_active, c->d_stats, halt);
+                                                  c->d_psl, c->d_orig, c->H, c->d_active, c->d_stats, halt);
     else
         k_trace<256><<<grid, 256, 0, c->stream>>>(c->d_rays + ray_first, tk, nray, c->d_nodes, c->d_field_root,
-                                                  c->d_ps, c->d_orig, c->H, c->d_active, c->d_stats, halt);
+                                                  c->d_psl, c->d_orig, c->H, c->d_active, c->d_stats, halt);
 }
 
 int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray, bool sync, const DevTick *tk, int nt) {
