@@ -31,6 +31,15 @@ __device__ __forceinline__ const __attribute__((address_space(1))) T *gmem(const
     return (const __attribute__((address_space(1))) T *)p;
 }
 
+// relaxed device-scope add without return through a global-memory pointer:
+// a global (not flat) atomic, which counts on the vector-memory wait counter
+// only, so the issuing wave's LDS waits do not wait for it
+template <typename T>
+__device__ __forceinline__ void gadd(T *p, T v) {
+    __hip_atomic_fetch_add((__attribute__((address_space(1))) T *)p, v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // src/sph-depth.c:224-231: nearest-lower entry, 0 outside the table; the
 // load is unconditional (clamped index) so no branch hides it
 __device__ __forceinline__ double sph_depth(const DevTables &T, double r_h) {
