@@ -129,7 +129,7 @@ __device__ __forceinline__ void sort_ray(int n, int64_t base, const DevHits &H, 
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         const int i = tid + k * NT;
-        dv[k] = i < n ? H.d[base + i] : 0.0;
+        dv[k] = i < n ? gmem(H.d)[base + i] : 0.0;
         if (i < n) {
             mn = fmin(mn, dv[k]);
             mx = fmax(mx, dv[k]);
@@ -204,11 +204,30 @@ __device__ __forceinline__ void sort_ray(int n, int64_t base, const DevHits &H, 
         }
     }
     __syncthreads();
-    for (int i = tid; i < n; i += NT) {
-        const uint32_t q = L.idx[i];
-        H.perm[base + i] = q;
-        H.sslot[base + i] = H.slot[base + q];
-        H.sb[base + i] = H.b[base + q];
+    // the permutation and the sorted columns: every gather of a thread is
+    // issued before its first store (so its PER round trips overlap)
+    uint32_t qv[PER], sv[PER];
+    double bv[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = tid + k * NT;
+        qv[k] = i < n ? (uint32_t)L.idx[i] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        if (tid + k * NT < n) {
+            sv[k] = gmem(H.slot)[base + qv[k]];
+            bv[k] = gmem(H.b)[base + qv[k]];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = tid + k * NT;
+        if (i < n) {
+            H.perm[base + i] = qv[k];
+            H.sslot[base + i] = sv[k];
+            H.sb[base + i] = bv[k];
+        }
     }
     __syncthreads();
 }
