@@ -99,7 +99,18 @@ struct Sys {
 
 struct StepIn {
     double yGdepHI, yGdepHeI, yGdepHeII, heat, time, nH, T, yeMET;
+    // the deposits per unit time, the constant terms of the right-hand side
+    // (the same quotients it forms, computed once per step instead of per
+    // evaluation): set by rates_per_time()
+    double gHI, gHeI, gHeII, qheat;
 };
+
+__device__ __forceinline__ void rates_per_time(StepIn &s) {
+    s.gHI = s.yGdepHI / s.time;
+    s.gHeI = s.yGdepHeI / s.time;
+    s.gHeII = s.yGdepHeII / s.time;
+    s.qheat = s.heat / s.time;
+}
 
 // f_i / time of src/solve.c:76-141 from the fractions, ye and the rates, in
 // the reference's expression order (rhs and rhs_jac share it, so their f
@@ -113,25 +124,25 @@ __device__ __forceinline__ void rhs_rows(const double x[7], double ye, const Rat
     const double xHeII = x[2];
     const double xHeIII = 1.0 - x[2] - x[1];
     if (!OTS) {
-        f[0] = -s.yGdepHI / s.time + (-R.gamma_HI * ye * xHI + R.alpha_HII_A * ye * xHII);
+        f[0] = -s.gHI + (-R.gamma_HI * ye * xHI + R.alpha_HII_A * ye * xHII);
         f[3] = R.alpha_HII_AB * ye * xHII;
         if (HONLY) {
             f[1] = f[2] = f[4] = f[5] = 0.0;
         } else {
-            f[1] = -s.yGdepHeI / s.time + (-R.gamma_HeI * ye * xHeI + R.alpha_HeII_A * ye * xHeII);
-            f[2] = -s.yGdepHeII / s.time +
+            f[1] = -s.gHeI + (-R.gamma_HeI * ye * xHeI + R.alpha_HeII_A * ye * xHeII);
+            f[2] = -s.gHeII +
                    (-R.gamma_HeII * ye * xHeII - R.alpha_HeII_A * ye * xHeII + R.alpha_HeIII_A * ye * xHeIII);
             f[4] = R.alpha_HeII_AB * ye * xHeII;
             f[5] = R.alpha_HeIII_AB * ye * xHeIII;
         }
     } else {
-        f[0] = -s.yGdepHI / s.time + (-R.gamma_HI * ye * xHI + R.alpha_HII_B * ye * xHII);
+        f[0] = -s.gHI + (-R.gamma_HI * ye * xHI + R.alpha_HII_B * ye * xHII);
         if (HONLY) {
             f[1] = f[2] = 0.0;
         } else {
             // x_HI in the He lines, as src/solve.c:103-104
-            f[1] = -s.yGdepHeI / s.time + (-R.gamma_HeI * ye * xHI + R.alpha_HeII_B * ye * xHeII);
-            f[2] = -s.yGdepHeII / s.time +
+            f[1] = -s.gHeI + (-R.gamma_HeI * ye * xHI + R.alpha_HeII_B * ye * xHeII);
+            f[2] = -s.gHeII +
                    (-R.gamma_HeII * ye * xHI - R.alpha_HeII_B * ye * xHeII + R.alpha_HeIII_B * ye * xHeIII);
         }
         f[3] = f[4] = f[5] = 0;
@@ -141,7 +152,7 @@ __device__ __forceinline__ void rhs_rows(const double x[7], double ye, const Rat
     } else {
         const double L = P.C_H_PER_MASS * ((R.zeta_HI + R.psi_HI) * ye * xHI + (R.eta_HII * ye * xHII) +
                                            R.beta * ye * xHII + R.chi * ye);
-        f[6] = s.heat / s.time - L;
+        f[6] = s.qheat - L;
     }
 }
 
@@ -376,19 +387,21 @@ __device__ void rhs_jac(const double x[7], double f[7], double *J, const StepIn 
 #define ROS3_E3 0.22354069897811569627360909276199
 #define ROS3_MAXSTEPS 100000
 
-// v^(1/3) for the step-size controller, v in [1e-10, 1e10]: Newton's
-// iteration on the mantissa with correctly rounded operations only, so the
-// host oracle computes the same bits
-__device__ __forceinline__ double ctl_cbrt(double v) {
+// v^(-1/3) for the step-size controller, v in [1e-10, 1e10]: with v = m 2^e,
+// m in [0.5, 1), e = 3q + r, v^(-1/3) = m^(-1/3) 2^(-r/3) 2^(-q), and
+// m^(-1/3) by Newton's iteration for the reciprocal cube root, y += y (1 -
+// m y^3) / 3 (no division; 4 steps from the linear guess reach 1 ulp).
+// Correctly rounded operations only, so the host oracle computes the same bits
+__device__ __forceinline__ double ctl_rcbrt(double v) {
     int e;
-    double m = frexp(v, &e);
+    const double m = frexp(v, &e);
     const int r = ((e % 3) + 3) % 3;
-    m = ldexp(m, r);
-    e -= r;
-    double y = 0.7 + 0.25 * m;
+    const int q = (e - r) / 3;
+    double y = 1.4 - 0.4 * m;
 #pragma unroll
-    for (int k = 0; k < 5; k++) y = y - (y * y * y - m) / (3.0 * y * y);
-    return ldexp(y, e / 3);
+    for (int k = 0; k < 4; k++) y = y + y * (1.0 - m * (y * y * y)) * (1.0 / 3.0);
+    const double c = r == 0 ? 1.0 : (r == 1 ? 0.79370052598409973738 : 0.62996052494743658238);
+    return ldexp(y * c, -q);
 }
 
 // Isothermal hydrogen-only on-the-spot: with the rates fixed over the step
@@ -472,6 +485,7 @@ __device__ __forceinline__ bool lu_factor(double a[N * N], int piv[N]) {
                 }
             }
         const double inv = 1.0 / a[k * N + k];
+        a[k * N + k] = inv;  // the solves multiply by the pivot's reciprocal
 #pragma unroll
         for (int i = k + 1; i < N; i++) {
             const double l = a[i * N + k] * inv;
@@ -508,7 +522,7 @@ __device__ __forceinline__ void lu_solve(const double a[N * N], const int piv[N]
         double s = b[k];
 #pragma unroll
         for (int j = k + 1; j < N; j++) s -= a[k * N + j] * b[j];
-        b[k] = s / a[k * N + k];
+        b[k] = s * a[k * N + k];
     }
 }
 
@@ -604,7 +618,7 @@ struct Integ {
             t += h;
 #pragma unroll
             for (int q = 0; q < 7; q++) x[q] = xn[q];
-            double fac = 0.9 / ctl_cbrt(fmin(fmax(E, 1e-10), 1e10));
+            double fac = 0.9 * ctl_rcbrt(fmin(fmax(E, 1e-10), 1e10));
             fac = fmin(5.0, fmax(0.2, fac));
             if (rejected) fac = fmin(fac, 1.0);
             rejected = false;
@@ -614,7 +628,7 @@ struct Integ {
             return !(t < 1.0 && nsteps < ROS3_MAXSTEPS);
         }
         rejected = true;
-        const double fac = (E == INFINITY) ? 0.2 : fmax(0.2, 0.9 / ctl_cbrt(fmin(E, 1e10)));
+        const double fac = (E == INFINITY) ? 0.2 : fmax(0.2, 0.9 * ctl_rcbrt(fmin(E, 1e10)));
         h *= fac;
         return false;
     }
@@ -668,6 +682,7 @@ __device__ __forceinline__ void load_step(const DevState &S, uint32_t i, int64_t
     s.heat = S.heat[i];
     s.yeMET = S.yeMET[i];
     s.nH = P.C_H_PER_MASS * rho;
+    rates_per_time(s);
     x0[0] = lH;
     x0[1] = lHeI;
     x0[2] = lHeII;
@@ -924,6 +939,7 @@ __global__ __launch_bounds__(256) void k_solve(psr_step *__restrict__ steps, int
     s.nH = st.nH;
     s.T = st.T;
     s.yeMET = st.yeMET;
+    rates_per_time(s);
     const double x0[7] = {st.lambdaH, st.lambdaHeI, st.lambdaHeII, st.yGrecHII, st.yGrecHeII, st.yGrecHeIII,
                           st.ie};
     double xc[7];
