@@ -984,14 +984,21 @@ __global__ __launch_bounds__(ORD_THREADS) void k_ray_order(const int32_t *__rest
     }
 }
 
-// what the apply wavefronts read besides the walk record: log10 x of the
-// secondary-ionization lookup (-inf where x <= 0), computed by the prep from
-// the fractions, so the record is 16 bytes (H + He: instead of 32) and the
+// what the apply wavefronts read besides the walk record.  H + He: log10 x
+// of the secondary-ionization lookup (-inf where x <= 0), computed by the
+// prep from the fractions, so the record is 16 bytes instead of 32 and the
 // apply's loop holds no logarithm (whose polynomial constants it spilled
 // and reloaded every iteration at its register budget; 256^3 H + He deposit
-// 6.19 -> 5.66 ms)
+// 6.19 -> 5.66 ms).  Hydrogen: x_HI, and the apply forms x (its prep wave,
+// which the short hydrogen walk waits on, takes no extra work)
 template <bool HE>
 struct MAux {
+    uint32_t slot;
+    float mass;
+    double xHI;
+};
+template <>
+struct MAux<true> {
     uint32_t slot;
     float mass;
     double lx;
@@ -1031,7 +1038,7 @@ __device__ __forceinline__ void mr_index(HitGather<HE> &g, const DevHits &H, con
 }
 
 template <bool HE, int MR>
-__global__ __launch_bounds__(MTHREADS, 4) void k_deposit_multi(const DevTick *__restrict__ tk, int64_t nray,
+__global__ __launch_bounds__(MTHREADS, MR == 2 ? 4 : 3) void k_deposit_multi(const DevTick *__restrict__ tk, int64_t nray,
                                                                DevHits H, const DevCtx *__restrict__ C,
                                                                DevStats *__restrict__ st0,
                                                                double *__restrict__ ray_len,
@@ -1116,16 +1123,13 @@ __global__ __launch_bounds__(MTHREADS, 4) void k_deposit_multi(const DevTick *__
                     MAux<HE> ma;
                     ma.slot = au.slot;
                     ma.mass = au.mass;
-                    {
-                        double xHeI = 0.0, xHeII = 0.0;
-                        if constexpr (HE) {
-                            xHeI = au.xHeI;
-                            xHeII = au.xHeII;
-                        }
-                        const double x = secion_x<HE>(au.xHI, xHeI, xHeII, au.mass, P);
+                    if constexpr (HE) {
+                        const double x = secion_x<HE>(au.xHI, au.xHeI, au.xHeII, au.mass, P);
                         // (-inf: x <= 0; without a table only the gate x > 0
                         // counts, so no logarithm is taken)
                         ma.lx = x > 0.0 ? (T.si_n1 > 0 ? log10(x) : 0.0) : -INFINITY;
+                    } else {
+                        ma.xHI = au.xHI;
                     }
                     L.aux[ring][j] = ma;
                 }
@@ -1229,7 +1233,10 @@ __global__ __launch_bounds__(MTHREADS, 4) void k_deposit_multi(const DevTick *__
                 const int s1 = i - 1;
                 if (s1 >= 0 && L.n[p] > 0) {
                     const MAux<HE> &au = L.aux[s1 % 3][j];
-                    nxt = secion_at_lx<HE>(au.lx, T, P, kc);
+                    if constexpr (HE)
+                        nxt = secion_at_lx<HE>(au.lx, T, P, kc);
+                    else
+                        nxt = secion_at<HE>(secion_x<HE>(au.xHI, 0.0, 0.0, au.mass, P), T, P, kc);
                 }
                 const int s = i - 2;
                 if (s >= 0 && s <= L.endseg[p] && s * MSEG + k <= L.last[p]) {

@@ -419,19 +419,28 @@ __global__ __launch_bounds__(NT, NT == 256 ? 8 : 6) void k_trace(
                     double d2 = (double)qx;
                     d2 += (double)qy;
                     d2 += (double)qz;
-                    const double dist = sqrt(d2);
-                    const double r2 = (dist - proj) * (dist + proj);
                     const float hh = p.w * p.w;
                     const bool valid = !(__float_as_uint(p.w) >> 31);
-                    const bool inside = !((double)hh < r2);
-                    if (valid & inside) {  // not &&: no branch between the two loads
-                        const int h = atomicAdd(&s_nh, 1);
-                        if (h < H.cap) {
-                            H.d[base + h] = dist;
-                            H.b[base + h] = sqrt(fabs(r2));
-                            H.slot[base + h] = (uint32_t)k;
+                    // The reference's r2 = (sqrt(d2) - proj)(sqrt(d2) + proj)
+                    // differs from d2 - proj^2 by under 8 units of 2^-53 d2
+                    // (the rounding of the square root, the sum, the
+                    // difference and the product), so a particle with hh
+                    // below d2 - proj^2 - 2^-48 d2 is outside whatever the
+                    // rounding: only the others need the square root and the
+                    // reference's expression, which decides for them
+                    const double r2a = d2 - proj * proj;
+                    if (valid & !((double)hh < r2a - d2 * 3.5527136788005009e-15)) {
+                        const double dist = sqrt(d2);
+                        const double r2 = (dist - proj) * (dist + proj);
+                        if (!((double)hh < r2)) {
+                            const int h = atomicAdd(&s_nh, 1);
+                            if (h < H.cap) {
+                                H.d[base + h] = dist;
+                                H.b[base + h] = sqrt(fabs(r2));
+                                H.slot[base + h] = (uint32_t)k;
+                            }
+                            active[k] = 1;
                         }
-                        active[k] = 1;
                     }
                 }
                 __syncthreads();
