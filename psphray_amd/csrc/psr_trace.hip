@@ -891,8 +891,14 @@ static int tune_decide(psr_ctx *c, bool block) {
     const hipEvent_t last = c->tune_ev[2 * (2 * nc) - 1];
     if (block) PSR_HIP(c, hipEventSynchronize(last));
     else if (hipEventQuery(last) != hipSuccess) return PSR_OK;
-    float best[DEP_NMODE];
-    for (int m = 0; m < DEP_NMODE; m++) best[m] = INFINITY;
+    // the mean of each candidate's two trials (their ticks are placed
+    // symmetrically, see tick_mode)
+    float sum[DEP_NMODE], best[DEP_NMODE];
+    int cnt[DEP_NMODE];
+    for (int m = 0; m < DEP_NMODE; m++) {
+        sum[m] = 0.0f;
+        cnt[m] = 0;
+    }
     for (int k = 0; k < 2 * nc; k++) {
         if (c->tune_slot[k] >= 0) {
             const int32_t v = ((volatile int32_t *)c->h_ring)[c->tune_slot[k]];
@@ -901,8 +907,10 @@ static int tune_decide(psr_ctx *c, bool block) {
         }
         float ms = 0;
         PSR_HIP(c, hipEventElapsedTime(&ms, c->tune_ev[2 * k], c->tune_ev[2 * k + 1]));
-        best[c->tune_mode[k]] = std::min(best[c->tune_mode[k]], ms);
+        sum[c->tune_mode[k]] += ms;
+        cnt[c->tune_mode[k]]++;
     }
+    for (int m = 0; m < DEP_NMODE; m++) best[m] = cnt[m] ? sum[m] / cnt[m] : INFINITY;
     int pick = -1;
     for (int i = 0; i < nc; i++) {
         if (!(best[cand[i]] < INFINITY)) {
@@ -937,8 +945,12 @@ static int tick_mode(const psr_ctx *c, int *trial) {
     if (nc == 1) return cand[0];
     if (c->dep_use >= 0) return c->dep_use;
     if (c->tune_n < 2 * nc) {
+        // two rounds, the second in reverse order: every candidate's two
+        // trial ticks then lie at the same mean position, so a deposit that
+        // grows or shrinks steadily over the first ticks (fronts forming,
+        // rays cut earlier or later) favours none of them
         *trial = c->tune_n;
-        return cand[*trial % nc];
+        return cand[*trial < nc ? *trial : 2 * nc - 1 - *trial];
     }
     return cand[0];
 }
