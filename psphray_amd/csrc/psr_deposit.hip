@@ -961,25 +961,47 @@ struct MShape {
     static constexpr int ROW = SEG + 1;     // padded LDS row (see below)
 };
 
-// rays by descending hit count: buckets of equal bit length (count 0 last)
+// rays by descending hit count, in buckets of ORD_W hits (a batch's walks
+// then end within ORD_W hits of one another; lists longer than ORD_NB
+// buckets share the first): a counting sort in one workgroup
+constexpr int ORD_NB = 4096, ORD_W = 16;
+__device__ __forceinline__ int ord_bucket(int32_t n) { return ORD_NB - 1 - min(n / ORD_W, ORD_NB - 1); }
+
 __global__ __launch_bounds__(ORD_THREADS) void k_ray_order(const int32_t *__restrict__ count, int64_t nray,
                                                            uint32_t *__restrict__ order, const int32_t *halt) {
-    __shared__ uint32_t hist[33], base[33];
+    static_assert(ORD_NB % ORD_THREADS == 0, "whole runs of buckets per thread");
+    constexpr int PER = ORD_NB / ORD_THREADS;
+    __shared__ uint32_t hist[ORD_NB];
+    __shared__ uint32_t wsum[ORD_THREADS / 64];
     if (halted(halt)) return;
-    if (threadIdx.x < 33) hist[threadIdx.x] = 0;
+    const int tid = threadIdx.x;
+    for (int b = tid; b < ORD_NB; b += ORD_THREADS) hist[b] = 0;
     __syncthreads();
-    for (int64_t i = threadIdx.x; i < nray; i += ORD_THREADS) atomicAdd(&hist[__clz(count[i])], 1u);
+    for (int64_t i = tid; i < nray; i += ORD_THREADS) atomicAdd(&hist[ord_bucket(count[i])], 1u);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t run = 0;
-        for (int k = 0; k < 33; k++) {
-            base[k] = run;
-            run += hist[k];
-        }
+    // exclusive scan of the counts: thread t owns buckets [PER t, PER t + PER)
+    uint32_t own = 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) own += hist[PER * tid + k];
+    uint32_t incl = own;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t v = __shfl_up(incl, o, 64);
+        if ((tid & 63) >= o) incl += v;
+    }
+    if ((tid & 63) == 63) wsum[tid >> 6] = incl;
+    __syncthreads();
+    uint32_t run = incl - own;
+    for (int w = 0; w < (tid >> 6); w++) run += wsum[w];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const uint32_t c = hist[PER * tid + k];
+        hist[PER * tid + k] = run;
+        run += c;
     }
     __syncthreads();
-    for (int64_t i = threadIdx.x; i < nray; i += ORD_THREADS) {
-        const uint32_t pos = atomicAdd(&base[__clz(count[i])], 1u);
+    for (int64_t i = tid; i < nray; i += ORD_THREADS) {
+        const uint32_t pos = atomicAdd(&hist[ord_bucket(count[i])], 1u);
         order[pos] = (uint32_t)i;
     }
 }
