@@ -35,7 +35,9 @@ BYTES_DEPOSIT_HE = 68
 # WRITE_SIZE x 1), the timed-tick dispatch of each workload (the 128^3,
 # 256^3 and batched-field traffic summaries under profiles/)
 TRAFFIC_DEPOSIT = {(2, "k_deposit"): (84968 * 2 + 50806) * 1024,
-                   (3, "k_deposit_multi"): (3307458 * 2 + 5078697) * 1024,   # two rays per workgroup
+                   (2, "k_deposit_multi"): (55818 * 2 + 47923) * 1024,       # two rays per workgroup
+                   (3, "k_deposit_multi"): (3420537 * 2 + 5079307) * 1024,   # two rays per workgroup
+                   (5, "k_deposit_multi"): (91585 * 2 + 81476) * 1024,       # four rays per workgroup
                    (3, "k_deposit"): (5984385 * 2 + 4237966) * 1024,
                    (5, "k_deposit"): (151006 * 2 + 92589) * 1024}
 WORKLOADS = {
