@@ -266,10 +266,13 @@ def main():
     run(args.warmup, psr.PSR_CHEM_EXACT)
     ctx.sync()
     extra = 0
+    # further ticks back to back, as the timed ones run (the tick-graph trials
+    # time each tick from the end of the one before), until the autotuner
+    # has decided
     while not decided() and extra < tune_extra:
         run(1, psr.PSR_CHEM_EXACT)
-        ctx.sync()
         extra += 1
+    ctx.sync()
     if not decided():
         raise SystemExit("autotuner undecided after the warm-up ticks; no line is printed")
     warmup_ticks = args.warmup + extra

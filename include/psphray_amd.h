@@ -403,12 +403,21 @@ int psr_get_timers(psr_ctx *ctx, double *ms);
  * launch, recorded once per tick shape (ray count, capacities, phases) and
  * replayed with the tick's values from a device tick record; ticks with
  * phase timers, accumulation trials or hit lists beyond 8192 hits run
- * kernel by kernel.  0: always kernel by kernel.  -1 (default): four timed
- * trial ticks, alternating, decide; psr_get_tuning then returns the choice
- * (-1 while undecided).
+ * kernel by kernel.  0: always kernel by kernel.  -1 (default): eight timed
+ * trial ticks (two kernel by kernel, two as graph launches, twice; each
+ * pair's second tick timed from the first one's end) decide;
+ * psr_get_tuning then returns the choice (-1 while undecided).
  * PSR_TUNE_GRAPH_LAUNCHES (read only): ticks run as graph launches so far. */
 #define PSR_TUNE_TICK_GRAPH 2
 #define PSR_TUNE_GRAPH_LAUNCHES 3
+/* PSR_TUNE_TICK_OVERLAP: 1 (default) runs the update of an asynchronous
+ * tick launched kernel by kernel (no phase timers) on a second stream, beside
+ * the next tick's trace and sort; the next tick's deposit, and every other
+ * call, waits for it.  The values are those of consecutive ticks (each tick
+ * has its own device record and active-particle array; the next tick's trace
+ * overflowing does not stop this tick's update).  0: one stream, the phases
+ * of consecutive ticks in order. */
+#define PSR_TUNE_TICK_OVERLAP 6
 /* PSR_TUNE_DEPOSIT_WALK: how many rays a deposit workgroup carries (their
  * serial transmission walks, src/loop.c:439-589, run one per lane).
  *   PSR_WALK_AUTO    timed together with the accumulation choices (two

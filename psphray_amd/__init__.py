@@ -15,14 +15,15 @@ from . import _abi
 from ._abi import (PSR_ACCUM_ATOMIC, PSR_ACCUM_AUTO, PSR_ACCUM_BINNED, PSR_CHEM_EXACT,
                    PSR_CHEM_SURROGATE, PSR_CHEM_SURROGATE_STIFF, PSR_PHASE_ALL, PSR_PHASE_DEPOSIT, PSR_PHASE_TRACE,
                    PSR_PHASE_UPDATE, PSR_TUNE_DEPOSIT_ACCUM, PSR_TUNE_DEPOSIT_WALK,
-                   PSR_TUNE_GRAPH_LAUNCHES, PSR_TUNE_TICK_GRAPH, PSR_TUNE_TRACE_THREADS, PSR_WALK_AUTO, PSR_WALK_MULTI,
+                   PSR_TUNE_GRAPH_LAUNCHES, PSR_TUNE_TICK_GRAPH, PSR_TUNE_TICK_OVERLAP, PSR_TUNE_TRACE_THREADS,
+                   PSR_WALK_AUTO, PSR_WALK_MULTI,
                    PSR_WALK_MULTI2, PSR_WALK_MULTI8, PSR_WALK_SINGLE, RAY_DTYPE, STEP_DTYPE, ptr)
 
 __all__ = ["Physics", "Context", "ParticleSystem", "PsrError", "RAY_DTYPE", "STEP_DTYPE",
            "PSR_PHASE_ALL", "PSR_PHASE_TRACE", "PSR_PHASE_DEPOSIT", "PSR_PHASE_UPDATE",
            "PSR_CHEM_EXACT", "PSR_CHEM_SURROGATE", "PSR_CHEM_SURROGATE_STIFF", "PSR_TUNE_DEPOSIT_ACCUM", "PSR_ACCUM_AUTO",
            "PSR_ACCUM_ATOMIC", "PSR_ACCUM_BINNED", "PSR_TUNE_TICK_GRAPH", "PSR_TUNE_GRAPH_LAUNCHES",
-           "PSR_TUNE_DEPOSIT_WALK", "PSR_TUNE_TRACE_THREADS", "PSR_WALK_AUTO", "PSR_WALK_SINGLE", "PSR_WALK_MULTI", "PSR_WALK_MULTI2",
+           "PSR_TUNE_DEPOSIT_WALK", "PSR_TUNE_TRACE_THREADS", "PSR_TUNE_TICK_OVERLAP", "PSR_WALK_AUTO", "PSR_WALK_SINGLE", "PSR_WALK_MULTI", "PSR_WALK_MULTI2",
            "PSR_WALK_MULTI8",
            "default_units", "make_flags"]
 

@@ -33,6 +33,7 @@ PSR_TUNE_TICK_GRAPH = 2
 PSR_TUNE_GRAPH_LAUNCHES = 3
 PSR_TUNE_DEPOSIT_WALK = 4
 PSR_TUNE_TRACE_THREADS = 5
+PSR_TUNE_TICK_OVERLAP = 6
 PSR_WALK_AUTO = 0
 PSR_WALK_SINGLE = 1
 PSR_WALK_MULTI = 2

@@ -41,7 +41,7 @@ DevState psr_ctx::state() const {
     s.lHeII = d_f64 + 2 * n;
     s.lasthit = d_lasthit;
     s.hits = d_hits;
-    s.active = d_active;
+    s.active = active_of(par);
     s.dv = d_dview;
     s.dv_he = dview_he;
     return s;
@@ -72,8 +72,16 @@ DevPhys psr_dev_phys(const psr_ctx *c) {
     return p;
 }
 
+int psr_join(psr_ctx *c) {
+    if (!c->upd_pending) return PSR_OK;
+    c->upd_pending = false;
+    PSR_HIP(c, hipStreamWaitEvent(c->stream, c->ev_upd, 0));
+    return PSR_OK;
+}
+
 int psr_sync_dev_ctx(psr_ctx *c) {
     if (!c->ctx_dirty && c->d_ctx) return PSR_OK;
+    if (int rc = psr_join(c)) return rc;
     if (!c->d_ctx) PSR_HIP(c, hipMalloc(&c->d_ctx, sizeof(DevCtx)));
     DevCtx h;
     h.S = c->state();
@@ -87,6 +95,7 @@ int psr_sync_dev_ctx(psr_ctx *c) {
 
 int psr_ensure_scratch(psr_ctx *c, size_t bytes) {
     if (bytes <= c->scratch_bytes) return PSR_OK;
+    if (int rc = psr_join(c)) return rc;
     PSR_HIP(c, hipStreamSynchronize(c->stream));  // queued kernels may use the old block
     if (c->d_scratch) PSR_HIP(c, hipFree(c->d_scratch));
     c->d_scratch = nullptr;
@@ -158,7 +167,9 @@ static void free_state(psr_ctx *c) {
 extern "C" void psr_destroy(psr_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
+    (void)psr_join(c);
     (void)hipStreamSynchronize(c->stream);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     if (c->h_ring) (void)hipHostFree(c->h_ring);
     free_tree(c);
     free_state(c);
@@ -187,14 +198,18 @@ extern "C" void psr_destroy(psr_ctx *c) {
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
     for (int k = 0; k < 2 * psr_ctx::DTUNE_N; k++)
         if (c->tune_ev[k]) (void)hipEventDestroy(c->tune_ev[k]);
-    for (int k = 0; k < 2 * psr_ctx::TUNE_N; k++)
+    for (int k = 0; k < psr_ctx::TUNE_N; k++)
         if (c->gtune_ev[k]) (void)hipEventDestroy(c->gtune_ev[k]);
+    if (c->ev_dep) (void)hipEventDestroy(c->ev_dep);
+    if (c->ev_upd) (void)hipEventDestroy(c->ev_upd);
     for (int k = 0; k < 2 * psr_ctx::TTUNE_N; k++)
         if (c->ttune_ev[k]) (void)hipEventDestroy(c->ttune_ev[k]);
     psr_graph_reset(c);
     psr_emit_free(c);
-    dfree(c->d_tick);
+    dfree(c->d_tick_buf);
+    c->d_tick = nullptr;
     if (c->d_scratch) (void)hipFree(c->d_scratch);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -204,6 +219,7 @@ extern "C" const char *psr_last_error(const psr_ctx *c) { return c ? c->err.c_st
 extern "C" int psr_sync(psr_ctx *c) {
     if (!c) return PSR_ERR_ARG;
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     int rc = psr_resolve(c);
     if (rc) return rc;
     PSR_HIP(c, hipStreamSynchronize(c->stream));
@@ -218,6 +234,7 @@ extern "C" int psr_set_physics(psr_ctx *c, const psr_units *u, const psr_flags *
         PSR_FAIL(c, PSR_ERR_ARG, "psr_set_physics: incomplete tables");
     if (f->comoving && f->redshift < 0) PSR_FAIL(c, PSR_ERR_ARG, "negative redshift");
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     if (int rc = psr_resolve(c)) return rc;
     dfree(c->d_tables);
     const size_t nar = 16 * (size_t)t->ar_nrows, nxs = 4 * (size_t)t->xs_nrows,
@@ -514,6 +531,7 @@ extern "C" int psr_tree_build(psr_ctx *c, const float *pos, const float *sml, co
         if (!(boxsizes[f] > 0)) PSR_FAIL(c, PSR_ERR_ARG, "boxsize must be > 0");
     }
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     if (int rc = psr_resolve(c)) return rc;
     hipStream_t st = c->stream;
     free_tree(c);
@@ -791,6 +809,7 @@ extern "C" int psr_state_upload(psr_ctx *c, const psr_state *s) {
     if (!c->have_tree) PSR_FAIL(c, PSR_ERR_STATE, "psr_state_upload before psr_tree_build");
     if (s->npar != c->npar) PSR_FAIL(c, PSR_ERR_ARG, "psr_state_upload: npar differs from the tree");
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     if (int rc = psr_resolve(c)) return rc;
     const int64_t n = c->npar, nal = n > 0 ? n : 1;
     if (!c->have_state) {
@@ -799,8 +818,8 @@ extern "C" int psr_state_upload(psr_ctx *c, const psr_state *s) {
         PSR_HIP(c, hipMalloc(&c->d_dview, sizeof(DepViewHe) * nal));
         PSR_HIP(c, hipMalloc(&c->d_lasthit, sizeof(int64_t) * nal));
         PSR_HIP(c, hipMalloc(&c->d_hits, sizeof(int32_t) * nal));
-        PSR_HIP(c, hipMalloc(&c->d_active, nal));
-        PSR_HIP(c, hipMemsetAsync(c->d_active, 0, nal, c->stream));
+        PSR_HIP(c, hipMalloc(&c->d_active, 2 * nal));
+        PSR_HIP(c, hipMemsetAsync(c->d_active, 0, 2 * nal, c->stream));
     }
     if (psr_ensure_scratch(c, sizeof(double) * nal)) return PSR_ERR_HIP;
     void *tmp = c->d_scratch;
@@ -838,6 +857,7 @@ extern "C" int psr_state_download(psr_ctx *c, psr_state *s) {
     if (!c->have_state) PSR_FAIL(c, PSR_ERR_STATE, "no particle state");
     if (s->npar != c->npar) PSR_FAIL(c, PSR_ERR_ARG, "psr_state_download: npar differs");
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     if (int rc = psr_resolve(c)) return rc;
     const int64_t nal = c->npar > 0 ? c->npar : 1;
     if (psr_ensure_scratch(c, sizeof(double) * 3 * nal)) return PSR_ERR_HIP;
@@ -892,6 +912,7 @@ extern "C" int psr_output_gather(psr_ctx *c, float *xHI, float *xHeI, float *xHe
     if (!c) return PSR_ERR_ARG;
     if (!c->have_state) PSR_FAIL(c, PSR_ERR_STATE, "no particle state");
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     if (int rc = psr_resolve(c)) return rc;
     const int64_t n = c->npar;
     if (n == 0) return PSR_OK;
@@ -915,6 +936,7 @@ extern "C" int psr_output_gather(psr_ctx *c, float *xHI, float *xHeI, float *xHe
 extern "C" int psr_rays_upload(psr_ctx *c, const psr_ray *rays, int64_t nray) {
     if (!c || nray < 0 || (nray > 0 && !rays)) return PSR_ERR_ARG;
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     // a replay of a pending tick needs the pool it was launched with
     if (int rc = psr_resolve(c)) return rc;
     // the recombination candidates of the tick tracing these rays: every hit
@@ -946,6 +968,7 @@ extern "C" int psr_rays_download(psr_ctx *c, int64_t first, int64_t nray, psr_ra
     if (!c || first < 0 || nray < 0 || (nray > 0 && !rays)) return PSR_ERR_ARG;
     if (first + nray > c->nray_pool) PSR_FAIL(c, PSR_ERR_ARG, "psr_rays_download: range outside the pool");
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     PSR_HIP(c, hipStreamSynchronize(c->stream));
     if (nray) PSR_HIP(c, hipMemcpy(rays, c->d_rays + first, sizeof(psr_ray) * nray, hipMemcpyDeviceToHost));
     return PSR_OK;
@@ -956,6 +979,7 @@ extern "C" int psr_get_stats(psr_ctx *c, psr_stats *out) {
     if (!c || !out) return PSR_ERR_ARG;
     std::vector<DevStats> hs(PSR_NSTAT);
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     if (int rc = psr_resolve(c)) return rc;
     PSR_HIP(c, hipStreamSynchronize(c->stream));
     PSR_HIP(c, hipMemcpy(hs.data(), c->d_stats, sizeof(DevStats) * PSR_NSTAT, hipMemcpyDeviceToHost));
@@ -999,6 +1023,7 @@ extern "C" int psr_get_stats(psr_ctx *c, psr_stats *out) {
 extern "C" int psr_reset_stats(psr_ctx *c) {
     if (!c) return PSR_ERR_ARG;
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     if (int rc = psr_resolve(c)) return rc;
     PSR_HIP(c, hipMemsetAsync(c->d_stats, 0, sizeof(DevStats) * PSR_NSTAT, c->stream));
     PSR_HIP(c, hipStreamSynchronize(c->stream));

@@ -821,7 +821,7 @@ __global__ __launch_bounds__(256) void k_update(DevState S, int64_t npar, const 
                                                 double *__restrict__ saved, uint32_t *__restrict__ list,
                                                 uint32_t *__restrict__ nlist) {
     using G = Integ<ISO, ADI, OTS, HONLY>;
-    if (halted(&st->flags)) return;  // this tick's trace overflowed: replayed later
+    if (halted(&tk->flags)) return;  // this tick's trace overflowed: replayed later
     const int64_t tick = tk->tick;
     const double tick_time = tk->tick_time;
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -882,7 +882,7 @@ __global__ __launch_bounds__(256) void k_update_rest(DevState S, const uint32_t 
                                                      const DevTick *__restrict__ tk, DevTables T, DevPhys P,
                                                      DevStats *__restrict__ st, const double *__restrict__ saved) {
     using G = Integ<ISO, ADI, OTS, HONLY>;
-    if (halted(&st->flags)) return;
+    if (halted(&tk->flags)) return;
     const int64_t tick = tk->tick;
     const double tick_time = tk->tick_time;
     const uint32_t n = nlist[0];
@@ -997,7 +997,7 @@ __global__ __launch_bounds__(256) void k_mlp_update(DevState S, int64_t npar, co
     __shared__ float s_feat[4][64][MLP_IN + 1];
     __shared__ float s_out[4][64][MLP_OUT];
     __shared__ int part[2][4];
-    if (halted(&st->flags)) return;
+    if (halted(&tk->flags)) return;
     const int64_t tick = tk->tick;
     const double tick_time = tk->tick_time;
     for (int k = threadIdx.x; k < 3 * MLP_HID + MLP_OUT; k += blockDim.x) {
@@ -1123,6 +1123,7 @@ __global__ void k_zero_u32x2(uint32_t *p) {
 int psr_prepare_update(psr_ctx *c) {
     const int64_t n = c->npar;
     if (c->qlist_cap < n) {
+        if (int rc = psr_join(c)) return rc;
         PSR_HIP(c, hipStreamSynchronize(c->stream));  // earlier ticks may still use them
         if (c->d_qlist) PSR_HIP(c, hipFree(c->d_qlist));
         if (c->d_saved) PSR_HIP(c, hipFree(c->d_saved));
@@ -1177,7 +1178,7 @@ int psr_launch_update(psr_ctx *c, bool timed) {
     if (rc) return rc;
     if (timed) PSR_EV(c, 3, 0);
     unsigned grid = (unsigned)((n + 255) / 256);
-    const int32_t *halt = &c->d_stats->flags;
+    const int32_t *halt = &c->d_tick->flags;  // as this tick's deposit left it
     uint32_t *row_of = nullptr;
     if (M.cap) {
         row_of = psr_mlp_rowof(c, n);
@@ -1315,6 +1316,7 @@ extern "C" int psr_solve_steps_noalloc(psr_ctx *c, psr_step *steps, int64_t n, d
     if (rtol <= 0) rtol = 1e-4;
     if (n == 0) return PSR_OK;
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     psr_step *d = nullptr;
     {
         std::lock_guard<std::mutex> lk(g_scratch_mu);
@@ -1344,6 +1346,7 @@ extern "C" int psr_solve_steps(psr_ctx *c, psr_step *steps, int64_t n, double rt
     if (!c->have_physics) PSR_FAIL(c, PSR_ERR_STATE, "no physics: call psr_set_physics");
     if (rtol <= 0) rtol = 1e-4;
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     psr_step *d = nullptr;
     PSR_HIP(c, hipMalloc(&d, sizeof(psr_step) * std::max<int64_t>(n, 1)));
     hipError_t e = hipMemcpyAsync(d, steps, sizeof(psr_step) * n, hipMemcpyHostToDevice, c->stream);

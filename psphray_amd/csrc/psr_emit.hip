@@ -489,6 +489,7 @@ extern "C" int psr_sources_upload(psr_ctx *c, int64_t nsrc, const double *pos, c
     if (!c || nsrc < 0 || (nsrc > 0 && (!pos || !ngamma_dot))) return PSR_ERR_ARG;
     if (nsrc >= (int64_t)UINT32_MAX) PSR_FAIL(c, PSR_ERR_ARG, "psr_sources_upload: too many sources");
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     if (int rc = psr_resolve(c)) return rc;
     std::vector<EmitSrc> hs(nsrc);
     int32_t nf = 1;
@@ -553,6 +554,7 @@ extern "C" int psr_spectra_upload(psr_ctx *c, int32_t nspec, const int64_t *offs
     for (const EmitSrc &s : e->hsrc)
         if (s.spec >= nspec) PSR_FAIL(c, PSR_ERR_ARG, "a source names a spectrum beyond nspec");
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     efree(e->d_spec_off);
     efree(e->d_spec_eng);
     efree(e->d_spec_cdf);
@@ -571,6 +573,7 @@ extern "C" int psr_sources_lastemit(psr_ctx *c, int64_t *lastemit) {
     if (!c || !lastemit) return PSR_ERR_ARG;
     if (!c->emit) PSR_FAIL(c, PSR_ERR_STATE, "no sources");
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     if (int rc = psr_resolve(c)) return rc;
     PSR_HIP(c, hipStreamSynchronize(c->stream));
     if (c->emit->nsrc)
@@ -708,6 +711,11 @@ extern "C" int psr_emit_rays(psr_ctx *c, int64_t tick, double tick_time, int64_t
     for (const EmitSrc &s : e->hsrc)
         if (s.field >= std::max(c->nfield, 1)) PSR_FAIL(c, PSR_ERR_ARG, "a source names a field the tree does not have");
     PSR_HIP(c, hipSetDevice(c->device));
+    // recombination rays are drawn from the particle state the last tick's
+    // update writes; source rays read none of it, and their kernels may run
+    // beside that update (the next tick's trace follows them in stream order)
+    if (c->rec_nrec > 0)
+        if (int rc_j = psr_join(c)) return rc_j;
     // groups: one of every source, or one per field; a group emits when it
     // has rays and a source with Ngamma_dot > 0 (every lastemit < tick)
     const int32_t nf = (int32_t)e->gsrc_off.size() - 2;

@@ -165,18 +165,24 @@ __device__ __forceinline__ DevStats *stat_slot(DevStats *st) { return st + (bloc
 
 __device__ __forceinline__ bool halted(const int32_t *halt) { return halt && (*(volatile const int32_t *)halt & PSR_DF_HALT); }
 
-// a tick launched without waiting for its trace's capacity check; validated
-// later from its published flag word (host ring slot)
 // per-tick values the tick's kernels read from device memory (written by
 // k_set_tick at the start of every tick), so that one recorded hipGraph
-// serves every tick of the same shape
+// serves every tick of the same shape.  Two records alternate between
+// consecutive ticks, so that a tick's update can still read its own while
+// the next tick's trace runs.
 struct DevTick {
     const psr_ray *rays;  // the tick's rays: pool + ray_first
     int64_t tick;
     double tick_time;
     int32_t *slot;  // pinned ring word k_publish writes
+    // the flag word as the tick's deposit left it (k_snap_flags): the halt
+    // test of its update and the word it publishes, untouched by an
+    // overflow in the next tick's trace
+    int32_t flags;
 };
 
+// a tick launched without waiting for its trace's capacity check; validated
+// later from its published flag word (host ring slot)
 struct PendingTick {
     int64_t ray_first, nray, tick;
     double tick_time;
@@ -227,7 +233,11 @@ struct psr_ctx {
     int32_t dview_he = -1;
     int64_t *d_lasthit = nullptr;
     int32_t *d_hits = nullptr;
+    // particles marked by a tick's trace: two arrays of npar bytes, the
+    // tick's parity (par) picks one; the update clears what it used
     uint8_t *d_active = nullptr;
+    int par = 0;
+    uint8_t *active_of(int p) const { return d_active ? d_active + (size_t)p * (size_t)(npar > 0 ? npar : 1) : nullptr; }
     // rays
     psr_ray *d_rays = nullptr;
     int64_t nray_pool = 0;
@@ -291,14 +301,25 @@ struct psr_ctx {
     // deposit accumulation: requested (PSR_ACCUM_*), chosen (ATOMIC or BINNED
     // once decided, AUTO while the trial ticks run), and the trials: ticks
     // alternating the two choices, each with an event pair around its deposit
-    static constexpr int TUNE_N = 4;  // timed trials per tuned choice
-    DevTick *d_tick = nullptr;
+    static constexpr int TUNE_N = 8;  // tick-graph trial ticks (two pairs per way)
+    DevTick *d_tick = nullptr;      // the current tick's record: d_tick_buf + par
+    DevTick *d_tick_buf = nullptr;  // two records
+    // overlap of a tick's update (on stream2) with the next tick's trace
+    // and sort: requested (PSR_TUNE_TICK_OVERLAP), the events between the
+    // streams, and whether stream2 holds an update the main stream has not
+    // been ordered after yet (psr_join)
+    int32_t overlap_req = 1;
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_dep = nullptr, ev_upd = nullptr;
+    bool upd_pending = false;
+    hipStream_t tick_end = nullptr;  // the stream the last tick ended on
     // the tick recorded as a hipGraph: its executable and the shape it was
     // recorded for (sizes and device pointers its kernels were given)
     int32_t graph_req = -1;  // -1: chosen by timed trials, 0 / 1: off / on
     int32_t graph_use = -1;  // the choice once made
-    hipEvent_t gtune_ev[2 * TUNE_N] = {};
+    hipEvent_t gtune_ev[TUNE_N] = {};  // the end of each trial tick
     int32_t gtune_slot[TUNE_N] = {};
+    uint32_t gtune_seq[TUNE_N] = {};
     int gtune_n = 0;
     hipGraphExec_t graph_exec = nullptr;
     std::vector<int64_t> graph_key;
@@ -324,6 +345,10 @@ struct psr_ctx {
 
     DevState state() const;
 };
+
+// order the main stream after the update of the last overlapped tick (every
+// call except psr_tick starts with it; a no-op when nothing is pending)
+int psr_join(psr_ctx *c);
 
 // ---- launchers implemented in the .hip files ----
 // sync: the trace waits for its capacity check (and retries); otherwise the

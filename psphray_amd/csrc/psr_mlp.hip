@@ -426,7 +426,7 @@ uint32_t *psr_mlp_rowof(psr_ctx *c, int64_t n) {
     MlpState *m = c->mlp;
     if (!m) return nullptr;
     if (m->rowof_cap < n) {
-        if (hipStreamSynchronize(c->stream) != hipSuccess) return nullptr;
+        if (psr_join(c) != PSR_OK || hipStreamSynchronize(c->stream) != hipSuccess) return nullptr;
         if (m->d_rowof) (void)hipFree(m->d_rowof);
         m->d_rowof = nullptr;
         m->rowof_cap = 0;
@@ -563,6 +563,7 @@ extern "C" int psr_mlp_features(const psr_step *s, const psr_units *, float *x) 
 extern "C" int psr_mlp_capture(psr_ctx *c, int64_t cap) {
     if (!c || cap < 0) return PSR_ERR_ARG;
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     if (int rc = psr_resolve(c)) return rc;
     if (int rc = mlp_state(c)) return rc;
     MlpState *m = c->mlp;
@@ -582,6 +583,7 @@ extern "C" int psr_mlp_capture(psr_ctx *c, int64_t cap) {
 extern "C" int psr_mlp_captured(psr_ctx *c, int64_t *n, float *x, float *y, int64_t max_rows) {
     if (!c || !n) return PSR_ERR_ARG;
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     if (int rc = psr_resolve(c)) return rc;
     *n = 0;
     MlpState *m = c->mlp;
@@ -607,6 +609,7 @@ extern "C" int psr_mlp_train(psr_ctx *c, const float *x, const float *y, int64_t
                              float lr, uint64_t seed, float *loss_out) {
     if (!c || epochs < 1 || batch < 16 || !(lr > 0) || (!x) != (!y)) return PSR_ERR_ARG;
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     if (int rc = psr_resolve(c)) return rc;
     if (int rc = mlp_state(c)) return rc;
     MlpState *m = c->mlp;
@@ -670,6 +673,7 @@ extern "C" int psr_mlp_train(psr_ctx *c, const float *x, const float *y, int64_t
 extern "C" int psr_mlp_grad(psr_ctx *c, const float *x, const float *y, int64_t n, float *grad, float *loss) {
     if (!c || !x || !y || n < 1 || n > (1 << 24) || !grad) return PSR_ERR_ARG;
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     if (int rc = psr_resolve(c)) return rc;
     if (!psr_mlp_ready(c)) PSR_FAIL(c, PSR_ERR_STATE, "psr_mlp_grad: set the parameters and normalisation first");
     MlpState *m = c->mlp;
@@ -690,6 +694,7 @@ extern "C" int psr_mlp_forward(psr_ctx *c, const float *x, int64_t n, float *y) 
     if (!c || !x || !y || n < 0) return PSR_ERR_ARG;
     if (n == 0) return PSR_OK;
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     if (int rc = psr_resolve(c)) return rc;
     if (!psr_mlp_ready(c)) PSR_FAIL(c, PSR_ERR_STATE, "psr_mlp_forward: no trained surrogate");
     MlpState *m = c->mlp;
@@ -715,6 +720,7 @@ extern "C" int64_t psr_mlp_param_count(void) { return MLP_NPARAM; }
 extern "C" int psr_mlp_get_params(psr_ctx *c, float *p) {
     if (!c || !p) return PSR_ERR_ARG;
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     if (int rc = psr_resolve(c)) return rc;
     if (!c->mlp || !c->mlp->have_params) PSR_FAIL(c, PSR_ERR_STATE, "psr_mlp_get_params: no parameters yet");
     PSR_HIP(c, hipStreamSynchronize(c->stream));
@@ -725,6 +731,7 @@ extern "C" int psr_mlp_get_params(psr_ctx *c, float *p) {
 extern "C" int psr_mlp_set_params(psr_ctx *c, const float *p) {
     if (!c || !p) return PSR_ERR_ARG;
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     if (int rc = psr_resolve(c)) return rc;
     if (int rc = mlp_state(c)) return rc;
     MlpState *m = c->mlp;
@@ -742,6 +749,7 @@ extern "C" int psr_mlp_set_norm(psr_ctx *c, const float *mean, const float *sigm
     for (int f = 0; f < MLP_IN; f++)
         if (!(sigma[f] > 0)) return PSR_ERR_ARG;
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     if (int rc = psr_resolve(c)) return rc;
     if (int rc = mlp_state(c)) return rc;
     float h[MLP_NORM];
@@ -760,6 +768,7 @@ extern "C" int psr_mlp_set_norm(psr_ctx *c, const float *mean, const float *sigm
 extern "C" int psr_mlp_get_norm(psr_ctx *c, float *mean, float *sigma, float *lo, float *hi) {
     if (!c) return PSR_ERR_ARG;
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     if (int rc = psr_resolve(c)) return rc;
     if (!c->mlp || !c->mlp->have_norm) PSR_FAIL(c, PSR_ERR_STATE, "psr_mlp_get_norm: no normalisation yet");
     float h[MLP_NORM];

@@ -536,10 +536,12 @@ static void launch_trace_kernel(psr_ctx *c, int nt, unsigned grid, int64_t ray_f
                                 const DevTick *tk, const int32_t *halt) {
     if (nt == 128)
         k_trace<128><<<grid, 128, 0, c->stream>>>(c->d_rays + ray_first, tk, nray, c->d_nodes, c->d_field_root,
-                                                  c->d_psl, c->d_orig, c->H, c->d_active, c->d_stats, halt);
+                                                  c->d_psl, c->d_orig, c->H, c->active_of(c->par), c->d_stats,
+                                                  halt);
     else
         k_trace<256><<<grid, 256, 0, c->stream>>>(c->d_rays + ray_first, tk, nray, c->d_nodes, c->d_field_root,
-                                                  c->d_psl, c->d_orig, c->H, c->d_active, c->d_stats, halt);
+                                                  c->d_psl, c->d_orig, c->H, c->active_of(c->par), c->d_stats,
+                                                  halt);
 }
 
 int psr_launch_trace(psr_ctx *c, int64_t ray_first, int64_t nray, bool sync, const DevTick *tk, int nt) {
@@ -661,13 +663,14 @@ extern "C" int psr_trace_lists(psr_ctx *c, int64_t ray_first, int64_t nray, int6
     int rc = check_ready(c, ray_first, nray);
     if (rc) return rc;
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     if ((rc = psr_resolve(c))) return rc;
     offsets[0] = 0;
     if (nray == 0) return PSR_OK;
     const int nt = c->trace_req > 0 ? c->trace_req : c->trace_use > 0 ? c->trace_use : 256;
     if ((rc = psr_launch_trace(c, ray_first, nray, true, nullptr, nt))) return rc;
     // parity mode marks particles active like the tick does; clear them
-    PSR_HIP(c, hipMemsetAsync(c->d_active, 0, c->npar > 0 ? c->npar : 1, c->stream));
+    PSR_HIP(c, hipMemsetAsync(c->active_of(c->par), 0, c->npar > 0 ? c->npar : 1, c->stream));
     if ((rc = psr_launch_sort(c, nray))) return rc;
     std::vector<int32_t> cnt(nray);
     PSR_HIP(c, hipMemcpyAsync(cnt.data(), c->H.count, sizeof(int32_t) * nray, hipMemcpyDeviceToHost, c->stream));
@@ -927,7 +930,21 @@ static int tune_decide(psr_ctx *c, bool block) {
 // the tick record of the kernels that follow (stream order keeps it from
 // changing under the previous tick's kernels)
 __global__ void k_set_tick(DevTick *tk, const psr_ray *rays, int64_t tick, double tick_time, int32_t *slot) {
-    if (threadIdx.x == 0) *tk = DevTick{rays, tick, tick_time, slot};
+    if (threadIdx.x == 0) *tk = DevTick{rays, tick, tick_time, slot, 0};
+}
+
+// the flag word after the tick's deposit, into the tick's record: its update
+// halts on it and it is what the tick publishes, so an overflow in the next
+// tick's trace (which may run beside this tick's update) stops only the
+// ticks from that one on
+__global__ void k_snap_flags(DevTick *tk, const int32_t *flags) {
+    if (threadIdx.x == 0) tk->flags = *(volatile const int32_t *)flags;
+}
+
+// the current tick's record and active array: parity p
+static void set_parity(psr_ctx *c, int p) {
+    c->par = p;
+    c->d_tick = c->d_tick_buf + p;
 }
 
 __global__ void k_fill_u8(uint8_t *p, int64_t n) {
@@ -955,23 +972,30 @@ static int tick_mode(const psr_ctx *c, int *trial) {
     return cand[0];
 }
 
-// the tick-graph trials: whole ticks alternately kernel by kernel (even
-// trials) and as graph launches, timed from after the recording to the end
-// of the tick; the faster way is kept.  Trials on ticks that overflowed do
-// not count.
+// the tick-graph trials: pairs of consecutive ticks kernel by kernel, then
+// as graph launches, twice over; each pair's second tick is timed from the
+// end of its first to its own end (the interval between tick ends, which
+// counts a kernel-by-kernel tick's update running beside the next tick's
+// trace once, not twice), and the faster way is kept.  Pairs broken by
+// another tick and trials on ticks that overflowed do not count.
 static int gtune_decide(psr_ctx *c, bool block) {
     if (c->graph_use >= 0 || c->gtune_n < psr_ctx::TUNE_N) return PSR_OK;
-    const hipEvent_t last = c->gtune_ev[2 * psr_ctx::TUNE_N - 1];
+    const hipEvent_t last = c->gtune_ev[psr_ctx::TUNE_N - 1];
     if (block) PSR_HIP(c, hipEventSynchronize(last));
     else if (hipEventQuery(last) != hipSuccess) return PSR_OK;
     float best[2] = {INFINITY, INFINITY};
-    for (int k = 0; k < psr_ctx::TUNE_N; k++) {
-        const int32_t v = ((volatile int32_t *)c->h_ring)[c->gtune_slot[k]];
-        if (v == 0 && !block) return PSR_OK;
-        if (v == 0 || (v & PSR_DF_HALT)) continue;
+    for (int k = 1; k < psr_ctx::TUNE_N; k += 2) {
+        if (c->gtune_seq[k] != c->gtune_seq[k - 1] + 1) continue;
+        bool ok = true;
+        for (int j = k - 1; j <= k; j++) {
+            const int32_t v = ((volatile int32_t *)c->h_ring)[c->gtune_slot[j]];
+            if (v == 0 && !block) return PSR_OK;
+            if (v == 0 || (v & PSR_DF_HALT)) ok = false;
+        }
+        if (!ok) continue;
         float ms = 0;
-        PSR_HIP(c, hipEventElapsedTime(&ms, c->gtune_ev[2 * k], c->gtune_ev[2 * k + 1]));
-        best[k & 1] = std::min(best[k & 1], ms);
+        PSR_HIP(c, hipEventElapsedTime(&ms, c->gtune_ev[k - 1], c->gtune_ev[k]));
+        best[(k >> 1) & 1] = std::min(best[(k >> 1) & 1], ms);
     }
     if (!(best[0] < INFINITY && best[1] < INFINITY)) {
         c->gtune_n = 0;
@@ -982,12 +1006,57 @@ static int gtune_decide(psr_ctx *c, bool block) {
     return PSR_OK;
 }
 
+// end of a tick: its flag word (with bit 8 set, so 0 means "not yet") into
+// the pinned ring slot the host polls
+__global__ void k_publish(const DevTick *__restrict__ tk) {
+    if (threadIdx.x == 0) *(volatile int32_t *)tk->slot = 0x100 | tk->flags;
+}
+
+// the update of the tick on stream2, after the tick's deposit, and its
+// publication; the main stream goes on with the next tick's trace and sort
+// and waits for this update only before the next deposit (psr_join), which
+// reads the particle view the update writes and adds into the sums it clears
+static int launch_overlapped_update(psr_ctx *c, const PendingTick &t) {
+    int rc;
+    // allocations (and the syncs they need) before the fork
+    if ((rc = psr_prepare_update(c))) return rc;
+    if (!c->stream2) {
+        PSR_HIP(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+        PSR_HIP(c, hipEventCreateWithFlags(&c->ev_dep, hipEventDisableTiming));
+        PSR_HIP(c, hipEventCreateWithFlags(&c->ev_upd, hipEventDisableTiming));
+    }
+    PSR_HIP(c, hipEventRecord(c->ev_dep, c->stream));
+    PSR_HIP(c, hipStreamWaitEvent(c->stream2, c->ev_dep, 0));
+    const hipStream_t main = c->stream;
+    c->stream = c->stream2;
+    if (t.chem_mode != PSR_CHEM_EXACT) rc = psr_mlp_update_dev(c, t.chem_mode == PSR_CHEM_SURROGATE_STIFF);
+    else rc = psr_launch_update(c);
+    if (!rc) {
+        k_publish<<<1, 64, 0, c->stream>>>(c->d_tick);
+        if (hipGetLastError() != hipSuccess) {
+            c->err = "launching the tick's publication";
+            rc = PSR_ERR_HIP;
+        }
+    }
+    c->stream = main;
+    if (rc) return rc;
+    PSR_HIP(c, hipEventRecord(c->ev_upd, c->stream2));
+    c->upd_pending = true;
+    c->tick_end = c->stream2;
+    return PSR_OK;
+}
+
 // the kernels of one tick; sync = wait for the trace's capacity check.
 // set: write the tick record first (a recorded graph leaves that to its
-// launch, since the record is what changes from tick to tick)
-static int tick_body(psr_ctx *c, const PendingTick &t, bool sync, bool set = true) {
+// launch, since the record is what changes from tick to tick).  publish:
+// end with the tick's flag word in its ring slot.  An asynchronous, untimed
+// tick launched kernel by kernel runs its update beside the next tick's
+// trace and sort (PSR_TUNE_TICK_OVERLAP).
+static int tick_body(psr_ctx *c, const PendingTick &t, bool sync, bool set = true, bool publish = false) {
     int rc;
     const int32_t *halt = sync ? nullptr : &c->d_stats->flags;
+    const bool overlap = publish && set && !sync && !c->timing && c->overlap_req != 0 &&
+                         (t.phases & PSR_PHASE_UPDATE) && c->npar > 0;
     if (set) {
         k_set_tick<<<1, 64, 0, c->stream>>>(c->d_tick, c->d_rays + t.ray_first, t.tick, t.tick_time,
                                             c->h_ring + t.slot);
@@ -997,6 +1066,8 @@ static int tick_body(psr_ctx *c, const PendingTick &t, bool sync, bool set = tru
         int ttrial;
         const int nt = trace_nt(c, &ttrial);
         if (ttrial >= 0) {
+            // a timed trace runs alone
+            if ((rc = psr_join(c))) return rc;
             if (!c->ttune_ev[0])
                 for (int k = 0; k < 2 * psr_ctx::TTUNE_N; k++) PSR_HIP(c, hipEventCreate(&c->ttune_ev[k]));
             PSR_HIP(c, hipEventRecord(c->ttune_ev[2 * ttrial], c->stream));
@@ -1010,6 +1081,7 @@ static int tick_body(psr_ctx *c, const PendingTick &t, bool sync, bool set = tru
         }
         if (t.phases & PSR_PHASE_DEPOSIT) {
             if ((rc = psr_launch_sort(c, t.nray))) return rc;
+            if ((rc = psr_join(c))) return rc;
             int trial;
             const int mode = tick_mode(c, &trial);
             if (trial >= 0) {
@@ -1025,11 +1097,18 @@ static int tick_body(psr_ctx *c, const PendingTick &t, bool sync, bool set = tru
                 c->tune_n++;
             }
         } else {
+            if ((rc = psr_join(c))) return rc;
             k_count_hits<<<(unsigned)((t.nray + 255) / 256), 256, 0, c->stream>>>(c->H.count, t.nray,
                                                                                   c->d_stats, halt);
             PSR_HIP(c, hipGetLastError());
         }
+    } else if ((rc = psr_join(c))) {
+        return rc;
     }
+    k_snap_flags<<<1, 64, 0, c->stream>>>(c->d_tick, &c->d_stats->flags);
+    PSR_HIP(c, hipGetLastError());
+    c->tick_end = c->stream;
+    if (overlap) return launch_overlapped_update(c, t);
     if (t.phases & PSR_PHASE_UPDATE) {
         if (t.chem_mode != PSR_CHEM_EXACT) rc = psr_mlp_update_dev(c, t.chem_mode == PSR_CHEM_SURROGATE_STIFF);
         else rc = psr_launch_update(c);
@@ -1037,17 +1116,15 @@ static int tick_body(psr_ctx *c, const PendingTick &t, bool sync, bool set = tru
     } else if (c->npar > 0) {
         // merge without update: the active set is dropped (traceOnly); an
         // overflowed tick's replay marks its particles again
-        k_fill_u8<<<(unsigned)std::min<int64_t>((c->npar + 255) / 256, 4096), 256, 0, c->stream>>>(c->d_active,
-                                                                                                 c->npar);
+        k_fill_u8<<<(unsigned)std::min<int64_t>((c->npar + 255) / 256, 4096), 256, 0, c->stream>>>(
+            c->active_of(c->par), c->npar);
+        PSR_HIP(c, hipGetLastError());
+    }
+    if (publish) {
+        k_publish<<<1, 64, 0, c->stream>>>(c->d_tick);
         PSR_HIP(c, hipGetLastError());
     }
     return PSR_OK;
-}
-
-// end of a tick: its flag word (with bit 8 set, so 0 means "not yet") into
-// the pinned ring slot the host polls
-__global__ void k_publish(const int32_t *__restrict__ flags, const DevTick *__restrict__ tk) {
-    if (threadIdx.x == 0) *(volatile int32_t *)tk->slot = 0x100 | *(volatile const int32_t *)flags;
 }
 
 void psr_graph_reset(psr_ctx *c) {
@@ -1106,16 +1183,17 @@ static int tick_graph(psr_ctx *c, const PendingTick &t, bool *done, int *trial) 
     if ((rc = psr_ensure_hits(c, t.nray, cap0))) return rc;
     if (!graph_usable(c, t)) return PSR_OK;
     if (c->graph_req < 0 && c->graph_use < 0) {
-        // a trial tick: even trials run kernel by kernel
+        // a trial tick: two kernel by kernel, two as graph launches, twice
         if (c->gtune_n >= psr_ctx::TUNE_N) return PSR_OK;
         *trial = c->gtune_n;
         if (!c->gtune_ev[0])
-            for (int k = 0; k < 2 * psr_ctx::TUNE_N; k++) PSR_HIP(c, hipEventCreate(&c->gtune_ev[k]));
-        if (!(*trial & 1)) {
-            PSR_HIP(c, hipEventRecord(c->gtune_ev[2 * *trial], c->stream));
-            return PSR_OK;
-        }
+            for (int k = 0; k < psr_ctx::TUNE_N; k++) PSR_HIP(c, hipEventCreate(&c->gtune_ev[k]));
+        if (!((*trial >> 1) & 1)) return PSR_OK;
     }
+    // a graph tick runs after the previous tick's update, on the first tick
+    // record and active array (the ones its kernels were recorded with)
+    if ((rc = psr_join(c))) return rc;
+    set_parity(c, 0);
     int atrial;
     const int mode = tick_mode(c, &atrial);
     if ((rc = psr_sync_dev_ctx(c))) return rc;
@@ -1125,8 +1203,7 @@ static int tick_graph(psr_ctx *c, const PendingTick &t, bool *done, int *trial) 
     if (!c->graph_exec || key != c->graph_key) {
         psr_graph_reset(c);
         PSR_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        rc = tick_body(c, t, false, false);
-        if (!rc) k_publish<<<1, 64, 0, c->stream>>>(&c->d_stats->flags, c->d_tick);
+        rc = tick_body(c, t, false, false, true);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (!rc && e != hipSuccess) {
@@ -1142,16 +1219,17 @@ static int tick_graph(psr_ctx *c, const PendingTick &t, bool *done, int *trial) 
         if (rc) return rc;
         c->graph_key = key;
     }
-    if (*trial >= 0) PSR_HIP(c, hipEventRecord(c->gtune_ev[2 * *trial], c->stream));
     k_set_tick<<<1, 64, 0, c->stream>>>(c->d_tick, c->d_rays + t.ray_first, t.tick, t.tick_time, c->h_ring + t.slot);
     PSR_HIP(c, hipGetLastError());
     PSR_HIP(c, hipGraphLaunch(c->graph_exec, c->stream));
+    c->tick_end = c->stream;
     c->graph_launches++;
     *done = true;
     return PSR_OK;
 }
 
 int psr_resolve(psr_ctx *c) {
+    if (int rc = psr_join(c)) return rc;
     // no tick pending: every trial tick was published, its events are done
     if (c->pending.empty()) {
         int rc = tune_decide(c, false);
@@ -1174,7 +1252,12 @@ int psr_resolve(psr_ctx *c) {
     }
     if (todo.empty()) return PSR_OK;
     PSR_HIP(c, hipMemsetAsync(&c->d_stats->flags, 0, sizeof(int32_t), c->stream));
+    // a halted trace may have marked part of its particles (in either array):
+    // the replays mark them again
+    if (c->d_active)
+        PSR_HIP(c, hipMemsetAsync(c->d_active, 0, 2 * (size_t)(c->npar > 0 ? c->npar : 1), c->stream));
     for (const PendingTick &t : todo) {
+        set_parity(c, (int)(t.slot & 1));
         const int rc = tick_body(c, t, true);
         if (rc) return rc;
     }
@@ -1208,14 +1291,18 @@ extern "C" int psr_tick(psr_ctx *c, int64_t ray_first, int64_t nray, int64_t tic
         PSR_HIP(c, hipHostMalloc((void **)&c->h_ring, sizeof(int32_t) * PSR_RING, hipHostMallocCoherent));
         memset(c->h_ring, 0, sizeof(int32_t) * PSR_RING);
     }
-    if (!c->d_tick) PSR_HIP(c, hipMalloc(&c->d_tick, sizeof(DevTick)));
+    if (!c->d_tick_buf) {
+        PSR_HIP(c, hipMalloc(&c->d_tick_buf, 2 * sizeof(DevTick)));
+        set_parity(c, 0);
+    }
     if ((rc = poll_pending(c))) return rc;
     tune_file_lookup(c);
     if ((rc = tune_decide(c, false))) return rc;
     if ((rc = ttune_decide(c, false))) return rc;
     if ((rc = gtune_decide(c, false))) return rc;
     if (c->pending.size() >= PSR_RING - 1 && (rc = psr_resolve(c))) return rc;
-    PendingTick t{ray_first, nray, tick, tick_time, phases, chem_mode, c->seq++ % PSR_RING};
+    const uint32_t seq = c->seq++;
+    PendingTick t{ray_first, nray, tick, tick_time, phases, chem_mode, seq % PSR_RING};
     ((volatile int32_t *)c->h_ring)[t.slot] = 0;
     c->ev_mask = 0;
     PSR_EV(c, 4, 0);
@@ -1223,13 +1310,14 @@ extern "C" int psr_tick(psr_ctx *c, int64_t ray_first, int64_t nray, int64_t tic
     int gtrial = -1;
     if ((rc = tick_graph(c, t, &graphed, &gtrial))) return rc;
     if (!graphed) {
-        if ((rc = tick_body(c, t, false))) return rc;
-        k_publish<<<1, 64, 0, c->stream>>>(&c->d_stats->flags, c->d_tick);
-        PSR_HIP(c, hipGetLastError());
+        // consecutive ticks alternate records and active arrays
+        set_parity(c, (int)(t.slot & 1));
+        if ((rc = tick_body(c, t, false, true, true))) return rc;
     }
     if (gtrial >= 0) {
-        PSR_HIP(c, hipEventRecord(c->gtune_ev[2 * gtrial + 1], c->stream));
+        PSR_HIP(c, hipEventRecord(c->gtune_ev[gtrial], c->tick_end));
         c->gtune_slot[gtrial] = (int32_t)t.slot;
+        c->gtune_seq[gtrial] = seq;
         c->gtune_n++;
     }
     c->pending.push_back(t);
@@ -1250,6 +1338,7 @@ extern "C" int psr_tick(psr_ctx *c, int64_t ray_first, int64_t nray, int64_t tic
 extern "C" int psr_set_timing(psr_ctx *c, int enable) {
     if (!c) return PSR_ERR_ARG;
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     if (enable && !c->ev[0])
         for (int k = 0; k < 10; k++) PSR_HIP(c, hipEventCreate(&c->ev[k]));
     c->timing = enable != 0;
@@ -1283,6 +1372,12 @@ extern "C" int psr_set_tuning(psr_ctx *c, int32_t param, int32_t value) {
         c->graph_req = value;
         c->gtune_n = 0;
         c->graph_use = -1;
+    } else if (param == PSR_TUNE_TICK_OVERLAP) {
+        if (value != 0 && value != 1) PSR_FAIL(c, PSR_ERR_ARG, "tick overlap: 0 or 1");
+        c->overlap_req = value;
+        // kernel-by-kernel ticks change speed: the graph trials start again
+        c->gtune_n = 0;
+        c->graph_use = -1;
     } else {
         PSR_FAIL(c, PSR_ERR_ARG, "unknown tuning parameter");
     }
@@ -1308,6 +1403,7 @@ extern "C" int psr_get_tuning(psr_ctx *c, int32_t param, int32_t *value) {
                                               : PSR_WALK_SINGLE;
     else if (param == PSR_TUNE_TICK_GRAPH) *value = c->graph_req >= 0 ? c->graph_req : c->graph_use;
     else if (param == PSR_TUNE_TRACE_THREADS) *value = c->trace_req > 0 ? c->trace_req : c->trace_use;
+    else if (param == PSR_TUNE_TICK_OVERLAP) *value = c->overlap_req;
     else if (param == PSR_TUNE_GRAPH_LAUNCHES) *value = (int32_t)std::min<int64_t>(c->graph_launches, INT32_MAX);
     else PSR_FAIL(c, PSR_ERR_ARG, "unknown tuning parameter");
     return PSR_OK;
@@ -1323,6 +1419,7 @@ extern "C" int psr_ray_results(psr_ctx *c, int64_t nray, double *length, double 
     if (!c) return PSR_ERR_ARG;
     if (nray > c->last_nray) PSR_FAIL(c, PSR_ERR_ARG, "more rays than the last tick traced");
     PSR_HIP(c, hipSetDevice(c->device));
+    if (int rc_j = psr_join(c)) return rc_j;
     if (int rc = psr_resolve(c)) return rc;
     PSR_HIP(c, hipStreamSynchronize(c->stream));
     if (length) PSR_HIP(c, hipMemcpy(length, c->d_ray_len, sizeof(double) * nray, hipMemcpyDeviceToHost));

@@ -485,17 +485,20 @@ def test_deposit_accumulation_is_tuned(ctx):
         assert_f32_sums(getattr(out, fld), nt * getattr(ref, fld).astype(np.float64), nt * ref.hits, fld)
 
 
-@pytest.mark.parametrize("accum,walk", [(psr.PSR_ACCUM_AUTO, psr.PSR_WALK_AUTO),
-                                        (psr.PSR_ACCUM_ATOMIC, psr.PSR_WALK_SINGLE),
-                                        (psr.PSR_ACCUM_ATOMIC, psr.PSR_WALK_MULTI)])
-def test_overflowed_tick_is_replayed(accum, walk):
+@pytest.mark.parametrize("accum,walk,overlap", [(psr.PSR_ACCUM_AUTO, psr.PSR_WALK_AUTO, 1),
+                                                (psr.PSR_ACCUM_ATOMIC, psr.PSR_WALK_SINGLE, 1),
+                                                (psr.PSR_ACCUM_ATOMIC, psr.PSR_WALK_MULTI, 1),
+                                                (psr.PSR_ACCUM_AUTO, psr.PSR_WALK_AUTO, 0)])
+def test_overflowed_tick_is_replayed(accum, walk, overlap):
     """ticks are launched without waiting for the trace's capacity check; a
     tick whose lists outgrow the capacity (2048 hits per ray at first) stops
     every later kernel and is replayed, with the ticks after it, at the next
-    call -- the result equals three sequential oracle ticks.  With a fixed
-    accumulation the ticks run as recorded graphs, including the overflowed
-    one."""
-    print_case("overflow-replay", accum, walk)
+    call -- the result equals three sequential oracle ticks.  With overlap
+    the second tick's trace overflows while the first tick's update runs
+    beside it: that update still completes, and only the ticks from the
+    second on are replayed.  With a fixed accumulation the ticks are the
+    tick-graph trials (kernel by kernel, then a graph launch)."""
+    print_case("overflow-replay", accum, walk, overlap)
     wl = make_workload(1, RATES, n=104, nray=16)
     box = wl.boxsize
     r1, r2, r3 = wl.rays(1), wl.rays(2), wl.rays(3)
@@ -510,6 +513,7 @@ def test_overflowed_tick_is_replayed(accum, walk):
     with psr.Context(0) as c2:
         c2.set_tuning(psr.PSR_TUNE_DEPOSIT_ACCUM, accum)
         c2.set_tuning(psr.PSR_TUNE_DEPOSIT_WALK, walk)
+        c2.set_tuning(psr.PSR_TUNE_TICK_OVERLAP, overlap)
         load(c2, wl)
         c2.rays_upload(pool)
         for t, (a, rr) in enumerate(((0, r1), (len(r1), r2), (len(r1) + len(r2), r3)), start=1):
@@ -537,24 +541,27 @@ def test_overflowed_tick_is_replayed(accum, walk):
                                             (3, psr.PSR_ACCUM_BINNED, psr.PSR_WALK_SINGLE),
                                             (3, psr.PSR_ACCUM_ATOMIC, psr.PSR_WALK_MULTI)])
 def test_tick_graph_matches_kernel_by_kernel(cfg, accum, walk):
-    """ten ticks recorded once as a hipGraph and replayed with each tick's
-    record against the same ten ticks launched kernel by kernel, from the
-    same state.  Two runs differ only through the f32 summation order of the
-    deposits, which moves fractions in the last bits and so, over ten ticks,
-    which hit ends a few rays at the 1e-10 cut-off: full hit counts are
-    equal, per-particle hits and last hits nearly everywhere, fractions and
-    energies to rounding."""
+    """ten ticks launched three ways from the same state: kernel by kernel on
+    one stream, kernel by kernel with each update beside the next tick's
+    trace and sort (two tick records and active arrays alternating), and
+    recorded once as a hipGraph replayed with each tick's record.  Runs
+    differ only through the f32 summation order of the deposits, which moves
+    fractions in the last bits and so, over ten ticks, which hit ends a few
+    rays at the 1e-10 cut-off: full hit counts are equal, per-particle hits
+    and last hits nearly everywhere, fractions and energies to rounding."""
     print_case("tick-graph", cfg, accum, walk)
     wl = make_workload(cfg, RATES, n=24 if cfg == 2 else 16, nray=256)
     ticks = [wl.rays(t) for t in range(1, 11)]
     pool = np.concatenate(ticks)
     firsts = np.concatenate([[0], np.cumsum([len(r) for r in ticks])])
     out = {}
-    for graph in (0, 1):
+    for way, graph, overlap in (("serial", 0, 0), ("overlap", 0, 1), ("graph", 1, 1)):
         with psr.Context(0) as c2:
             c2.set_tuning(psr.PSR_TUNE_DEPOSIT_ACCUM, accum)
             c2.set_tuning(psr.PSR_TUNE_DEPOSIT_WALK, walk)
             c2.set_tuning(psr.PSR_TUNE_TICK_GRAPH, graph)
+            c2.set_tuning(psr.PSR_TUNE_TICK_OVERLAP, overlap)
+            assert c2.tuning(psr.PSR_TUNE_TICK_OVERLAP) == overlap
             c2.set_tuning(psr.PSR_TUNE_TRACE_THREADS, 256 if cfg == 2 else 128)
             load(c2, wl)
             c2.rays_upload(pool)
@@ -562,26 +569,29 @@ def test_tick_graph_matches_kernel_by_kernel(cfg, accum, walk):
                 c2.tick(t, wl.tick_time, ray_first=int(firsts[t - 1]), nray=len(ticks[t - 1]))
             ps = wl.ps.copy()
             c2.state_download(ps)
-            out[graph] = (ps, c2.stats(), c2.tuning(psr.PSR_TUNE_GRAPH_LAUNCHES))
-    (a, sa, ga), (b, sb, gb) = out[0], out[1]
-    assert ga == 0 and gb >= 8                     # the first tick(s) may size the hit slots
-    assert sa["total_deposit_count"] == sb["total_deposit_count"]
-    # (measured: 0.1% of particles; a stale tick record would change most)
-    assert np.mean(a.hits != b.hits) < 1e-2 and np.mean(a.lasthit != b.lasthit) < 1e-2
-    for f in ("lambdaH", "lambdaHeI", "lambdaHeII"):
-        va, vb = getattr(a, f), getattr(b, f)
-        d = np.abs(va - vb)
-        if wl.physics.flags.isothermal:
-            # closed-form update: last-bit differences stay last-bit
-            assert d.max() < 1e-5, f
-        else:
-            # an adaptive step decision can flip on a last-bit difference (see
-            # test_full_tick_parity), which moves that particle's result within
-            # the integrator's tolerance (rtol 1e-4); bounded by the 1e-3 bar
-            assert np.median(d) < 1e-9, f
-            assert (d <= 1e-3 * np.maximum(np.abs(va), np.abs(vb)) + 1e-6).all(), f
-    dr = np.abs(a.ie / b.ie - 1)
-    assert np.median(dr) < 1e-6 and dr.max() < 1e-2
+            out[way] = (ps, c2.stats(), c2.tuning(psr.PSR_TUNE_GRAPH_LAUNCHES))
+    a, sa, ga = out["serial"]
+    assert ga == 0 and out["overlap"][2] == 0 and out["graph"][2] >= 8  # the first tick(s) may size the hit slots
+    for way in ("overlap", "graph"):
+        b, sb, _ = out[way]
+        assert sa["total_deposit_count"] == sb["total_deposit_count"], way
+        # (measured: 0.1% of particles; a stale tick record would change most)
+        assert np.mean(a.hits != b.hits) < 1e-2 and np.mean(a.lasthit != b.lasthit) < 1e-2, way
+        for f in ("lambdaH", "lambdaHeI", "lambdaHeII"):
+            va, vb = getattr(a, f), getattr(b, f)
+            d = np.abs(va - vb)
+            if wl.physics.flags.isothermal:
+                # closed-form update: last-bit differences stay last-bit
+                assert d.max() < 1e-5, (way, f)
+            else:
+                # an adaptive step decision can flip on a last-bit difference
+                # (see test_full_tick_parity), which moves that particle's
+                # result within the integrator's tolerance (rtol 1e-4);
+                # bounded by the 1e-3 bar
+                assert np.median(d) < 1e-9, (way, f)
+                assert (d <= 1e-3 * np.maximum(np.abs(va), np.abs(vb)) + 1e-6).all(), (way, f)
+        dr = np.abs(a.ie / b.ie - 1)
+        assert np.median(dr) < 1e-6 and dr.max() < 1e-2, way
 
 
 @pytest.mark.parametrize("cfg,n,nray", [(1, 32, 32), (2, 24, 256), (3, 16, 128)])
