@@ -351,7 +351,8 @@ def main():
     main_mode = "exact" if exact_timed else "surrogate"
     wall, st, graph_launches = res[main_mode]
     # per-phase pass: HIP events on the context stream around each kernel
-    # (they synchronise every tick, so they are kept out of the timed run)
+    # (they synchronise every tick and keep its phases in order, so they are
+    # kept out of the timed run)
     next_tick[0] = t_first
     ctx.state_upload(snap)
     ctx.reset_stats()
@@ -406,6 +407,9 @@ def main():
         "deposit_walk": walk,
         "trace_threads_per_ray": ctx_trace_threads,
         "tick_launch": graph,
+        # kernel-by-kernel ticks run each update beside the next tick's trace
+        # and sort (the per-phase pass above runs them one after another)
+        "tick_overlap": graph == "kernel by kernel" and ctx.tuning(psr.PSR_TUNE_TICK_OVERLAP) == 1,
         "graph_launches_timed": graph_launches,
         "surrogate": sur,
         "roofline": {
