@@ -202,7 +202,7 @@ extern "C" void psr_destroy(psr_ctx *c) {
         if (c->gtune_ev[k]) (void)hipEventDestroy(c->gtune_ev[k]);
     if (c->ev_dep) (void)hipEventDestroy(c->ev_dep);
     if (c->ev_upd) (void)hipEventDestroy(c->ev_upd);
-    for (int k = 0; k < 2 * psr_ctx::TTUNE_N; k++)
+    for (int k = 0; k < 4 * psr_ctx::TTUNE_N; k++)
         if (c->ttune_ev[k]) (void)hipEventDestroy(c->ttune_ev[k]);
     psr_graph_reset(c);
     psr_emit_free(c);

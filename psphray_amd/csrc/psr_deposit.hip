@@ -1136,6 +1136,11 @@ __global__ __launch_bounds__(MTHREADS, MR == 2 ? 4 : 3) void k_deposit_multi(con
 #pragma unroll
                 for (int h = 0; h < HPL; h++) {
                     const int j = h * 64 + lane, p = j / MSEG, k = j % MSEG;
+                    // a segment past its ray's last (the list's end, or the
+                    // walk's cut, which only lowers it) is never walked or
+                    // applied: the pipeline's last two iterations and a cut
+                    // ray's rest cost no prep
+                    if (i > L.endseg[p]) continue;
                     WalkRec<HE> rr;
                     HitAux<HE> au;
                     const RayConst &kc = L.kc[p];

@@ -333,13 +333,13 @@ struct psr_ctx {
     int32_t tune_mode[DTUNE_N] = {};
     int tune_n = 0;
     // traversal threads per ray: requested (-1: timed trials, 128 or 256),
-    // chosen, and the trials (trace launches alternating 256 and 128)
+    // chosen, and the trials (ticks that trace their rays with both shapes,
+    // an event pair around each launch)
     int32_t trace_req = -1;
     int32_t trace_use = -1;
-    static constexpr int TTUNE_N = 4;
-    hipEvent_t ttune_ev[2 * TTUNE_N] = {};
+    static constexpr int TTUNE_N = 2;
+    hipEvent_t ttune_ev[4 * TTUNE_N] = {};
     int32_t ttune_slot[TTUNE_N] = {};
-    int32_t ttune_nt[TTUNE_N] = {};
     int ttune_n = 0;
     bool tune_file_tried = false;  // PSR_TUNING_FILE looked up for this workload
 

@@ -842,42 +842,53 @@ extern "C" int psr_tuning_load(psr_ctx *c, const char *path) {
 }
 
 // threads per ray of this tick's trace: the requested or decided one, else
-// the next trial (256 and 128 alternating); *trial its index or -1
+// (a trial tick, *trial its index) the shape traced second; -1 otherwise
 static int trace_nt(const psr_ctx *c, int *trial) {
     *trial = -1;
     if (c->trace_req > 0) return c->trace_req;
     if (c->trace_use > 0) return c->trace_use;
     if (c->ttune_n < psr_ctx::TTUNE_N) {
         *trial = c->ttune_n;
-        return (c->ttune_n & 1) ? 128 : 256;
+        return (c->ttune_n & 1) ? 256 : 128;
     }
     return 256;
 }
 
-// the trace trials: the faster shape's best launch wins (an overflowed
-// asynchronous tick's trial does not count)
+// the trace trials: each trial tick traces its rays with both shapes, 256
+// threads first on even trials and 128 first on odd ones (the second
+// launch writes the same hit lists), so both are timed on the same rays;
+// the shape with the smaller total wins.  The second launch of a tick finds
+// part of the first one's lines still cached, so the decision needs both
+// orders: a trial on an overflowed asynchronous tick does not count, and
+// the trials start again unless both do.  (Single launches on different
+// ticks compared rays of different lengths: 256^3 chose 128 threads in 2 of
+// 19 runs that way, trace 2.55 ms instead of 2.30.)
 static int ttune_decide(psr_ctx *c, bool block) {
     if (c->trace_req > 0 || c->trace_use > 0 || c->ttune_n < psr_ctx::TTUNE_N) return PSR_OK;
-    const hipEvent_t last = c->ttune_ev[2 * psr_ctx::TTUNE_N - 1];
+    const hipEvent_t last = c->ttune_ev[4 * psr_ctx::TTUNE_N - 1];
     if (block) PSR_HIP(c, hipEventSynchronize(last));
     else if (hipEventQuery(last) != hipSuccess) return PSR_OK;
-    float best[2] = {INFINITY, INFINITY};
+    float sum[2] = {0.0f, 0.0f};
+    int valid = 0;
     for (int k = 0; k < psr_ctx::TTUNE_N; k++) {
         if (c->ttune_slot[k] >= 0) {
             const int32_t v = ((volatile int32_t *)c->h_ring)[c->ttune_slot[k]];
             if (v == 0 && !block) return PSR_OK;
             if (v == 0 || (v & PSR_DF_HALT)) continue;
         }
-        float ms = 0;
-        PSR_HIP(c, hipEventElapsedTime(&ms, c->ttune_ev[2 * k], c->ttune_ev[2 * k + 1]));
-        const int b = c->ttune_nt[k] == 128 ? 1 : 0;
-        best[b] = std::min(best[b], ms);
+        for (int l = 0; l < 2; l++) {
+            float ms = 0;
+            PSR_HIP(c, hipEventElapsedTime(&ms, c->ttune_ev[4 * k + 2 * l], c->ttune_ev[4 * k + 2 * l + 1]));
+            // launch l of trial k: 256 threads for (k + l) even
+            sum[(k + l) & 1] += ms;
+        }
+        valid++;
     }
-    if (!(best[0] < INFINITY && best[1] < INFINITY)) {
+    if (valid < psr_ctx::TTUNE_N) {
         c->ttune_n = 0;
         return PSR_OK;
     }
-    c->trace_use = best[1] < best[0] ? 128 : 256;
+    c->trace_use = sum[1] < sum[0] ? 128 : 256;
     tune_file_store(c);
     return PSR_OK;
 }
@@ -1064,19 +1075,27 @@ static int tick_body(psr_ctx *c, const PendingTick &t, bool sync, bool set = tru
     }
     if (t.nray > 0) {
         int ttrial;
-        const int nt = trace_nt(c, &ttrial);
+        int nt = trace_nt(c, &ttrial);
+        if (sync && ttrial >= 0) {
+            // a replay's trace may retrace after growing the hit slots: no
+            // trial there
+            ttrial = -1;
+            nt = 256;
+        }
         if (ttrial >= 0) {
-            // a timed trace runs alone
+            // a timed trace runs alone, first with the other shape
             if ((rc = psr_join(c))) return rc;
             if (!c->ttune_ev[0])
-                for (int k = 0; k < 2 * psr_ctx::TTUNE_N; k++) PSR_HIP(c, hipEventCreate(&c->ttune_ev[k]));
-            PSR_HIP(c, hipEventRecord(c->ttune_ev[2 * ttrial], c->stream));
+                for (int k = 0; k < 4 * psr_ctx::TTUNE_N; k++) PSR_HIP(c, hipEventCreate(&c->ttune_ev[k]));
+            PSR_HIP(c, hipEventRecord(c->ttune_ev[4 * ttrial], c->stream));
+            if ((rc = psr_launch_trace(c, t.ray_first, t.nray, sync, c->d_tick, nt == 256 ? 128 : 256))) return rc;
+            PSR_HIP(c, hipEventRecord(c->ttune_ev[4 * ttrial + 1], c->stream));
+            PSR_HIP(c, hipEventRecord(c->ttune_ev[4 * ttrial + 2], c->stream));
         }
         if ((rc = psr_launch_trace(c, t.ray_first, t.nray, sync, c->d_tick, nt))) return rc;
         if (ttrial >= 0) {
-            PSR_HIP(c, hipEventRecord(c->ttune_ev[2 * ttrial + 1], c->stream));
+            PSR_HIP(c, hipEventRecord(c->ttune_ev[4 * ttrial + 3], c->stream));
             c->ttune_slot[ttrial] = sync ? -1 : (int32_t)t.slot;
-            c->ttune_nt[ttrial] = nt;
             c->ttune_n++;
         }
         if (t.phases & PSR_PHASE_DEPOSIT) {
@@ -1085,6 +1104,9 @@ static int tick_body(psr_ctx *c, const PendingTick &t, bool sync, bool set = tru
             int trial;
             const int mode = tick_mode(c, &trial);
             if (trial >= 0) {
+                // the view build and context upload a deposit may need
+                // first stay out of its timing
+                if ((rc = psr_prepare_deposit(c, mode))) return rc;
                 if (!c->tune_ev[0])
                     for (int k = 0; k < 2 * psr_ctx::DTUNE_N; k++) PSR_HIP(c, hipEventCreate(&c->tune_ev[k]));
                 PSR_HIP(c, hipEventRecord(c->tune_ev[2 * trial], c->stream));
