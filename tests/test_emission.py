@@ -157,6 +157,39 @@ def test_tick_with_device_emitted_rays():
 
 
 @pytest.mark.gpu
+def test_back_to_back_emitted_ticks_equal_oracle():
+    """five ticks issued back to back, each over rays the device emits into
+    one of three pool ranges: each tick's update runs beside the next
+    tick's emission, trace and sort (source rays do not wait for it); the
+    state equals five sequential oracle ticks over the same rays (hit
+    counts and last hits nearly everywhere, fractions to rounding: the
+    closed-form update over deposits summed in another f32 order)"""
+    print_case("back-to-back emitted ticks")
+    wl = make_workload(2, RATES, n=24, nray=512)
+    oe = oracle_emitter(wl)
+    room = 2 * wl.nray
+    with _device_context(wl) as ctx:
+        assert ctx.tuning(psr.PSR_TUNE_TICK_OVERLAP) == 1
+        for t in range(1, 6):
+            first = (t % 3) * room
+            n = ctx.emit_rays(t, wl.nray, pool_first=first)
+            assert 0 < n <= room
+            ctx.tick(t, wl.tick_time, ray_first=first, nray=n)
+        out = ctx.state_download(wl.ps.copy())
+        s = ctx.stats()
+    from test_gpu_parity import oracle_for
+    orc = oracle_for(wl)
+    ref = wl.ps.copy()
+    tree = po.Tree(ref.pos, ref.sml, wl.boxsize)
+    for t in range(1, 6):
+        rc, _ = po.tick(orc, tree, ref, oe.emit(t, wl.nray), t, wl.tick_time)
+        assert rc == 0
+    assert s["total_deposit_count"] > 0
+    assert np.mean(out.hits != ref.hits) < 1e-2 and np.mean(out.lasthit != ref.lasthit) < 1e-2
+    assert np.abs(out.lambdaH - ref.lambdaH).max() < 1e-5
+
+
+@pytest.mark.gpu
 def test_emission_argument_checks():
     """ticks must increase; a source's field must exist in the tree"""
     print_case("emission checks")
