@@ -138,6 +138,7 @@ __device__ __forceinline__ void noop_rec(WalkRec<HE> &r, HitAux<HE> &a) {
 // per-ray constants
 struct RayConst {
     double sigmaHI, sigmaHeI, sigmaHeII, dEHI, dEHeI, dEHeII, hf_HI, hf_HeI, hf_HeII;
+    double sHI_HI, sHeI_HI, sHI_HeI, sHeI_HeI, rCH, rCHe;
     bool secion_terms;
 };
 
@@ -154,6 +155,15 @@ __device__ __forceinline__ RayConst ray_const(double freq, const DevTables &T, c
     k.hf_HeI = P.C_HE_PER_MASS * k.dEHeI;
     k.hf_HeII = P.C_HE_PER_MASS * k.dEHeII;
     k.secion_terms = P.secondary_ionization && (k.dEHI > 0 || k.dEHeI > 0) && T.si_n1 > 0;
+    k.sHI_HI = k.sHeI_HI = k.sHI_HeI = k.sHeI_HeI = k.rCH = k.rCHe = 0.0;
+    if constexpr (HE) {
+        k.sHI_HI = k.dEHI / P.C_HI_ENERGY;
+        k.sHeI_HI = k.dEHeI / P.C_HI_ENERGY;
+        k.sHI_HeI = k.dEHI / P.C_HEI_ENERGY;
+        k.sHeI_HeI = k.dEHeI / P.C_HEI_ENERGY;
+        k.rCH = 1.0 / P.C_H_PER_MASS;
+        k.rCHe = 1.0 / P.C_HE_PER_MASS;
+    }
     return k;
 }
 
@@ -345,8 +355,20 @@ __device__ __forceinline__ DepRec<HE> apply_core(double tm, const WalkRec<HE> &v
     const double mass = (double)massf;
     const double NH = mass * P.C_H_PER_MASS;
     const double NHe = mass * P.C_HE_PER_MASS;
-    const double NH_inv = 1.0 / NH;
-    const double NHe_inv = 1.0 / NHe;
+    // with helium, the per-hit divisions (six per hit: 1/NH, 1/NHe and the
+    // four (delta dE) / E_ion of the secondary terms) become products with
+    // per-ray reciprocals and one reciprocal of the mass; the values move
+    // by an FP64 rounding, below the f32 the deposits are summed in (the
+    // hydrogen-only kernel keeps the divisions, which cost it nothing)
+    double NH_inv, NHe_inv;
+    if constexpr (HE) {
+        const double rmass = 1.0 / mass;
+        NH_inv = rmass * k.rCH;
+        NHe_inv = rmass * k.rCHe;
+    } else {
+        NH_inv = 1.0 / NH;
+        NHe_inv = 1.0 / NHe;
+    }
     // the walk's operations on the TM this hit saw
     const double absorb_est = -tm * em;
     double fHI = 1.0, fHeI = 0.0, fHeII = 0.0, NHeI = 0.0, NHeII = 0.0;
@@ -386,14 +408,14 @@ __device__ __forceinline__ DepRec<HE> apply_core(double tm, const WalkRec<HE> &v
         yHI += deltaHI;
         acc.fiHI += deltaHI * NH;
         if (k.secion_terms) {
-            const double sec =
-                deltaHI * k.dEHI / P.C_HI_ENERGY * sv.vH[0] + deltaHeI * k.dEHeI / P.C_HI_ENERGY * sv.vHe[0];
+            const double sec = HE ? deltaHI * k.sHI_HI * sv.vH[0] + deltaHeI * k.sHeI_HI * sv.vHe[0]
+                                  : deltaHI * k.dEHI / P.C_HI_ENERGY * sv.vH[0] +
+                                        deltaHeI * k.dEHeI / P.C_HI_ENERGY * sv.vHe[0];
             yHI += sec;
             acc.siHI += sec * NH;
             ht += k.hf_HI * deltaHI * sv.vH[2];
             if (he) {
-                const double sec2 =
-                    deltaHI * k.dEHI / P.C_HEI_ENERGY * sv.vH[1] + deltaHeI * k.dEHeI / P.C_HEI_ENERGY * sv.vHe[1];
+                const double sec2 = deltaHI * k.sHI_HeI * sv.vH[1] + deltaHeI * k.sHeI_HeI * sv.vHe[1];
                 yHeI += deltaHeI + sec2;
                 acc.siHeI += sec2 * NHe;
                 acc.fiHeI += deltaHeI * NHe;
