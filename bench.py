@@ -30,16 +30,20 @@ HBM_PEAK_GBS = 8000.0
 BYTES_TRACE = 16
 BYTES_DEPOSIT_H = 36
 BYTES_DEPOSIT_HE = 68
-# HBM bytes per k_deposit launch from rocprofv3 FETCH_SIZE + WRITE_SIZE
-# passes, scaled by the same counters on a copy of known size (FETCH_SIZE x 2,
-# WRITE_SIZE x 1), the timed-tick dispatch of each workload (the 128^3,
-# 256^3 and batched-field traffic summaries under profiles/)
-TRAFFIC_DEPOSIT = {(2, "k_deposit"): (84968 * 2 + 50806) * 1024,
-                   (2, "k_deposit_multi"): (55818 * 2 + 47923) * 1024,       # two rays per workgroup
-                   (3, "k_deposit_multi"): (3420537 * 2 + 5079307) * 1024,   # two rays per workgroup
-                   (5, "k_deposit_multi"): (91585 * 2 + 81476) * 1024,       # four rays per workgroup
-                   (3, "k_deposit"): (5984385 * 2 + 4237966) * 1024,
-                   (5, "k_deposit"): (151006 * 2 + 92589) * 1024}
+# HBM bytes per deposit launch from rocprofv3 FETCH_SIZE + WRITE_SIZE passes,
+# scaled by the same counters on a copy of known size (FETCH_SIZE x 2,
+# WRITE_SIZE x 1), the last dispatch of each several-ray kernel (its LDS
+# size names it) in the traffic passes of each workload
+# (profiles/k_trace_and_deposit_traffic.md); other kernels: null
+TRAFFIC_DEPOSIT = {(2, "k_deposit_multi<false, 2>"): (56229 * 2 + 48219) * 1024,
+                   (2, "k_deposit_multi<false, 4>"): (55851 * 2 + 47922) * 1024,
+                   (2, "k_deposit_multi<false, 8>"): (55127 * 2 + 54072) * 1024,
+                   (3, "k_deposit_multi<true, 2>"): (3397351 * 2 + 4979083) * 1024,
+                   (3, "k_deposit_multi<true, 4>"): (3213322 * 2 + 5058251) * 1024,
+                   (3, "k_deposit_multi<true, 8>"): (3128020 * 2 + 6280064) * 1024,
+                   (5, "k_deposit_multi<false, 2>"): (95600 * 2 + 55917) * 1024,
+                   (5, "k_deposit_multi<false, 4>"): (88597 * 2 + 88264) * 1024,
+                   (5, "k_deposit_multi<false, 8>"): (90718 * 2 + 87264) * 1024}
 WORKLOADS = {
     1: "32^3 jittered lattice, H-only, isothermal 1e4 K, one source",
     2: "128^3 lognormal, H-only, isothermal 1e4 K, 64 sources",
@@ -372,7 +376,12 @@ def main():
     roof_achieved = st_t["total_deposit_count"] / K * bytes_dep / dep_launch_s / 1e9
     # the deposit kernel the autotuner chose; traffic is the counters' figure
     # for that kernel on this workload, where measured
-    dep_kernel = "k_deposit" if walk == "one ray per workgroup" else "k_deposit_multi"
+    he_t = "true" if he else "false"
+    if walk == "one ray per workgroup":
+        dep_kernel = f"k_deposit<{he_t}, {'true' if accum == 'binned' else 'false'}>"
+    else:
+        mr = {"two rays per workgroup": 2, "four rays per workgroup": 4, "eight rays per workgroup": 8}[walk]
+        dep_kernel = f"k_deposit_multi<{he_t}, {mr}>"
     out = {
         "metric": "ray-particle intersections/s + ticks/s, exact FP64 chemistry (value) vs "
                   "MLP-surrogate chemistry (surrogate) at <=1e-3 x_HII error",
