@@ -233,7 +233,7 @@ def main():
     surrogate = args.chem in ("both", "surrogate")
     K = args.steps
     ntrain = args.train_ticks if surrogate else 0
-    tune_extra = 24      # ticks allowed beyond --warmup for the autotuner's trials
+    tune_extra = 48      # ticks allowed beyond --warmup for the autotuner's trials (restarted trials included)
     # ray lists of every tick the run may need, resident before any timing
     nt = args.warmup + tune_extra + ntrain + 3 * K
     ray_ticks = [wl.rays(t) for t in range(1, nt + 1)]
