@@ -1396,8 +1396,17 @@ int psr_prepare_deposit(psr_ctx *c, int mode) {
     return mode == PSR_DEP_WG_BINNED ? ensure_bins(c, !c->F.h_only) : PSR_OK;
 }
 
+// the several-ray deposits' batch order (rays by descending hit count); it
+// reads only the tick's hit counts, so the tick launches it right after the
+// sort, before waiting for the previous tick's update
+int psr_launch_ray_order(psr_ctx *c, int64_t nray) {
+    k_ray_order<<<1, ORD_THREADS, 0, c->stream>>>(c->H.count, nray, c->H.order, &c->d_stats->flags);
+    PSR_HIP(c, hipGetLastError());
+    return PSR_OK;
+}
+
 // the rays of the tick come from the tick record (ray_first is kept for the
-// call's shape checks)
+// call's shape checks); the several-ray modes need psr_launch_ray_order first
 int psr_launch_deposit(psr_ctx *c, int64_t, int64_t nray, int mode) {
     int rc = psr_prepare_deposit(c, mode);
     if (rc) return rc;
@@ -1407,8 +1416,6 @@ int psr_launch_deposit(psr_ctx *c, int64_t, int64_t nray, int mode) {
         const int mr = psr_dep_rays(mode);
         const int32_t *halt = &c->d_stats->flags;
         PSR_EV(c, 2, 0);
-        k_ray_order<<<1, ORD_THREADS, 0, c->stream>>>(c->H.count, nray, c->H.order, halt);
-        PSR_HIP(c, hipGetLastError());
         const unsigned gm = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nray + mr - 1) / mr, 65535 * 4));
 #define DEP_M(HE_, MR_)                                                                                 \
     k_deposit_multi<HE_, MR_><<<gm, MTHREADS, 0, c->stream>>>(rays, nray, c->H, c->d_ctx, c->d_stats, \

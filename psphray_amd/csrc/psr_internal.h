@@ -358,6 +358,7 @@ int psr_launch_sort(psr_ctx *c, int64_t nray);
 // hit slots for nray rays of cap hits each (reallocates after a sync)
 int psr_ensure_hits(psr_ctx *c, int64_t nray, int32_t cap);
 int psr_launch_deposit(psr_ctx *c, int64_t ray_first, int64_t nray, int mode);
+int psr_launch_ray_order(psr_ctx *c, int64_t nray);
 // restart the accumulation trials (tree or physics changed)
 void psr_tune_reset(psr_ctx *c);
 // timed: record the update phase's event pair (the surrogate records its own);

@@ -1100,9 +1100,10 @@ static int tick_body(psr_ctx *c, const PendingTick &t, bool sync, bool set = tru
         }
         if (t.phases & PSR_PHASE_DEPOSIT) {
             if ((rc = psr_launch_sort(c, t.nray))) return rc;
-            if ((rc = psr_join(c))) return rc;
             int trial;
             const int mode = tick_mode(c, &trial);
+            if (mode >= PSR_DEP_MULTI && (rc = psr_launch_ray_order(c, t.nray))) return rc;
+            if ((rc = psr_join(c))) return rc;
             if (trial >= 0) {
                 // the view build and context upload a deposit may need
                 // first stay out of its timing
