@@ -434,7 +434,8 @@ int psr_get_timers(psr_ctx *ctx, double *ms);
 #define PSR_TUNE_DEPOSIT_WALK 4
 /* PSR_TUNE_TRACE_THREADS: threads per ray of the traversal, 256 (32 nodes
  * or leaves per step, a 4096-entry stack) or 128 (16, 2048); -1 (default):
- * two trace launches of each on the first asynchronous ticks decide */
+ * two trial ticks that trace their rays with both shapes (both launch
+ * orders) decide */
 #define PSR_TUNE_TRACE_THREADS 5
 #define PSR_WALK_AUTO 0
 #define PSR_WALK_SINGLE 1

@@ -1050,10 +1050,13 @@ static int launch_overlapped_update(psr_ctx *c, const PendingTick &t) {
         }
     }
     c->stream = main;
-    if (rc) return rc;
-    PSR_HIP(c, hipEventRecord(c->ev_upd, c->stream2));
+    // recorded even after a failed launch, so that every later call still
+    // waits for whatever reached stream2
+    const hipError_t e = hipEventRecord(c->ev_upd, c->stream2);
     c->upd_pending = true;
     c->tick_end = c->stream2;
+    if (rc) return rc;
+    if (e != hipSuccess) PSR_FAIL(c, PSR_ERR_HIP, std::string("recording the update's event: ") + hipGetErrorString(e));
     return PSR_OK;
 }
 
