@@ -170,6 +170,7 @@ extern "C" void psr_destroy(psr_ctx *c) {
     (void)psr_join(c);
     (void)hipStreamSynchronize(c->stream);
     if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+    if (c->stream3) (void)hipStreamSynchronize(c->stream3);
     if (c->h_ring) (void)hipHostFree(c->h_ring);
     free_tree(c);
     free_state(c);
@@ -210,6 +211,9 @@ extern "C" void psr_destroy(psr_ctx *c) {
     c->d_tick = nullptr;
     if (c->d_scratch) (void)hipFree(c->d_scratch);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
+    if (c->stream3) (void)hipStreamDestroy(c->stream3);
+    if (c->ev_tr) (void)hipEventDestroy(c->ev_tr);
+    if (c->ev_s8) (void)hipEventDestroy(c->ev_s8);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -313,6 +313,11 @@ struct psr_ctx {
     hipEvent_t ev_dep = nullptr, ev_upd = nullptr;
     bool upd_pending = false;
     hipStream_t tick_end = nullptr;  // the stream the last tick ended on
+    // an overlapped tick sorts its 8192-hit tier on stream3, beside the
+    // 4096-hit tier (their rays are disjoint); the main stream waits for it
+    bool sort_split = false;
+    hipStream_t stream3 = nullptr;
+    hipEvent_t ev_tr = nullptr, ev_s8 = nullptr;
     // the tick recorded as a hipGraph: its executable and the shape it was
     // recorded for (sizes and device pointers its kernels were given)
     int32_t graph_req = -1;  // -1: chosen by timed trials, 0 / 1: off / on

@@ -596,13 +596,33 @@ int psr_launch_sort(psr_ctx *c, int64_t nray) {
     // lists of up to TR_SORT hits were sorted by the trace; the longer ones
     // here, in LDS tiers by list length (10 B per hit + 8 B per bucket)
     static_assert(TR_SORT == 1024, "the first tier starts above the trace's sort");
-    if (c->H.cap > TR_SORT) {
+    if (c->sort_split && c->H.cap > 4096) {
+        // the 8192-hit tier on stream3 beside the 4096-hit tier: the long
+        // lists find room in the CUs the shorter tier's tail leaves idle
+        if (!c->stream3) {
+            PSR_HIP(c, hipStreamCreateWithFlags(&c->stream3, hipStreamNonBlocking));
+            PSR_HIP(c, hipEventCreateWithFlags(&c->ev_tr, hipEventDisableTiming));
+            PSR_HIP(c, hipEventCreateWithFlags(&c->ev_s8, hipEventDisableTiming));
+        }
+        PSR_HIP(c, hipEventRecord(c->ev_tr, c->stream));
+        PSR_HIP(c, hipStreamWaitEvent(c->stream3, c->ev_tr, 0));
+        k_sort<8192, 2048, SORT_NT8><<<grid, SORT_NT8, 0, c->stream3>>>(nray, c->H, c->d_orig, 4096, 8192, halt);
+        PSR_HIP(c, hipGetLastError());
+        PSR_HIP(c, hipEventRecord(c->ev_s8, c->stream3));
         k_sort<4096, 1024, SORT_NT4><<<grid, SORT_NT4, 0, c->stream>>>(nray, c->H, c->d_orig, TR_SORT, 4096, halt);
         PSR_HIP(c, hipGetLastError());
-    }
-    if (c->H.cap > 4096) {
-        k_sort<8192, 2048, SORT_NT8><<<grid, SORT_NT8, 0, c->stream>>>(nray, c->H, c->d_orig, 4096, 8192, halt);
-        PSR_HIP(c, hipGetLastError());
+        PSR_HIP(c, hipStreamWaitEvent(c->stream, c->ev_s8, 0));
+    } else {
+        if (c->H.cap > TR_SORT) {
+            k_sort<4096, 1024, SORT_NT4><<<grid, SORT_NT4, 0, c->stream>>>(nray, c->H, c->d_orig, TR_SORT, 4096,
+                                                                           halt);
+            PSR_HIP(c, hipGetLastError());
+        }
+        if (c->H.cap > 4096) {
+            k_sort<8192, 2048, SORT_NT8><<<grid, SORT_NT8, 0, c->stream>>>(nray, c->H, c->d_orig, 4096, 8192,
+                                                                           halt);
+            PSR_HIP(c, hipGetLastError());
+        }
     }
     if (c->H.cap > 8192) {
         std::vector<int32_t> cnt(nray), big;
@@ -1102,7 +1122,10 @@ static int tick_body(psr_ctx *c, const PendingTick &t, bool sync, bool set = tru
             c->ttune_n++;
         }
         if (t.phases & PSR_PHASE_DEPOSIT) {
-            if ((rc = psr_launch_sort(c, t.nray))) return rc;
+            c->sort_split = overlap;
+            rc = psr_launch_sort(c, t.nray);
+            c->sort_split = false;
+            if (rc) return rc;
             int trial;
             const int mode = tick_mode(c, &trial);
             if (mode >= PSR_DEP_MULTI && (rc = psr_launch_ray_order(c, t.nray))) return rc;
